@@ -1,0 +1,208 @@
+"""Python host API over the C ABI: the PhysicsEngine of the reference
+(PhysicsEngine.h:30-122) with the state on the GPU.
+
+``Engine`` mirrors the reference's member functions that drive the step:
+
+- ``runSimulation_Classic_DoublePass_MPLocks(dt_total)`` returns 0 while
+  ``time < time_end`` and 1 when done (PhysicsEngine_runSimulation_Classic_
+  DoublePass_MPLocks.cpp:806-814) and calls ``reportConsole`` when the console
+  interval has elapsed (same file :796-800);
+- ``getTime_*`` / ``getCount_*`` getters (PhysicsEngine.h:66-75);
+- ``get_particle`` / ``get_node`` are the lazy device->host mirrors behind
+  ``getMaterialPoints`` / ``getGridPoints`` (PhysicsEngine.h:76-78).
+
+Scenes (``code_mpm_amd.scenes.Scene``) play the part of the reference's
+``initializeWorld_*`` functions.
+"""
+import ctypes
+
+import numpy as np
+
+from . import capi
+from .capi import MON, NF, PF, check, ptr
+
+
+class Engine:
+    """One simulation on one GPU (one HIP stream)."""
+
+    def __init__(self, device=0):
+        self._lib = capi.load()
+        h = ctypes.c_void_p()
+        rc = self._lib.mpm_create(ctypes.byref(h), int(device))
+        check(rc, None, "mpm_create")
+        self._h = h
+        self.n = 0
+        self.nnodes = 0
+        self.console_interval = None
+        self.console_last = -1.0e12
+        self.console_log = []
+        self.scene = None
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mpm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        check(rc, self._h, what)
+
+    # -- set-up -------------------------------------------------------------
+    def set_grid(self, length, cells):
+        L = np.ascontiguousarray(length, dtype=np.float64)
+        C = np.ascontiguousarray(cells, dtype=np.int32)
+        self._check(self._lib.mpm_set_grid(self._h, L.ctypes.data_as(capi.PDBL), C.ctypes.data_as(capi.PI32)),
+                    "set_grid")
+        n = ctypes.c_int64()
+        self._lib.mpm_get_node_count(self._h, ctypes.byref(n))
+        self.nnodes = n.value
+
+    def set_node_boundary(self, fixed=None, friction=None):
+        f = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8)
+        mu = None if friction is None else np.ascontiguousarray(friction, dtype=np.float64)
+        self._check(self._lib.mpm_set_node_boundary(self._h, ptr(f), ptr(mu), self.nnodes), "set_node_boundary")
+
+    def set_materials(self, materials):
+        arr = (capi.Material * max(1, len(materials)))(*[capi.make_material(m) for m in materials])
+        self._check(self._lib.mpm_set_materials(self._h, ctypes.cast(arr, ctypes.c_void_p), len(materials)),
+                    "set_materials")
+
+    def set_particle_count(self, n):
+        self._check(self._lib.mpm_set_particle_count(self._h, int(n)), "set_particle_count")
+        self.n = int(n)
+
+    def set_particle(self, name, values):
+        fid, nc = PF[name]
+        a = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(nc, self.n))
+        self._check(self._lib.mpm_set_particle_field(self._h, fid, ptr(a), self.n), f"set_particle {name}")
+
+    def set_particle_material(self, mat):
+        a = np.ascontiguousarray(mat, dtype=np.int32)
+        self._check(self._lib.mpm_set_particle_material(self._h, ptr(a), self.n), "set_particle_material")
+
+    def set_displacement_control(self, index, axes, multiplier):
+        i = np.ascontiguousarray(index, dtype=np.int64)
+        a = np.ascontiguousarray(axes, dtype=np.uint8)
+        m = np.ascontiguousarray(multiplier, dtype=np.float32)
+        self._check(self._lib.mpm_set_displacement_control(self._h, ptr(i), ptr(a), ptr(m), len(i)),
+                    "set_displacement_control")
+
+    def set_monitor_list(self, kind, index):
+        i = np.ascontiguousarray(index, dtype=np.int64)
+        self._check(self._lib.mpm_set_monitor_list(self._h, MON[kind], ptr(i), len(i)), "set_monitor_list")
+
+    def set_timeline(self, t, v):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+        self._check(self._lib.mpm_set_timeline(self._h, ptr(t), ptr(v), len(t)), "set_timeline")
+
+    def set_time(self, time, dt_max, time_end):
+        self._check(self._lib.mpm_set_time(self._h, float(time), float(dt_max), float(time_end)), "set_time")
+
+    def set_damping(self, damping, mass_min=0.0):
+        self._check(self._lib.mpm_set_damping(self._h, float(damping), float(mass_min)), "set_damping")
+
+    def load_scene(self, scene):
+        """Upload a Scene (the equivalent of an initializeWorld_* call)."""
+        scene.apply(self)
+        self.scene = scene
+        self.console_interval = scene.console_interval
+        self.console_last = 0.0
+
+    # -- the step -------------------------------------------------------------
+    def step(self, dt, nsteps=1):
+        self._check(self._lib.mpm_step(self._h, float(dt), int(nsteps)), "step")
+
+    def run(self, dt_total):
+        fin = ctypes.c_int32()
+        self._check(self._lib.mpm_run(self._h, float(dt_total), ctypes.byref(fin)), "run")
+        return fin.value
+
+    def synchronize(self):
+        self._check(self._lib.mpm_synchronize(self._h), "synchronize")
+
+    def runSimulation_Classic_DoublePass_MPLocks(self, dt_total):
+        """The reference's step loop with its console report
+        (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:439-815)."""
+        acc = 0.0
+        dt_max = self.scene.dt if self.scene is not None else None
+        time_end = self.scene.time_end if self.scene is not None else float("inf")
+        while acc < dt_total:
+            dt = dt_max
+            if dt_max > (dt_total - acc):
+                dt = dt_total - acc
+            acc += dt
+            t_before = self.getTime_Current()
+            # the console check of the reference sees d_Time before the increment
+            self.step(dt, 1)
+            if self.console_interval is not None and t_before - self.console_last > self.console_interval:
+                self.console_last = t_before
+                self.reportConsole(time_override=t_before)
+        if self.getTime_Current() < time_end:
+            return 0
+        self.console_last = self.getTime_Current()
+        self.reportConsole()
+        return 1
+
+    def reportConsole(self, time_override=None):
+        m = self.monitors()
+        if time_override is not None:
+            m["time"] = time_override
+        self.console_log.append(m)
+        return m
+
+    # -- getters -------------------------------------------------------------
+    def getTime_Current(self):
+        t = ctypes.c_double()
+        c = ctypes.c_int64()
+        self._lib.mpm_get_time(self._h, ctypes.byref(t), ctypes.byref(c))
+        return t.value
+
+    def getTime_Cycle(self):
+        t = ctypes.c_double()
+        c = ctypes.c_int64()
+        self._lib.mpm_get_time(self._h, ctypes.byref(t), ctypes.byref(c))
+        return c.value
+
+    def getCount_MaterialPoint(self):
+        return self.n
+
+    def getCount_GridPoint(self):
+        return self.nnodes
+
+    def get_particle(self, name):
+        fid, nc = PF[name]
+        out = np.empty((nc, self.n))
+        self._check(self._lib.mpm_get_particle_field(self._h, fid, ptr(out), self.n), f"get_particle {name}")
+        return out
+
+    def get_node(self, name):
+        fid, nc = NF[name]
+        out = np.empty((nc, self.nnodes))
+        self._check(self._lib.mpm_get_node_field(self._h, fid, ptr(out), self.nnodes), f"get_node {name}")
+        return out
+
+    def monitors(self):
+        m = capi.Monitors()
+        self._check(self._lib.mpm_get_monitors(self._h, ctypes.byref(m)), "monitors")
+        return m.as_dict()
+
+    def health(self):
+        h = capi.Health()
+        self._check(self._lib.mpm_get_health(self._h, ctypes.byref(h)), "health")
+        return h.as_dict()
+
+    def reset_health(self):
+        self._check(self._lib.mpm_reset_health(self._h), "reset_health")

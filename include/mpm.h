@@ -1,0 +1,235 @@
+/*
+ * mpm.h — C ABI of the MI355X-native Classic MPM timestep (libmpm_amd.so).
+ *
+ * One opaque engine per simulation.  All state lives in device memory as
+ * structure-of-arrays FP64; the host passes plain pointers and sizes.  Every
+ * entry point returns MPM_OK (0) or a negative MPM_ERR_* code and never calls
+ * exit(); mpm_last_error() gives the message of the last failure.
+ *
+ * Each declaration names the reference interface it replaces (paths are in
+ * Sina-mon/code_MPM).  Layouts:
+ *   - particle field arrays are component-major: component c of particle i is
+ *     src[c * n + i];
+ *   - the deformation gradient has 9 components in glm column-major order,
+ *     component (3 * col + row) = F[col][row];
+ *   - Voigt order is [xx, yy, zz, xy, yz, zx]; strains carry engineering
+ *     shears (gamma = 2 eps), stresses tensor shears;
+ *   - nodes are indexed ix + iy * nx + iz * nx * ny with n = cells + 1 per
+ *     axis (GridPoint_Factory.h:18-22), node position (i * L) / cells.
+ *
+ * Threading: an engine is not re-entrant (neither is the reference's
+ * PhysicsEngine, PhysicsEngine.h:53); different engines may be driven from
+ * different host threads.  Each engine owns one HIP stream.
+ */
+#ifndef MPM_AMD_MPM_H
+#define MPM_AMD_MPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mpm_engine mpm_engine;
+
+enum {
+    MPM_OK = 0,
+    MPM_ERR_ARG = -1,     /* bad argument (null pointer, size mismatch, ...) */
+    MPM_ERR_HIP = -2,     /* a HIP runtime call failed                       */
+    MPM_ERR_STATE = -3,   /* call out of order (e.g. step before set_grid)   */
+    MPM_ERR_NOMEM = -4,   /* device allocation failed                        */
+    MPM_ERR_RANGE = -5    /* size beyond what the engine supports            */
+};
+
+/* Material type codes, Material_BC.h:6-11. Unknown codes use the
+ * perfectly-plastic map, as PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:238-239. */
+enum {
+    MPM_MAT_ELASTIC = 0,
+    MPM_MAT_NEOHOOKEAN = 1,
+    MPM_MAT_VISCOELASTIC = 2,
+    MPM_MAT_PLASTIC = 3,
+    MPM_MAT_VONMISESHARDENING = 4,
+    MPM_MAT_GASS = 5
+};
+
+/* Material_BC (Material_BC.h:13-35). */
+typedef struct mpm_material {
+    int32_t type;
+    int32_t id;
+    double density;
+    double elastic_modulus;
+    double poisson_ratio;
+    double yield_stress;
+    double viscosity;
+    double hardening_c0;
+    double hardening_c1;
+} mpm_material;
+
+/* Particle fields (MaterialPoint_BC, MaterialPoint_BC.h:25-85), FP64. */
+enum {
+    MPM_PF_POSITION = 0,            /* 3 */
+    MPM_PF_VELOCITY = 1,            /* 3 */
+    MPM_PF_FORCE_EXTERNAL = 2,      /* 3 */
+    MPM_PF_MASS = 3,                /* 1 */
+    MPM_PF_VOLUME = 4,              /* 1 */
+    MPM_PF_VOLUME_INITIAL = 5,      /* 1 */
+    MPM_PF_DEFORMATION_GRADIENT = 6,/* 9 */
+    MPM_PF_STRAIN = 7,              /* 6 */
+    MPM_PF_STRESS = 8,              /* 6 */
+    MPM_PF_STRAIN_PLASTIC = 9,      /* 6 */
+    MPM_PF_BACKSTRESS = 10,         /* 1  d_BackStress_Isotropic */
+    MPM_PF_ENERGY_STRAIN = 11,      /* 1 */
+    MPM_PF_ENERGY_STRAIN_MAX = 12,  /* 1  d_Energy_Strain_MaxHistory */
+    MPM_PF_ENERGY_STRAIN_TI = 13,   /* 1  d_Energy_Strain_TimeIntegral */
+    MPM_PF_ENERGY_PLASTIC = 14,     /* 1 */
+    MPM_PF_COUNT = 15
+};
+
+/* Node fields (GridPoint, GridPoint.h:6-45), FP64, state after the last step.
+ * Nodes no particle touched in the last step read as zero, as after
+ * runSimulation_ResetGrid (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:4-22). */
+enum {
+    MPM_NF_MASS = 0,        /* 1 */
+    MPM_NF_MOMENTUM = 1,    /* 3 */
+    MPM_NF_VELOCITY = 2,    /* 3 */
+    MPM_NF_FORCE = 3,       /* 3 */
+    MPM_NF_FORCE_TEMP = 4,  /* 3  support reaction; only y is ever non-zero */
+    MPM_NF_ACTIVE = 5,      /* 1  1.0 if touched in the last step */
+    MPM_NF_COUNT = 6
+};
+
+/* Monitor lists of PhysicsEngine (PhysicsEngine.h:92-98). */
+enum {
+    MPM_MON_DISPLACEMENT = 0, /* v_MarkedMaterialPoints_Displacement_Monitor */
+    MPM_MON_MOMENTUM = 1,     /* v_MarkedMaterialPoints_Momentum */
+    MPM_MON_ENERGY = 2,       /* v_MarkedMaterialPoints_Monitor_Energy */
+    MPM_MON_STRESS = 3,       /* v_MarkedMaterialPoints_Stress_Monitor */
+    MPM_MON_COUNT = 4
+};
+
+/* Step variant. */
+enum {
+    MPM_VARIANT_DOUBLEPASS = 0   /* runSimulation_Classic_DoublePass_MPLocks */
+};
+
+/* reportConsole columns (PhysicsEngine.cpp:40-183) without the principal
+ * monitor.  A column whose list is empty reports has_* = 0. */
+typedef struct mpm_monitors {
+    double time;
+    double position_y;      /* mean platen y, PhysicsEngine.cpp:48-59 */
+    double force_y;         /* sum of Force_Temp.y, PhysicsEngine.cpp:61-70 */
+    double momentum_y;      /* PhysicsEngine.cpp:72-80 */
+    double kinetic_energy;  /* PhysicsEngine.cpp:140-156 */
+    double strain_energy;
+    double plastic_energy;
+    double strain_y;        /* mean over the stress monitor, PhysicsEngine.cpp:122-138 */
+    double stress_y;
+    int32_t has_position;
+    int32_t has_momentum;
+    int32_t has_energy;
+    int32_t has_stress;
+} mpm_monitors;
+
+/* Per-engine health counters, cumulative since the last reset.  The
+ * reference checks none of these; a null material prints and skips the
+ * particle (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:226-230). */
+typedef struct mpm_health {
+    int64_t steps;
+    int64_t nonfinite_particles;  /* particle-steps ending with a non-finite position or stress */
+    int64_t outside_grid;         /* particle-steps whose adjacent nodes all lie off the grid */
+    int64_t null_material;        /* particle-steps skipped for a missing material */
+    int64_t plastic_points;       /* particle-steps whose return map yielded */
+} mpm_health;
+
+/* --- lifetime ------------------------------------------------------------ */
+
+/* PhysicsEngine::PhysicsEngine() (PhysicsEngine.h:33). device = HIP ordinal. */
+int mpm_create(mpm_engine **out, int32_t device);
+/* PhysicsEngine::~PhysicsEngine() (PhysicsEngine.cpp:4-38). */
+int mpm_destroy(mpm_engine *e);
+/* Message of the last failing call on e (or of the last failing mpm_create
+ * when e is NULL).  Never NULL. */
+const char *mpm_last_error(const mpm_engine *e);
+
+/* --- world set-up (replaces the scene code's direct member writes) -------- */
+
+/* GridPoint_Factory::createGrid (GridPoint_Factory.h:33-68) plus the
+ * mediator's d3_Length_Cell / i3_Node_Count (GridPoint_Mediator.h:25-28).
+ * Resets all node state and boundary conditions. */
+int mpm_set_grid(mpm_engine *e, const double length[3], const int32_t cells[3]);
+/* GridPoint::b3_Fixed (bit0 x, bit1 y, bit2 z) and d_FrictionCoefficient
+ * (GridPoint.h:20-23) for every node; either pointer may be NULL (all zero). */
+int mpm_set_node_boundary(mpm_engine *e, const uint8_t *fixed_mask,
+                          const double *friction, int64_t n_nodes);
+/* PhysicsEngine::v_allMaterial (PhysicsEngine.h:90). */
+int mpm_set_materials(mpm_engine *e, const mpm_material *materials, int32_t n);
+/* Resizes the particle set; every particle gets the MaterialPoint_BC
+ * defaults (F = I, everything else 0, multiplier 1, no material). */
+int mpm_set_particle_count(mpm_engine *e, int64_t n);
+int mpm_get_particle_count(const mpm_engine *e, int64_t *n);
+/* Host -> device copy of one particle field (n must equal the particle count). */
+int mpm_set_particle_field(mpm_engine *e, int32_t field, const double *src, int64_t n);
+/* Device -> host copy; the lazy mirror behind PhysicsEngine::getMaterialPoints (PhysicsEngine.h:76). */
+int mpm_get_particle_field(mpm_engine *e, int32_t field, double *dst, int64_t n);
+/* MaterialPoint_BC::p_Material as an index into the material table; -1 = NULL. */
+int mpm_set_particle_material(mpm_engine *e, const int32_t *material, int64_t n);
+/* v_MarkedMaterialPoints_Displacement_Control (PhysicsEngine.h:93) in list
+ * order, with b3_DisplacementControl (bit0 x, bit1 y, bit2 z) and
+ * f_DisplacementControl_Multiplier of each listed particle.  When two
+ * listed particles write the same node the later one in the list wins, as in
+ * runSimulation_DisplacementControl run by one thread. */
+int mpm_set_displacement_control(mpm_engine *e, const int64_t *index, const uint8_t *axes,
+                                 const float *multiplier, int64_t count);
+/* The monitor lists of PhysicsEngine.h:92-98 (kind = MPM_MON_*), in list order. */
+int mpm_set_monitor_list(mpm_engine *e, int32_t kind, const int64_t *index, int64_t count);
+/* TimeLine::addTimePoint sequence (TimeLine.h:27-31); v3 = n x (vx, vy, vz). */
+int mpm_set_timeline(mpm_engine *e, const double *time, const double *v3, int32_t n);
+/* d_Time, d_TimeIncrement_Maximum, d_TimeEnd (PhysicsEngine.h:110-113). */
+int mpm_set_time(mpm_engine *e, double time, double dt_max, double time_end);
+/* d_DampingCoefficient and d_Mass_Minimum (PhysicsEngine.h:83-84). */
+int mpm_set_damping(mpm_engine *e, double damping, double mass_minimum);
+
+/* --- the hot path ---------------------------------------------------------- */
+
+/* nsteps steps of size dt, each the body of the while loop of
+ * runSimulation_Classic_DoublePass_MPLocks
+ * (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:462-803):
+ * time += dt and cycle += 1 after each. Asynchronous on the engine's stream. */
+int mpm_step(mpm_engine *e, double dt, int32_t nsteps);
+/* runSimulation_Classic_DoublePass_MPLocks(dTimeIncrement_Total)
+ * (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:439-815)
+ * without the console report: steps of d_TimeIncrement_Maximum, the last one
+ * clamped to the remainder; *finished = 0 while time < time_end, else 1. */
+int mpm_run(mpm_engine *e, double dt_total, int32_t *finished);
+/* Waits for the engine's stream. */
+int mpm_synchronize(mpm_engine *e);
+
+/* --- observation ------------------------------------------------------------ */
+
+/* getTime_Current and the cycle counter i_TimeCycle (PhysicsEngine.h:67,113). */
+int mpm_get_time(const mpm_engine *e, double *time, int64_t *cycle);
+int mpm_get_node_count(const mpm_engine *e, int64_t *n);
+int mpm_get_node_field(mpm_engine *e, int32_t field, double *dst, int64_t n);
+/* reportConsole values (PhysicsEngine.cpp:40-183), reduced on the device in
+ * the reference's summation order. */
+int mpm_get_monitors(mpm_engine *e, mpm_monitors *out);
+int mpm_get_health(mpm_engine *e, mpm_health *out);
+int mpm_reset_health(mpm_engine *e);
+
+/* --- stand-alone constitutive update ----------------------------------------- */
+
+/* ConstitutiveRelation::calculateIncrement_{Elastic,Plastic,VonMisesHardening}
+ * (ConstitutiveRelation.h:31-34; ConstitutiveRelation_calculateIncrement_*.cpp)
+ * for n independent points on device `device`, host arrays in and out:
+ * stress / dstrain / dstress / dstrain_plastic are n x 6 row-major,
+ * backstress / dbackstress are n.  The material type selects the map as the
+ * step does. */
+int mpm_constitutive_batch(int32_t device, const mpm_material *material, const double *backstress,
+                           const double *stress, const double *dstrain, double *dstress,
+                           double *dstrain_plastic, double *dbackstress, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPM_AMD_MPM_H */
