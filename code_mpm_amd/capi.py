@@ -25,6 +25,7 @@ PF = dict(
 )
 NF = dict(MASS=(0, 1), MOMENTUM=(1, 3), VELOCITY=(2, 3), FORCE=(3, 3), FORCE_TEMP=(4, 3), ACTIVE=(5, 1))
 MON = dict(DISPLACEMENT=0, MOMENTUM=1, ENERGY=2, STRESS=3)
+PHASES = ("bin", "p2g_grid", "g2p_velocity", "remap_grid", "g2p_strain")
 
 
 class Material(ctypes.Structure):
@@ -69,6 +70,8 @@ class Health(ctypes.Structure):
         ("outside_grid", ctypes.c_int64),
         ("null_material", ctypes.c_int64),
         ("plastic_points", ctypes.c_int64),
+        ("surrogate_calls", ctypes.c_int64),
+        ("surrogate_fallbacks", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -103,6 +106,7 @@ SIGNATURES = {
     "mpm_set_damping": (I32, [P, DBL, DBL]),
     "mpm_step": (I32, [P, DBL, I32]),
     "mpm_run": (I32, [P, DBL, PI32]),
+    "mpm_step_timed": (I32, [P, DBL, I32, PDBL]),
     "mpm_synchronize": (I32, [P]),
     "mpm_get_time": (I32, [P, PDBL, PI64]),
     "mpm_get_node_count": (I32, [P, PI64]),
@@ -111,6 +115,19 @@ SIGNATURES = {
     "mpm_get_health": (I32, [P, ctypes.POINTER(Health)]),
     "mpm_reset_health": (I32, [P]),
     "mpm_constitutive_batch": (I32, [I32, ctypes.POINTER(Material), P, P, P, P, P, P, I64]),
+    "mpm_surrogate_param_count": (I64, []),
+    "mpm_surrogate_create": (I32, [ctypes.POINTER(P), I32, ctypes.c_uint64]),
+    "mpm_surrogate_destroy": (I32, [P]),
+    "mpm_surrogate_set_params": (I32, [P, P, I64]),
+    "mpm_surrogate_get_params": (I32, [P, P, I64]),
+    "mpm_surrogate_set_normalization": (I32, [P, DBL, DBL, P, P, P, P, P, P]),
+    "mpm_surrogate_forward": (I32, [P, P, P, I64]),
+    "mpm_surrogate_gradient": (I32, [P, P, P, I64, P, P]),
+    "mpm_surrogate_load_training_set": (I32, [P, P, P, I64]),
+    "mpm_surrogate_train": (I32, [P, I32, I32, ctypes.c_float, ctypes.c_uint64, P]),
+    "mpm_set_surrogate": (I32, [P, P, I32, DBL]),
+    "mpm_set_sample_capture": (I32, [P, I64]),
+    "mpm_get_captured_samples": (I32, [P, P, I64, PI64]),
 }
 
 _lib = None

@@ -18,6 +18,7 @@
 
 #include "../../include/mpm.h"
 #include "mpm_kernels.h"
+#include "mpm_surrogate.h"
 
 namespace {
 
@@ -88,6 +89,54 @@ struct mpm_engine {
     unsigned long long *d_health = nullptr;
     double *d_mon_out = nullptr;
     int64_t steps = 0;
+
+    /* surrogate and sample capture */
+    mpm_surrogate *sur = nullptr;
+    int sur_material = -1;
+    double sur_tol = 1.0e-4;
+    uint32_t *qcount = nullptr, *qid = nullptr;
+    double *qde = nullptr;
+    float *qx = nullptr, *qy = nullptr;
+    uint8_t *qenv = nullptr;
+    int64_t qcap = 0;
+    double *cap = nullptr;
+    unsigned long long *capcount = nullptr;
+    int64_t capcap = 0;
+
+    void free_queue() {
+        dfree(qid); dfree(qde); dfree(qx); dfree(qy); dfree(qenv);
+        qcap = 0;
+    }
+    int ensure_queue() {
+        if (qcap == n && qid) return MPM_OK;
+        free_queue();
+        if (n == 0) return MPM_OK;
+        const size_t sn = (size_t)n;
+        int rc;
+        if ((rc = hip(hipMalloc(&qid, 4 * sn), "alloc queue")) || (rc = hip(hipMalloc(&qde, 48 * sn), "alloc queue")) ||
+            (rc = hip(hipMalloc(&qx, 4 * MLP_IN * sn), "alloc queue")) ||
+            (rc = hip(hipMalloc(&qy, 4 * MLP_OUT * sn), "alloc queue")) ||
+            (rc = hip(hipMalloc(&qenv, sn), "alloc queue"))) {
+            free_queue();
+            return rc;
+        }
+        qcap = n;
+        return MPM_OK;
+    }
+    SurrogateArgs sargs() const {
+        SurrogateArgs a{};
+        a.mode = sur ? MODE_SURROGATE : (cap ? MODE_CAPTURE : MODE_EXACT);
+        a.material = sur_material;
+        a.tol = sur_tol;
+        if (sur) {
+            a.stress_scale = sur->stress_scale;
+            a.strain_scale = sur->strain_scale;
+            a.norm = sur->norm;
+        }
+        a.qcount = qcount; a.qid = qid; a.qde = qde; a.qx = qx; a.qy = qy; a.qenv = qenv;
+        a.cap = cap; a.capcount = capcount; a.capcap = capcap;
+        return a;
+    }
 
     int fail(int code, const char *fmt, ...) {
         char buf[512];
@@ -167,7 +216,9 @@ struct mpm_engine {
         return MPM_OK;
     }
 
-    int step_once(double dt) {
+    /* One step; with ev != nullptr an event is recorded before every phase
+     * and after the last (ev[0..PHASE_COUNT]). */
+    int step_once(double dt, hipEvent_t *ev = nullptr) {
         StepParams sp;
         sp.dt = dt;
         sp.mass_min = mass_min;
@@ -177,10 +228,25 @@ struct mpm_engine {
         sp.timeline_nonempty = timeline.t.empty() ? 0 : 1;
         int rc = ensure_keys();
         if (rc) return rc;
+        if (ev && (rc = hip(hipEventRecord(ev[0], stream), "event"))) return rc;
         rc = hip(launch_bin(g, pview(), bview(), bblock, stream), "cell binning");
         if (rc) return rc;
-        rc = hip(launch_step_kernels(g, pview(), nview(), bview(), sp, mt, d_health, stream), "step kernels");
-        if (rc) return rc;
+        const SurrogateArgs sa = sargs();
+        if (sur && (rc = ensure_queue())) return rc;
+        for (int ph = PHASE_P2G_GRID; ph < PHASE_COUNT; ph++) {
+            if (ev && (rc = hip(hipEventRecord(ev[ph], stream), "event"))) return rc;
+            if (ph == PHASE_G2P_STRAIN && sur && (rc = hip(launch_zero_u32(qcount, 1, stream), "queue reset")))
+                return rc;
+            rc = hip(launch_phase(ph, g, pview(), nview(), bview(), sp, mt, d_health, sa, stream), "step kernels");
+            if (rc) return rc;
+        }
+        if (sur) {
+            /* the network on the queued points, then acceptance / fallback */
+            rc = hip(mlp_launch_forward(sur->params, sur->wbf16, qx, qy, n, stream, qcount), "surrogate forward");
+            if (!rc) rc = hip(launch_surrogate_post(pview(), sp, mt, d_health, sa, stream), "surrogate post");
+            if (rc) return rc;
+        }
+        if (ev && (rc = hip(hipEventRecord(ev[PHASE_COUNT], stream), "event"))) return rc;
         keys_valid = true; /* g2p_strain wrote the next keys and counts */
         time += dt;
         cycle++;
@@ -219,6 +285,8 @@ int mpm_create(mpm_engine **out, int32_t device) {
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&e->d_health, sizeof(unsigned long long) * HEALTH_SLOTS) != hipSuccess ||
         hipMalloc(&e->d_mon_out, sizeof(double) * 8) != hipSuccess ||
+        hipMalloc(&e->qcount, sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&e->capcount, sizeof(unsigned long long)) != hipSuccess ||
         hipMemset(e->d_health, 0, sizeof(unsigned long long) * HEALTH_SLOTS) != hipSuccess) {
         g_create_error = "stream or buffer creation failed";
         mpm_destroy(e);
@@ -236,6 +304,10 @@ int mpm_destroy(mpm_engine *e) {
     e->free_particles();
     dfree(e->d_health);
     dfree(e->d_mon_out);
+    e->free_queue();
+    dfree(e->qcount);
+    dfree(e->cap);
+    dfree(e->capcount);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return MPM_OK;
@@ -504,6 +576,29 @@ int mpm_step(mpm_engine *e, double dt, int32_t nsteps) {
     return MPM_OK;
 }
 
+int mpm_step_timed(mpm_engine *e, double dt, int32_t nsteps, double phase_ms[MPM_PHASE_COUNT]) {
+    CHECK_E(e);
+    if (!e->grid_set) return e->fail(MPM_ERR_STATE, "step before set_grid");
+    if (nsteps < 0 || !phase_ms) return e->fail(MPM_ERR_ARG, "bad timed-step arguments");
+    static_assert(MPM_PHASE_COUNT == PHASE_COUNT, "phase enums disagree");
+    for (int k = 0; k < MPM_PHASE_COUNT; k++) phase_ms[k] = 0.0;
+    hipEvent_t ev[PHASE_COUNT + 1] = {};
+    int rc = MPM_OK;
+    for (int k = 0; k <= PHASE_COUNT && !rc; k++) rc = e->hip(hipEventCreate(&ev[k]), "event create");
+    for (int s = 0; s < nsteps && !rc; s++) {
+        rc = e->step_once(dt, ev);
+        if (!rc) rc = e->hip(hipEventSynchronize(ev[PHASE_COUNT]), "event sync");
+        for (int k = 0; k < PHASE_COUNT && !rc; k++) {
+            float ms = 0.f;
+            rc = e->hip(hipEventElapsedTime(&ms, ev[k], ev[k + 1]), "event time");
+            phase_ms[k] += ms;
+        }
+    }
+    for (int k = 0; k <= PHASE_COUNT; k++)
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
+    return rc;
+}
+
 int mpm_run(mpm_engine *e, double dt_total, int32_t *finished) {
     CHECK_E(e);
     if (!e->grid_set) return e->fail(MPM_ERR_STATE, "run before set_grid");
@@ -604,7 +699,51 @@ int mpm_get_health(mpm_engine *e, mpm_health *out) {
     out->null_material = (int64_t)h[HEALTH_NULL];
     out->plastic_points = (int64_t)h[HEALTH_PLASTIC];
     out->nonfinite_particles = (int64_t)h[HEALTH_NONFINITE];
+    out->surrogate_calls = (int64_t)h[HEALTH_SURROGATE_CALLS];
+    out->surrogate_fallbacks = (int64_t)h[HEALTH_SURROGATE_FALLBACKS];
     return MPM_OK;
+}
+
+int mpm_set_surrogate(mpm_engine *e, mpm_surrogate *s, int32_t material, double residual_tolerance) {
+    CHECK_E(e);
+    if (s && s->device != e->device) return e->fail(MPM_ERR_ARG, "surrogate lives on another device");
+    if (s && (material < 0 || material >= e->mt.n)) return e->fail(MPM_ERR_ARG, "surrogate material out of range");
+    if (s && !(residual_tolerance >= 0.0)) return e->fail(MPM_ERR_ARG, "negative residual tolerance");
+    (void)hipStreamSynchronize(e->stream);
+    if (s) (void)hipStreamSynchronize(s->stream);
+    e->sur = s;
+    e->sur_material = s ? material : -1;
+    e->sur_tol = residual_tolerance;
+    if (!s) e->free_queue();
+    return MPM_OK;
+}
+
+int mpm_set_sample_capture(mpm_engine *e, int64_t capacity) {
+    CHECK_E(e);
+    if (capacity < 0) return e->fail(MPM_ERR_ARG, "negative capture capacity");
+    (void)hipStreamSynchronize(e->stream);
+    dfree(e->cap);
+    e->capcap = 0;
+    int rc = e->hip(hipMemset(e->capcount, 0, sizeof(unsigned long long)), "capture reset");
+    if (rc || capacity == 0) return rc;
+    if ((rc = e->hip(hipMalloc(&e->cap, sizeof(double) * 14 * (size_t)capacity), "alloc capture"))) return rc;
+    e->capcap = capacity;
+    return MPM_OK;
+}
+
+int mpm_get_captured_samples(mpm_engine *e, double *rows, int64_t capacity, int64_t *count) {
+    CHECK_E(e);
+    if (!count) return e->fail(MPM_ERR_ARG, "null count");
+    unsigned long long c = 0;
+    int rc = e->hip(hipStreamSynchronize(e->stream), "capture sync");
+    if (!rc) rc = e->hip(hipMemcpy(&c, e->capcount, sizeof(c), hipMemcpyDeviceToHost), "capture count");
+    if (rc) return rc;
+    int64_t have = std::min<int64_t>((int64_t)c, e->capcap);
+    *count = (int64_t)c;
+    if (rows && capacity > 0 && have > 0)
+        rc = e->hip(hipMemcpy(rows, e->cap, sizeof(double) * 14 * (size_t)std::min(have, capacity), hipMemcpyDeviceToHost),
+                    "capture copy");
+    return rc;
 }
 
 int mpm_reset_health(mpm_engine *e) {

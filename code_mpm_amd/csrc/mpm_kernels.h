@@ -8,13 +8,37 @@
 #include <stdint.h>
 
 #include "mpm_device.h"
+#include "mpm_mlp.h"
 
 /* Node tile of one 256-thread block: 8 x 8 x 4 nodes, x fastest. */
 #define TILE_X 8
 #define TILE_Y 8
 #define TILE_Z 4
 
-enum { HEALTH_OUTSIDE = 0, HEALTH_NULL = 1, HEALTH_PLASTIC = 2, HEALTH_NONFINITE = 3, HEALTH_SLOTS = 4 };
+enum { HEALTH_OUTSIDE = 0, HEALTH_NULL = 1, HEALTH_PLASTIC = 2, HEALTH_NONFINITE = 3, HEALTH_SURROGATE_CALLS = 4,
+       HEALTH_SURROGATE_FALLBACKS = 5, HEALTH_SLOTS = 6 };
+
+enum { MODE_EXACT = 0, MODE_CAPTURE = 1, MODE_SURROGATE = 2 };
+
+/* Return-map mode of the G2P pass 2 kernel, with the surrogate's queue and
+ * the sample-capture buffer. */
+struct SurrogateArgs {
+    int mode;
+    int material;               /* material index the network serves */
+    double stress_scale;        /* sigma_y of that material */
+    double strain_scale;        /* sigma_y / E */
+    double tol;                 /* accepted yield residual, in units of sigma_y */
+    MLPNorm norm;
+    uint32_t *qcount;           /* queue length (device) */
+    uint32_t *qid;              /* particle of each queue slot */
+    double *qde;                /* 6 x capacity strain increments */
+    float *qx;                  /* capacity x 13 normalised inputs */
+    float *qy;                  /* capacity x 7 normalised outputs */
+    uint8_t *qenv;              /* inputs inside the training envelope */
+    double *cap;                /* capture rows: sigma6 de6 R material */
+    unsigned long long *capcount;
+    int64_t capcap;
+};
 
 /* Particle state, structure of arrays; vector/tensor fields are
  * component-major (component c of particle i at [c * n + i]). */
@@ -50,13 +74,29 @@ struct BView {
     int64_t necells;
 };
 
-struct StepParams {
-    double dt;
-    double mass_min;
-    double damping;
+/* One instance of a batch (a single scene is a batch of one): geometry,
+ * where its nodes / cells / tiles / particles start in the shared arrays,
+ * and its step parameters.  time and vt live on the device so that steps
+ * can be replayed from a graph without host writes: k_prepare sets vt from
+ * the instance's timeline at step start, k_advance adds dt at step end. */
+struct Inst {
+    GridGeom g;
+    int64_t node_off, cell_off, part_off, npart;
+    uint32_t tile_off, tiles;
+    double dt, damping, mass_min, time;
     double vt[3];            /* TimeLine::getVelocity(time at step start) */
-    int has_fext;
-    int timeline_nonempty;
+    int32_t tl_off, tl_count;
+    int32_t has_fext, pad;
+};
+
+struct Batch {
+    Inst *inst;               /* device table, ninst entries */
+    const uint32_t *tile_off; /* ninst + 1 prefix of node tiles */
+    const uint32_t *pinst;    /* instance of each particle; null when ninst == 1 */
+    const double *tl;         /* timeline points (t, vx, vy, vz) of all instances */
+    int ninst;
+    uint32_t ntiles;
+    int64_t nnodes;           /* all instances: component stride of node arrays */
 };
 
 struct MonLists {
@@ -66,13 +106,22 @@ struct MonLists {
     int64_t nfy;
 };
 
-hipError_t launch_keys(const GridGeom &g, const PView &P, const BView &B, hipStream_t s);
+hipError_t launch_keys(const Batch &bt, const PView &P, const BView &B, hipStream_t s);
 int64_t scan_block_count(int64_t necells);
-hipError_t launch_bin(const GridGeom &g, const PView &P, const BView &B, uint32_t *block_sum, hipStream_t s);
-hipError_t launch_step_kernels(const GridGeom &g, const PView &P, const NView &N, const BView &B,
-                               const StepParams &sp, const MatTable &mt, unsigned long long *health,
-                               hipStream_t s);
-hipError_t launch_monitors(const GridGeom &g, const PView &P, const NView &N, const MonLists &ml, double *out,
+hipError_t launch_bin(const PView &P, const BView &B, uint32_t *block_sum, hipStream_t s);
+/* Step phases after binning, in launch order (PHASE_BIN is launch_bin). */
+enum { PHASE_BIN = 0, PHASE_P2G_GRID = 1, PHASE_G2P_VELOCITY = 2, PHASE_REMAP_GRID = 3, PHASE_G2P_STRAIN = 4,
+       PHASE_COUNT = 5 };
+hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView &N, const BView &B,
+                        const MatTable &mt, unsigned long long *health, const SurrogateArgs &sg, hipStream_t s);
+hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t s);
+hipError_t launch_surrogate_post(const Batch &bt, const PView &P, const MatTable &mt, unsigned long long *health,
+                                 const SurrogateArgs &sg, hipStream_t s);
+/* step bracket: vt from each instance's timeline; time += dt; dt of one instance */
+hipError_t launch_prepare(const Batch &bt, hipStream_t s);
+hipError_t launch_advance(const Batch &bt, hipStream_t s);
+hipError_t launch_set_dt(const Batch &bt, int inst, double dt, hipStream_t s);
+hipError_t launch_monitors(const PView &P, const NView &N, int64_t nnodes, const MonLists &ml, double *out,
                            hipStream_t s);
 hipError_t launch_node_export(const NView &N, int field, int64_t nn, double *dst, hipStream_t s);
 hipError_t launch_constitutive(const MatTable &mt, const double *R, const double *sig, const double *de,

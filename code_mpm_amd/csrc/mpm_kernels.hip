@@ -28,6 +28,8 @@
  */
 #include "mpm_kernels.h"
 
+#include <algorithm>
+
 #define BLOCK 256
 
 /* Blocks are dealt round-robin to the 8 XCDs; give each XCD one contiguous
@@ -36,6 +38,57 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nblk) {
     uint32_t q = nblk / 8u, r = nblk % 8u;
     uint32_t x = bid % 8u, k = bid / 8u;
     return (x < r) ? x * (q + 1u) + k : r * (q + 1u) + (x - r) * q + k;
+}
+
+__device__ __forceinline__ uint32_t inst_of_particle(const Batch &bt, uint32_t i) {
+    return bt.pinst ? bt.pinst[i] : 0u;
+}
+
+/* instance owning global node tile t: last k with tile_off[k] <= t */
+__device__ __forceinline__ int inst_of_tile(const Batch &bt, uint32_t t) {
+    int lo = 0, hi = bt.ninst - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (bt.tile_off[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+/* TimeLine::getVelocity (TimeLine.h:33-46): the last interval [t_i, t_i+1)
+ * containing the time wins; zero outside every interval. */
+__device__ __forceinline__ void timeline_velocity(const double *tl, int off, int count, double time, double out[3]) {
+    out[0] = out[1] = out[2] = 0.0;
+    for (int i = 0; i + 1 < count; i++) {
+        const double *a = tl + 4 * (off + i), *b = a + 4;
+        if (a[0] <= time && time < b[0]) {
+            double s = (time - a[0]) / (b[0] - a[0]);
+            for (int k = 0; k < 3; k++) out[k] = (1.0 - s) * a[1 + k] + (s)*b[1 + k];
+        }
+    }
+}
+
+__global__ void k_prepare(Batch bt) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= bt.ninst) return;
+    Inst &I = bt.inst[k];
+    double vt[3];
+    timeline_velocity(bt.tl, I.tl_off, I.tl_count, I.time, vt);
+    I.vt[0] = vt[0];
+    I.vt[1] = vt[1];
+    I.vt[2] = vt[2];
+}
+
+__global__ void k_advance(Batch bt) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= bt.ninst) return;
+    bt.inst[k].time += bt.inst[k].dt;
+}
+
+__global__ void k_set_dt(Batch bt, int k, double dt) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) bt.inst[k].dt = dt;
 }
 
 /* Ext-cell key of a position: cell floor(x/h) + 1 per axis when inside
@@ -51,10 +104,12 @@ __device__ __forceinline__ uint32_t cell_key(const GridGeom &g, double x, double
     return (uint32_t)c[0] + (uint32_t)g.ec[0] * ((uint32_t)c[1] + (uint32_t)g.ec[1] * (uint32_t)c[2]);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_keys(GridGeom g, PView P, BView B) {
+__global__ __launch_bounds__(BLOCK) void k_keys(Batch bt, PView P, BView B) {
     uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= P.n) return;
-    uint32_t k = cell_key(g, P.x[i], P.x[P.n + i], P.x[2 * P.n + i]);
+    const Inst &I = bt.inst[inst_of_particle(bt, i)];
+    uint32_t k = cell_key(I.g, P.x[i], P.x[P.n + i], P.x[2 * P.n + i]);
+    if (k != MPM_NONE) k += (uint32_t)I.cell_off;
     B.key[i] = k;
     if (k != MPM_NONE) atomicAdd(&B.count[k], 1u);
 }
@@ -161,12 +216,14 @@ struct Merge8 {
     uint32_t p[8], e[8], h[8];
 };
 
-__device__ __forceinline__ void merge_init(Merge8 &M, const BView &B, const GridGeom &g, int ix, int iy, int iz) {
+__device__ __forceinline__ void merge_init(Merge8 &M, const BView &B, const GridGeom &g, int64_t cell_off, int ix,
+                                           int iy, int iz) {
     /* node i touches cells i-1 and i per axis = ext cells i and i+1 */
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         int cx = ix + ((k >> 2) & 1), cy = iy + ((k >> 1) & 1), cz = iz + (k & 1);
-        uint32_t c = (uint32_t)cx + (uint32_t)g.ec[0] * ((uint32_t)cy + (uint32_t)g.ec[1] * (uint32_t)cz);
+        uint32_t c = (uint32_t)cell_off +
+                     (uint32_t)cx + (uint32_t)g.ec[0] * ((uint32_t)cy + (uint32_t)g.ec[1] * (uint32_t)cz);
         M.p[k] = B.start[c];
         M.e[k] = B.start[c + 1];
         M.h[k] = (M.p[k] < M.e[k]) ? B.sorted[M.p[k]] : MPM_NONE;
@@ -220,16 +277,19 @@ __device__ __forceinline__ void dc_consider(DCWin &w, const PView &P, uint32_t i
             }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_p2g_grid(GridGeom g, PView P, NView Nd, BView B, StepParams sp) {
+__global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd, BView B) {
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const Inst &I = bt.inst[inst_of_tile(bt, tile)];
+    const GridGeom &g = I.g;
     int ix, iy, iz;
-    node_of_tile(g, xcd_remap(blockIdx.x, gridDim.x), ix, iy, iz);
+    node_of_tile(g, tile - I.tile_off, ix, iy, iz);
     if (ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) return;
-    const int64_t node = (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
+    const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
     const size_t n = P.n;
     const double xn[3] = {node_coord(ix, g.L[0], g.cells[0]), node_coord(iy, g.L[1], g.cells[1]),
                           node_coord(iz, g.L[2], g.cells[2])};
     Merge8 M;
-    merge_init(M, B, g, ix, iy, iz);
+    merge_init(M, B, g, I.cell_off, ix, iy, iz);
     uint32_t id = merge_min(M);
     if (id == MPM_NONE) {
         Nd.active[node] = 0;
@@ -258,7 +318,7 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_grid(GridGeom g, PView P, NView N
         f[0] += -V * (gx * s0 + gy * s3 + gz * s5);
         f[1] += -V * (gy * s1 + gx * s3 + gz * s4);
         f[2] += -V * (gz * s2 + gx * s5 + gy * s4);
-        if (sp.has_fext) {
+        if (I.has_fext) {
             f[0] += N * P.fext[id];
             f[1] += N * P.fext[n + id];
             f[2] += N * P.fext[2 * n + id];
@@ -277,13 +337,13 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_grid(GridGeom g, PView P, NView N
             f[a] = 0.0;
         }
     /* runSimulation_IntegrateGrid, DP:90-142 */
-    const double dt = sp.dt;
+    const double dt = I.dt;
     {
         double pp = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
         if (sqrt(pp) > 1.0e-24) {
             double inv = 1.0 / sqrt(pp);
             double nx = p[0] * inv, ny = p[1] * inv, nz = p[2] * inv;
-            double sc = sp.damping * fabs(f[0] * nx + f[1] * ny + f[2] * nz);
+            double sc = I.damping * fabs(f[0] * nx + f[1] * ny + f[2] * nz);
             f[0] -= sc * nx;
             f[1] -= sc * ny;
             f[2] -= sc * nz;
@@ -316,7 +376,7 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_grid(GridGeom g, PView P, NView N
     p[0] += f[0] * dt;
     p[1] += f[1] * dt;
     p[2] += f[2] * dt;
-    const size_t nn = (size_t)g.nnodes;
+    const size_t nn = (size_t)bt.nnodes;
     Nd.active[node] = 1;
     Nd.m[node] = m;
 #pragma unroll
@@ -330,10 +390,12 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_grid(GridGeom g, PView P, NView N
 
 /* G2P pass 1: v_p += N * (f_i / m_i) * dt over the adjacent nodes in
  * GridPoint_Mediator order (x, then y, then z innermost), DP:490-510. */
-__global__ __launch_bounds__(BLOCK) void k_g2p_velocity(GridGeom g, PView P, NView Nd, StepParams sp) {
+__global__ __launch_bounds__(BLOCK) void k_g2p_velocity(Batch bt, PView P, NView Nd) {
     uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= P.n) return;
-    const size_t n = P.n, nn = (size_t)g.nnodes;
+    const Inst &I = bt.inst[inst_of_particle(bt, i)];
+    const GridGeom &g = I.g;
+    const size_t n = P.n, nn = (size_t)bt.nnodes;
     double x[3] = {P.x[i], P.x[n + i], P.x[2 * n + i]};
     int b[3], lo[3], hi[3];
     double s[3][2];
@@ -343,25 +405,25 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_velocity(GridGeom g, PView P, NVi
 #pragma unroll
         for (int o = 0; o < 2; o++) s[a][o] = axis_shape(x[a], node_coord(b[a] + o, g.L[a], g.cells[a]), g.h[a]).s;
     }
+    bool ok[3][2];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int o = 0; o < 2; o++) ok[a][o] = o >= lo[a] && o < hi[a] && b[a] + o >= 0 && b[a] + o <= g.nn[a] - 1;
     double v[3] = {P.v[i], P.v[n + i], P.v[2 * n + i]};
-    const double dt = sp.dt;
-    for (int ox = lo[0]; ox < hi[0]; ox++) {
-        int jx = b[0] + ox;
-        if (jx < 0 || jx > g.nn[0] - 1) continue;
-        for (int oy = lo[1]; oy < hi[1]; oy++) {
-            int jy = b[1] + oy;
-            if (jy < 0 || jy > g.nn[1] - 1) continue;
-            for (int oz = lo[2]; oz < hi[2]; oz++) {
-                int jz = b[2] + oz;
-                if (jz < 0 || jz > g.nn[2] - 1) continue;
-                size_t node = (size_t)jx + (size_t)g.nn[0] * ((size_t)jy + (size_t)g.nn[1] * jz);
-                if (Nd.m[node] > sp.mass_min) {
-                    double N = s[0][ox] * s[1][oy] * s[2][oz];
-                    v[0] += N * Nd.a[node] * dt;
-                    v[1] += N * Nd.a[nn + node] * dt;
-                    v[2] += N * Nd.a[2 * nn + node] * dt;
-                }
-            }
+    const double dt = I.dt;
+    /* corners in GridPoint_Mediator order: x outer, z inner */
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int ox = c >> 2, oy = (c >> 1) & 1, oz = c & 1;
+        if (!(ok[0][ox] && ok[1][oy] && ok[2][oz])) continue;
+        size_t node = (size_t)I.node_off + (size_t)(b[0] + ox) +
+                      (size_t)g.nn[0] * ((size_t)(b[1] + oy) + (size_t)g.nn[1] * (b[2] + oz));
+        if (Nd.m[node] > I.mass_min) {
+            double N = s[0][ox] * s[1][oy] * s[2][oz];
+            v[0] += N * Nd.a[node] * dt;
+            v[1] += N * Nd.a[nn + node] * dt;
+            v[2] += N * Nd.a[2 * nn + node] * dt;
         }
     }
     P.v[i] = v[0];
@@ -373,19 +435,22 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_velocity(GridGeom g, PView P, NVi
  * order, then the second displacement-control override (DP:568) and the BC
  * re-application (DP:573-599). Stores the final v, p, the zeroed force
  * components and the hoisted p/m used by G2P pass 2 (DP:198). */
-__global__ __launch_bounds__(BLOCK) void k_remap_grid(GridGeom g, PView P, NView Nd, BView B, StepParams sp) {
+__global__ __launch_bounds__(BLOCK) void k_remap_grid(Batch bt, PView P, NView Nd, BView B) {
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const Inst &I = bt.inst[inst_of_tile(bt, tile)];
+    const GridGeom &g = I.g;
     int ix, iy, iz;
-    node_of_tile(g, xcd_remap(blockIdx.x, gridDim.x), ix, iy, iz);
+    node_of_tile(g, tile - I.tile_off, ix, iy, iz);
     if (ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) return;
-    const int64_t node = (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
+    const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
     if (!Nd.active[node]) return;
-    const size_t n = P.n, nn = (size_t)g.nnodes;
+    const size_t n = P.n, nn = (size_t)bt.nnodes;
     const double xn[3] = {node_coord(ix, g.L[0], g.cells[0]), node_coord(iy, g.L[1], g.cells[1]),
                           node_coord(iz, g.L[2], g.cells[2])};
     const double m = Nd.m[node];
-    const bool massive = m > sp.mass_min;
+    const bool massive = m > I.mass_min;
     Merge8 M;
-    merge_init(M, B, g, ix, iy, iz);
+    merge_init(M, B, g, I.cell_off, ix, iy, iz);
     double v[3] = {0.0, 0.0, 0.0};
     DCWin w;
     w.rank[0] = w.rank[1] = w.rank[2] = -1;
@@ -427,15 +492,73 @@ __device__ __forceinline__ void block_count_add(unsigned long long *dst, uint32_
     if ((threadIdx.x & 63) == 0 && b) atomicAdd(dst, (unsigned long long)__popcll(b));
 }
 
+/* State update after the return map (DP:242-278 minus F and V, which the
+ * caller has done): eps, sigma, plastic strain, energies, hardening. */
+__device__ __forceinline__ uint32_t finish_update(const PView &P, size_t i, const double de[6], double sig[6],
+                                                  const CRResult &cr, double R, double V, double dt) {
+    const size_t n = P.n;
+#pragma unroll
+    for (int k = 0; k < 6; k++) P.eps[k * n + i] += de[k];
+#pragma unroll
+    for (int k = 0; k < 6; k++) sig[k] += cr.dsig[k];
+#pragma unroll
+    for (int k = 0; k < 6; k++) P.sig[k * n + i] = sig[k];
+#pragma unroll
+    for (int k = 0; k < 6; k++) P.epsp[k * n + i] += cr.depsp[k];
+    double wi = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) wi += (sig[k] - 0.5 * cr.dsig[k]) * de[k] * V;
+    double Ws = P.Ws[i] + wi;
+    P.Ws[i] = Ws;
+    P.Wti[i] += Ws * dt;
+    if (Ws > P.Wmax[i]) P.Wmax[i] = Ws;
+    double wp = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) wp += (sig[k] - 0.5 * cr.dsig[k]) * cr.depsp[k] * V;
+    P.Wp[i] += wp;
+    P.R[i] = R + cr.dR;
+    uint32_t bad = 0;
+    for (int k = 0; k < 6; k++) bad |= !isfinite(sig[k]);
+    return bad;
+}
+
+/* Trial state of the radial return: J2 of sigma + C:de and the yield radius
+ * (the first half of ConstitutiveRelation_calculateIncrement_{PerfectlyPlastic,
+ * VonMisesHardening}.cpp, identical expressions). */
+__device__ __forceinline__ void trial_state(const MatTable &mt, int m, double R, const double sig[6],
+                                            const double de[6], double dtr[6], double &J2, double &K) {
+    double str[6];
+    trial_increment(mt.E[m], mt.nu[m], de, dtr);
+    for (int k = 0; k < 6; k++) str[k] = sig[k] + dtr[k];
+    J2 = 0.0;
+    J2 += 1.0 / 6.0 * (str[0] - str[1]) * (str[0] - str[1]);
+    J2 += 1.0 / 6.0 * (str[1] - str[2]) * (str[1] - str[2]);
+    J2 += 1.0 / 6.0 * (str[2] - str[0]) * (str[2] - str[0]);
+    J2 += str[3] * str[3];
+    J2 += str[4] * str[4];
+    J2 += str[5] * str[5];
+    K = mt.yield[m] / sqrt(3.0);
+    if (mt.type[m] == 4) K = K + R / sqrt(2.0);
+}
+
 /* runSimulation_G2P_P2_SmallStrain (DP:175-280) + the displacement-control
- * schedule (DP:766-789) + next step's binning key. */
-__global__ __launch_bounds__(BLOCK) void k_g2p_strain(GridGeom g, PView P, NView Nd, BView B, StepParams sp,
-                                                     MatTable mt, unsigned long long *health) {
+ * schedule (DP:766-789) + next step's binning key.
+ *   MODE_EXACT      the exact FP64 return map for every particle;
+ *   MODE_CAPTURE    as exact, and every yielding point's (sigma, de, R,
+ *                   material) is appended to the sample buffer;
+ *   MODE_SURROGATE  points of the surrogate's material whose trial state
+ *                   yields are queued for the MLP; everything else, and
+ *                   all kinematics, is finished here. */
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void k_g2p_strain(Batch bt, PView P, NView Nd, BView B, MatTable mt,
+                                                     unsigned long long *health, SurrogateArgs sg) {
     uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     const bool live = i < P.n;
-    uint32_t f_outside = 0, f_null = 0, f_plastic = 0, f_nonfinite = 0;
+    uint32_t f_outside = 0, f_null = 0, f_plastic = 0, f_nonfinite = 0, f_queued = 0;
     if (live) {
-        const size_t n = P.n, nn = (size_t)g.nnodes;
+        const Inst &I = bt.inst[inst_of_particle(bt, i)];
+        const GridGeom &g = I.g;
+        const size_t n = P.n, nn = (size_t)bt.nnodes;
         const double x0[3] = {P.x[i], P.x[n + i], P.x[2 * n + i]};
         double x[3] = {x0[0], x0[1], x0[2]};
         int b[3], lo[3], hi[3];
@@ -451,33 +574,34 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_strain(GridGeom g, PView P, NView
         for (int c = 0; c < 3; c++)
 #pragma unroll
             for (int r = 0; r < 3; r++) L[c][r] = 0.0;
-        const double dt = sp.dt;
+        const double dt = I.dt;
         int touched = 0;
-        for (int ox = lo[0]; ox < hi[0]; ox++) {
-            int jx = b[0] + ox;
-            if (jx < 0 || jx > g.nn[0] - 1) continue;
-            for (int oy = lo[1]; oy < hi[1]; oy++) {
-                int jy = b[1] + oy;
-                if (jy < 0 || jy > g.nn[1] - 1) continue;
-                for (int oz = lo[2]; oz < hi[2]; oz++) {
-                    int jz = b[2] + oz;
-                    if (jz < 0 || jz > g.nn[2] - 1) continue;
-                    touched++;
-                    size_t node = (size_t)jx + (size_t)g.nn[0] * ((size_t)jy + (size_t)g.nn[1] * jz);
-                    double N, gr[3];
-                    shape_full(as[0][ox], as[1][oy], as[2][oz], g.h, N, gr[0], gr[1], gr[2]);
-                    if (Nd.m[node] > sp.mass_min) {
-                        x[0] += N * Nd.u[node] * dt;
-                        x[1] += N * Nd.u[nn + node] * dt;
-                        x[2] += N * Nd.u[2 * nn + node] * dt;
-                    }
-                    double vv[3] = {Nd.v[node], Nd.v[nn + node], Nd.v[2 * nn + node]};
+        bool ok[3][2];
 #pragma unroll
-                    for (int c = 0; c < 3; c++)
+        for (int a = 0; a < 3; a++)
 #pragma unroll
-                        for (int r = 0; r < 3; r++) L[c][r] += vv[r] * gr[c];
-                }
+            for (int o = 0; o < 2; o++)
+                ok[a][o] = o >= lo[a] && o < hi[a] && b[a] + o >= 0 && b[a] + o <= g.nn[a] - 1;
+        /* corners in GridPoint_Mediator order: x outer, z inner */
+#pragma unroll
+        for (int cc = 0; cc < 8; cc++) {
+            const int ox = cc >> 2, oy = (cc >> 1) & 1, oz = cc & 1;
+            if (!(ok[0][ox] && ok[1][oy] && ok[2][oz])) continue;
+            touched++;
+            size_t node = (size_t)I.node_off + (size_t)(b[0] + ox) +
+                          (size_t)g.nn[0] * ((size_t)(b[1] + oy) + (size_t)g.nn[1] * (b[2] + oz));
+            double N, gr[3];
+            shape_full(as[0][ox], as[1][oy], as[2][oz], g.h, N, gr[0], gr[1], gr[2]);
+            if (Nd.m[node] > I.mass_min) {
+                x[0] += N * Nd.u[node] * dt;
+                x[1] += N * Nd.u[nn + node] * dt;
+                x[2] += N * Nd.u[2 * nn + node] * dt;
             }
+            double vv[3] = {Nd.v[node], Nd.v[nn + node], Nd.v[2 * nn + node]};
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) L[c][r] += vv[r] * gr[c];
         }
         f_outside = (touched == 0);
         P.x[i] = x[0];
@@ -486,6 +610,7 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_strain(GridGeom g, PView P, NView
         const int mat = P.mat[i];
         if (mat < 0 || mat >= mt.n) {
             f_null = 1;
+            for (int k = 0; k < 6; k++) f_nonfinite |= !isfinite(P.sig[k * n + i]);
         } else {
             double dF[3][3];
 #pragma unroll
@@ -504,14 +629,7 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_strain(GridGeom g, PView P, NView
             de[3] = E[0][1] + E[1][0];
             de[4] = E[1][2] + E[2][1];
             de[5] = E[2][0] + E[0][2];
-            double sig[6];
-#pragma unroll
-            for (int k = 0; k < 6; k++) sig[k] = P.sig[k * n + i];
-            const double R = P.R[i];
-            CRResult cr;
-            constitutive(mt, mat, R, sig, de, cr);
-            f_plastic = cr.yielded;
-            /* F = F * dF (DP:242) */
+            /* F = F * dF (DP:242), V = det(F) V0 (DP:244-245) */
             double F[3][3], Fn[3][3];
 #pragma unroll
             for (int c = 0; c < 3; c++)
@@ -530,42 +648,67 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_strain(GridGeom g, PView P, NView
                          Fn[2][0] * (Fn[0][1] * Fn[1][2] - Fn[1][1] * Fn[0][2]);
             const double V = det * P.V0[i];
             P.V[i] = V;
+            double sig[6];
 #pragma unroll
-            for (int k = 0; k < 6; k++) P.eps[k * n + i] += de[k];
+            for (int k = 0; k < 6; k++) sig[k] = P.sig[k * n + i];
+            const double R = P.R[i];
+            bool queued = false;
+            if (MODE == MODE_SURROGATE && mat == sg.material) {
+                double dtr[6], J2, K;
+                trial_state(mt, mat, R, sig, de, dtr, J2, K);
+                if (!(J2 <= K * K)) {
+                    /* queue for the network: exact strain increment + normalised features */
+                    uint32_t slot = atomicAdd(sg.qcount, 1u);
+                    sg.qid[slot] = i;
+                    const size_t cap = (size_t)P.n;
 #pragma unroll
-            for (int k = 0; k < 6; k++) sig[k] += cr.dsig[k];
-#pragma unroll
-            for (int k = 0; k < 6; k++) P.sig[k * n + i] = sig[k];
-#pragma unroll
-            for (int k = 0; k < 6; k++) P.epsp[k * n + i] += cr.depsp[k];
-            double wi = 0.0;
-#pragma unroll
-            for (int k = 0; k < 6; k++) wi += (sig[k] - 0.5 * cr.dsig[k]) * de[k] * V;
-            double Ws = P.Ws[i] + wi;
-            P.Ws[i] = Ws;
-            P.Wti[i] += Ws * dt;
-            if (Ws > P.Wmax[i]) P.Wmax[i] = Ws;
-            double wp = 0.0;
-#pragma unroll
-            for (int k = 0; k < 6; k++) wp += (sig[k] - 0.5 * cr.dsig[k]) * cr.depsp[k] * V;
-            P.Wp[i] += wp;
-            P.R[i] = R + cr.dR;
-            for (int k = 0; k < 6; k++) f_nonfinite |= !isfinite(sig[k]);
+                    for (int k = 0; k < 6; k++) sg.qde[k * cap + slot] = de[k];
+                    float feat[MLP_IN];
+                    for (int k = 0; k < 6; k++) feat[k] = (float)(sig[k] / sg.stress_scale);
+                    for (int k = 0; k < 6; k++) feat[6 + k] = (float)(de[k] / sg.strain_scale);
+                    feat[12] = (float)(R / sg.stress_scale);
+                    uint8_t inside = 1;
+                    for (int k = 0; k < MLP_IN; k++) {
+                        float z = (feat[k] - sg.norm.in_mean[k]) / sg.norm.in_std[k];
+                        inside &= (z >= sg.norm.in_lo[k] && z <= sg.norm.in_hi[k]) ? 1 : 0;
+                        sg.qx[(size_t)slot * MLP_IN + k] = z;
+                    }
+                    sg.qenv[slot] = inside;
+                    queued = true;
+                    f_queued = 1;
+                    f_plastic = 1;
+                    /* strain, stress and the rest are finished by the post kernel */
+                }
+            }
+            if (!queued) {
+                CRResult cr;
+                constitutive(mt, mat, R, sig, de, cr);
+                f_plastic = cr.yielded;
+                if (MODE == MODE_CAPTURE && cr.yielded) {
+                    unsigned long long slot = atomicAdd(sg.capcount, 1ull);
+                    if (slot < (unsigned long long)sg.capcap) {
+                        double *row = sg.cap + slot * 14;
+                        for (int k = 0; k < 6; k++) row[k] = sig[k];
+                        for (int k = 0; k < 6; k++) row[6 + k] = de[k];
+                        row[12] = R;
+                        row[13] = (double)mat;
+                    }
+                }
+                f_nonfinite |= finish_update(P, i, de, sig, cr, R, V, dt);
+            }
         }
         f_nonfinite |= !(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]));
-        if (f_null) {
-            for (int k = 0; k < 6; k++) f_nonfinite |= !isfinite(P.sig[k * n + i]);
-        }
         /* displacement-control schedule, DP:766-789 */
-        if (sp.timeline_nonempty && P.dc_rank[i] >= 0) {
+        if (I.tl_count != 0 && P.dc_rank[i] >= 0) {
             uint8_t axm = P.dc_axes[i];
             double mult = (double)P.dc_mult[i];
 #pragma unroll
             for (int a = 0; a < 3; a++)
-                if ((axm >> a) & 1) P.v[a * n + i] = mult * sp.vt[a];
+                if ((axm >> a) & 1) P.v[a * n + i] = mult * I.vt[a];
         }
         /* next step's binning */
         uint32_t k = cell_key(g, x[0], x[1], x[2]);
+        if (k != MPM_NONE) k += (uint32_t)I.cell_off;
         B.key[i] = k;
         if (k != MPM_NONE) atomicAdd(&B.count[k], 1u);
     }
@@ -573,6 +716,81 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_strain(GridGeom g, PView P, NView
     block_count_add(&health[HEALTH_NULL], f_null);
     block_count_add(&health[HEALTH_PLASTIC], f_plastic);
     block_count_add(&health[HEALTH_NONFINITE], f_nonfinite);
+    if (MODE == MODE_SURROGATE) block_count_add(&health[HEALTH_SURROGATE_CALLS], f_queued);
+}
+
+/* Applies the network's output to the queued points: the stress increment
+ * is the exact elastic trial increment minus the predicted plastic
+ * correction; it is accepted when the inputs were inside the training
+ * envelope and the new stress lies on the yield surface within
+ * tol * sigma_y; otherwise the exact radial return runs.  The plastic strain
+ * and hardening increments follow from the accepted stress increment as in
+ * ConstitutiveRelation_calculateIncrement_VonMisesHardening.cpp:69-90. */
+__global__ __launch_bounds__(BLOCK) void k_surrogate_post(Batch bt, PView P, MatTable mt, unsigned long long *health,
+                                                         SurrogateArgs sg) {
+    const uint32_t count = *sg.qcount;
+    uint32_t f_fallback = 0, f_nonfinite = 0;
+    uint32_t slot = blockIdx.x * BLOCK + threadIdx.x;
+    for (; slot < ((count + BLOCK - 1) / BLOCK) * BLOCK; slot += gridDim.x * BLOCK) {
+        f_fallback = 0;
+        f_nonfinite = 0;
+        if (slot < count) {
+            const size_t n = P.n, cap = (size_t)P.n;
+            const uint32_t i = sg.qid[slot];
+            const int m = P.mat[i];
+            double de[6], sig[6];
+            for (int k = 0; k < 6; k++) de[k] = sg.qde[k * cap + slot];
+            for (int k = 0; k < 6; k++) sig[k] = P.sig[k * n + i];
+            const double R = P.R[i];
+            double dtr[6], J2, K;
+            trial_state(mt, m, R, sig, de, dtr, J2, K);
+            CRResult cr;
+            bool accept = sg.qenv[slot] != 0;
+            const float *y = sg.qy + (size_t)slot * MLP_OUT;
+            for (int k = 0; k < 6; k++) {
+                double corr = ((double)y[k] * (double)sg.norm.out_std[k] + (double)sg.norm.out_mean[k]) * sg.stress_scale;
+                cr.dsig[k] = dtr[k] - corr;
+            }
+            if (accept) {
+                double s2[6];
+                for (int k = 0; k < 6; k++) s2[k] = sig[k] + cr.dsig[k];
+                double J = 0.0;
+                J += 1.0 / 6.0 * (s2[0] - s2[1]) * (s2[0] - s2[1]);
+                J += 1.0 / 6.0 * (s2[1] - s2[2]) * (s2[1] - s2[2]);
+                J += 1.0 / 6.0 * (s2[2] - s2[0]) * (s2[2] - s2[0]);
+                J += s2[3] * s2[3] + s2[4] * s2[4] + s2[5] * s2[5];
+                double resid = fabs(sqrt(J) - K);
+                accept = isfinite(resid) && resid <= sg.tol * mt.yield[m];
+            }
+            if (accept) {
+                const double E = mt.E[m], nu = mt.nu[m];
+                double G = 0.5 * E / (1.0 + nu);
+                double dee[6];
+                dee[0] = cr.dsig[0] / E - cr.dsig[1] * nu / E - cr.dsig[2] * nu / E;
+                dee[1] = cr.dsig[1] / E - cr.dsig[0] * nu / E - cr.dsig[2] * nu / E;
+                dee[2] = cr.dsig[2] / E - cr.dsig[0] * nu / E - cr.dsig[1] * nu / E;
+                dee[3] = cr.dsig[3] / G;
+                dee[4] = cr.dsig[4] / G;
+                dee[5] = cr.dsig[5] / G;
+                for (int k = 0; k < 6; k++) cr.depsp[k] = de[k] - dee[k];
+                cr.dR = 0.0;
+                if (mt.type[m] == 4) {
+                    double ep = 0.0;
+                    for (int k = 0; k < 6; k++) ep += cr.depsp[k] * cr.depsp[k];
+                    ep *= 2.0 / 3.0;
+                    ep = sqrt(ep);
+                    if ((sqrt(2.0 / 3.0) * mt.c1[m] - R) > 0.0) cr.dR = mt.c0[m] * (sqrt(2.0 / 3.0) * mt.c1[m] - R) * ep;
+                }
+                cr.yielded = true;
+            } else {
+                constitutive(mt, m, R, sig, de, cr);
+                f_fallback = 1;
+            }
+            f_nonfinite = finish_update(P, i, de, sig, cr, R, P.V[i], bt.inst[inst_of_particle(bt, i)].dt);
+        }
+        block_count_add(&health[HEALTH_SURROGATE_FALLBACKS], f_fallback);
+        block_count_add(&health[HEALTH_NONFINITE], f_nonfinite);
+    }
 }
 
 /* ---- reportConsole sums (PhysicsEngine.cpp:40-183) in list order ---- */
@@ -581,9 +799,9 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_strain(GridGeom g, PView P, NView
 
 /* One block. Lanes stage a chunk of values in LDS; lane 0 adds them in list
  * order, so every sum is the reference's sequential sum. */
-__global__ __launch_bounds__(BLOCK) void k_monitors(GridGeom g, PView P, NView Nd, MonLists ml, double *out) {
+__global__ __launch_bounds__(BLOCK) void k_monitors(PView P, NView Nd, MonLists ml, double *out) {
     __shared__ double a0[MON_CHUNK], a1[MON_CHUNK], a2[MON_CHUNK];
-    const size_t n = P.n, nn = (size_t)g.nnodes;
+    const size_t n = P.n;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     /* Position_y */
     for (int64_t base = 0; base < ml.count[0]; base += MON_CHUNK) {
@@ -714,19 +932,25 @@ __global__ __launch_bounds__(BLOCK) void k_constitutive(MatTable mt, const doubl
     dR[i] = o.dR;
 }
 
+/* Graph-safe clear of a small counter array (no memset node). */
+__global__ void k_zero_u32(uint32_t *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
 /* ---- launchers ---- */
 
 static inline unsigned blocks_for(int64_t n, int64_t per) { return (unsigned)((n + per - 1) / per); }
 
-hipError_t launch_keys(const GridGeom &g, const PView &P, const BView &B, hipStream_t s) {
+hipError_t launch_keys(const Batch &bt, const PView &P, const BView &B, hipStream_t s) {
     if (P.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_keys, dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, g, P, B);
+    hipLaunchKernelGGL(k_keys, dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, bt, P, B);
     return hipGetLastError();
 }
 
 int64_t scan_block_count(int64_t necells) { return (necells + SCAN_TILE - 1) / SCAN_TILE; }
 
-hipError_t launch_bin(const GridGeom &g, const PView &P, const BView &B, uint32_t *block_sum, hipStream_t s) {
+hipError_t launch_bin(const PView &P, const BView &B, uint32_t *block_sum, hipStream_t s) {
     int64_t nb = scan_block_count(B.necells);
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(BLOCK), 0, s, B.count, B.necells, block_sum);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(BLOCK), 0, s, block_sum, nb);
@@ -736,29 +960,67 @@ hipError_t launch_bin(const GridGeom &g, const PView &P, const BView &B, uint32_
     return hipGetLastError();
 }
 
-static unsigned node_tiles(const GridGeom &g) {
-    int64_t tnx = (g.nn[0] + TILE_X - 1) / TILE_X, tny = (g.nn[1] + TILE_Y - 1) / TILE_Y,
-            tnz = (g.nn[2] + TILE_Z - 1) / TILE_Z;
-    return (unsigned)(tnx * tny * tnz);
-}
-
-hipError_t launch_step_kernels(const GridGeom &g, const PView &P, const NView &N, const BView &B,
-                               const StepParams &sp, const MatTable &mt, unsigned long long *health,
-                               hipStream_t s) {
-    unsigned tiles = node_tiles(g);
-    hipLaunchKernelGGL(k_p2g_grid, dim3(tiles), dim3(BLOCK), 0, s, g, P, N, B, sp);
-    if (P.n > 0)
-        hipLaunchKernelGGL(k_g2p_velocity, dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, g, P, N, sp);
-    hipLaunchKernelGGL(k_remap_grid, dim3(tiles), dim3(BLOCK), 0, s, g, P, N, B, sp);
-    if (P.n > 0)
-        hipLaunchKernelGGL(k_g2p_strain, dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, g, P, N, B, sp, mt,
-                           health);
+hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView &N, const BView &B,
+                        const MatTable &mt, unsigned long long *health, const SurrogateArgs &sg, hipStream_t s) {
+    switch (phase) {
+    case PHASE_P2G_GRID:
+        if (bt.ntiles) hipLaunchKernelGGL(k_p2g_grid, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B);
+        break;
+    case PHASE_G2P_VELOCITY:
+        if (P.n > 0) hipLaunchKernelGGL(k_g2p_velocity, dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, bt, P, N);
+        break;
+    case PHASE_REMAP_GRID:
+        if (bt.ntiles) hipLaunchKernelGGL(k_remap_grid, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B);
+        break;
+    case PHASE_G2P_STRAIN:
+        if (P.n > 0) {
+            dim3 grid(blocks_for(P.n, BLOCK));
+            if (sg.mode == MODE_SURROGATE)
+                hipLaunchKernelGGL(k_g2p_strain<MODE_SURROGATE>, grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health, sg);
+            else if (sg.mode == MODE_CAPTURE)
+                hipLaunchKernelGGL(k_g2p_strain<MODE_CAPTURE>, grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health, sg);
+            else
+                hipLaunchKernelGGL(k_g2p_strain<MODE_EXACT>, grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health, sg);
+        }
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_monitors(const GridGeom &g, const PView &P, const NView &N, const MonLists &ml, double *out,
+hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_surrogate_post(const Batch &bt, const PView &P, const MatTable &mt, unsigned long long *health,
+                                 const SurrogateArgs &sg, hipStream_t s) {
+    if (P.n == 0) return hipSuccess;
+    unsigned grid = (unsigned)std::min<int64_t>(blocks_for(P.n, BLOCK), 2048);
+    hipLaunchKernelGGL(k_surrogate_post, dim3(grid), dim3(BLOCK), 0, s, bt, P, mt, health, sg);
+    return hipGetLastError();
+}
+
+hipError_t launch_prepare(const Batch &bt, hipStream_t s) {
+    hipLaunchKernelGGL(k_prepare, dim3((bt.ninst + 63) / 64), dim3(64), 0, s, bt);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance(const Batch &bt, hipStream_t s) {
+    hipLaunchKernelGGL(k_advance, dim3((bt.ninst + 63) / 64), dim3(64), 0, s, bt);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_dt(const Batch &bt, int inst, double dt, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, s, bt, inst, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_monitors(const PView &P, const NView &N, int64_t nnodes, const MonLists &ml, double *out,
                            hipStream_t s) {
-    hipLaunchKernelGGL(k_monitors, dim3(1), dim3(BLOCK), 0, s, g, P, N, ml, out);
+    (void)nnodes;
+    hipLaunchKernelGGL(k_monitors, dim3(1), dim3(BLOCK), 0, s, P, N, ml, out);
     return hipGetLastError();
 }
 

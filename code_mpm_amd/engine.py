@@ -125,6 +125,13 @@ class Engine:
     def step(self, dt, nsteps=1):
         self._check(self._lib.mpm_step(self._h, float(dt), int(nsteps)), "step")
 
+    def step_timed(self, dt, nsteps=1):
+        """Steps with HIP-event timing per phase; returns {phase: total ms}."""
+        out = np.zeros(len(capi.PHASES))
+        self._check(self._lib.mpm_step_timed(self._h, float(dt), int(nsteps), out.ctypes.data_as(capi.PDBL)),
+                    "step_timed")
+        return dict(zip(capi.PHASES, out.tolist()))
+
     def run(self, dt_total):
         fin = ctypes.c_int32()
         self._check(self._lib.mpm_run(self._h, float(dt_total), ctypes.byref(fin)), "run")
@@ -206,3 +213,25 @@ class Engine:
 
     def reset_health(self):
         self._check(self._lib.mpm_reset_health(self._h), "reset_health")
+
+    # -- learned return map ----------------------------------------------------
+    def set_surrogate(self, surrogate, material=0, residual_tolerance=1.0e-4):
+        """Route `material`'s return map through a trained Surrogate (None: exact map)."""
+        h = surrogate.handle if surrogate is not None else None
+        self._check(self._lib.mpm_set_surrogate(self._h, h, int(material), float(residual_tolerance)),
+                    "set_surrogate")
+        self._surrogate = surrogate
+
+    def set_sample_capture(self, capacity):
+        self._check(self._lib.mpm_set_sample_capture(self._h, int(capacity)), "set_sample_capture")
+        self._capture_capacity = int(capacity)
+
+    def captured_samples(self):
+        """(rows, produced): rows is (k, 14) = sigma6, dstrain6, R, material."""
+        cnt = ctypes.c_int64()
+        self._check(self._lib.mpm_get_captured_samples(self._h, None, 0, ctypes.byref(cnt)), "captured count")
+        k = min(cnt.value, getattr(self, "_capture_capacity", 0))
+        rows = np.empty((k, 14))
+        if k:
+            self._check(self._lib.mpm_get_captured_samples(self._h, ptr(rows), k, ctypes.byref(cnt)), "captured")
+        return rows, cnt.value

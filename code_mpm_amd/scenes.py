@@ -363,3 +363,98 @@ def random_block(seed=0, cells=(6, 5, 4), n=300, h=1.0e-3, features=True):
         timeline_v=np.array([[0.0, 0.0, 0.0], [0.3, -2.0, 0.1], [0.3, -2.0, 0.1]]) if features else np.zeros((0, 3)),
         damping=0.05 if features else 0.0,
         console_interval=10 * dt)
+
+
+def crop_x(scene, cells_x):
+    """The part of a scene with x < cells_x * h on a grid cut to cells_x cells
+    in x (node rows beyond it dropped); lists are filtered and re-indexed."""
+    h = scene.h
+    keep = np.nonzero(scene.fields["POSITION"][0] < cells_x * h[0])[0]
+    remap = -np.ones(scene.n, np.int64)
+    remap[keep] = np.arange(len(keep))
+    cells = np.array([cells_x, scene.cells[1], scene.cells[2]], np.int32)
+    length = np.array([cells_x * h[0], scene.length[1], scene.length[2]])
+    nn_old = np.asarray(scene.cells) + 1
+    nn = cells + 1
+    iz, iy, ix = np.meshgrid(np.arange(nn[2]), np.arange(nn[1]), np.arange(nn[0]), indexing="ij")
+    old = (ix + nn_old[0] * (iy + nn_old[1] * iz)).ravel()
+
+    def sub(idx):
+        r = remap[np.asarray(idx, np.int64)]
+        return r[r >= 0]
+
+    dc_keep = remap[scene.dc_index] >= 0
+    return Scene(
+        name=scene.name + f"_x{cells_x}", length=length, cells=cells, fixed=scene.fixed[old],
+        friction=scene.friction[old], materials=scene.materials,
+        fields={k: np.ascontiguousarray(v[..., keep]) for k, v in scene.fields.items()},
+        material=scene.material[keep], dt=scene.dt, time_end=scene.time_end,
+        dc_index=remap[scene.dc_index][dc_keep], dc_axes=scene.dc_axes[dc_keep], dc_mult=scene.dc_mult[dc_keep],
+        monitors={k: sub(v) for k, v in scene.monitors.items()}, timeline_t=scene.timeline_t,
+        timeline_v=scene.timeline_v, damping=scene.damping, mass_min=scene.mass_min, time0=scene.time0,
+        console_interval=scene.console_interval, info=dict(scene.info, particles=len(keep)))
+
+
+def graded_foam(seed, h=1.0e-4, n_target=None, E=10.0e9, nu=0.3, rho=1000.0, yield_stress=100.0e6,
+                platen_speed=5.0, steps=500):
+    """One graded-foam instance (the batched workload): a cuboid of 2x2x2
+    particles per cell with spherical voids whose radii are lognormal
+    (mu = ln 3h, sigma = 0.3) and whose number density grows linearly along
+    z; porosity ~ U[0.3, 0.7] and particle count ~ log-uniform [1e4, 2e5],
+    both drawn from ``seed``.  J2 perfectly plastic.  Fixed-y support below,
+    displacement-controlled platen on top at platen_speed."""
+    rng = np.random.default_rng(seed)
+    porosity = rng.uniform(0.3, 0.7)
+    if n_target is None:
+        n_target = int(np.exp(rng.uniform(np.log(1.0e4), np.log(2.0e5))))
+    solid_cells = n_target / 8.0 / (1.0 - porosity)
+    s = max(6, int(round((solid_cells / 1.5) ** (1.0 / 3.0))))
+    body = np.array([s, int(round(1.5 * s)), s])  # y is the loading axis
+    margin = 2
+    cells = body + np.array([2 * margin, 2 * margin + 3, 2 * margin])
+    hh = np.full(3, h)
+    length = cells * hh
+    iz, iy, ix = np.meshgrid(np.arange(body[2]), np.arange(body[1]), np.arange(body[0]), indexing="ij")
+    cidx = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], 1) + np.array([margin, 1, margin])
+    x = lattice_in_cells(cidx, hh)
+    lo = np.array([margin, 1, margin]) * h
+    hi = lo + body * h
+    keep = np.ones(x.shape[1], bool)
+    # voids: add until the porosity is reached; centre density linear in y (1 at the bottom, 3 at the top)
+    target_removed = porosity * x.shape[1]
+    for _ in range(100000):
+        if (~keep).sum() >= target_removed:
+            break
+        r = np.exp(rng.normal(np.log(3.0 * h), 0.3))
+        while True:
+            c = lo + rng.uniform(0, 1, 3) * (hi - lo)
+            if rng.uniform() < (1.0 + 2.0 * (c[1] - lo[1]) / (hi[1] - lo[1])) / 3.0:
+                break
+        d2 = np.sum((x - c[:, None]) ** 2, 0)
+        keep &= d2 > r * r
+    x = np.ascontiguousarray(x[:, keep])
+    n_body = x.shape[1]
+    # platen: one cell layer right above the body
+    py = 1 + body[1]
+    pc = np.array([[i, py, k] for k in range(margin, margin + body[2]) for i in range(margin, margin + body[0])])
+    xp = lattice_in_cells(pc, hh)
+    x = np.concatenate([x, xp], 1)
+    n = x.shape[1]
+    fields = default_fields(x, np.zeros((3, n)), (h / 2) ** 3, rho)
+    mats = [dict(type=PLASTIC, density=rho, elastic_modulus=E, poisson_ratio=nu, yield_stress=yield_stress)]
+    nodes = node_positions(length, cells)
+    fixed = np.where(nodes[1] < 1.5 * h, AXIS_Y, 0).astype(np.uint8)
+    platen = np.arange(n_body, n, dtype=np.int64)
+    c = np.sqrt(E / rho)
+    dt = 0.1 * h / c
+    return Scene(
+        name=f"graded_foam_{seed}", length=length, cells=cells.astype(np.int32), fixed=fixed,
+        friction=np.zeros(fixed.shape), materials=mats, fields=fields, material=np.zeros(n, np.int32), dt=dt,
+        time_end=steps * dt, dc_index=platen, dc_axes=np.full(len(platen), AXIS_Y, np.uint8),
+        dc_mult=np.full(len(platen), -1.0, np.float32),
+        monitors={"DISPLACEMENT": platen, "MOMENTUM": np.arange(n_body, dtype=np.int64),
+                  "ENERGY": np.arange(n_body, dtype=np.int64)},
+        timeline_t=np.array([0.0, 10.0]), timeline_v=np.array([[0.0, platen_speed, 0.0], [0.0, platen_speed, 0.0]]),
+        console_interval=50 * dt,
+        info=dict(porosity=porosity, particles=n, body_particles=n_body, seed=seed, E=E, nu=nu, rho=rho,
+                  yield_stress=yield_stress))

@@ -139,6 +139,8 @@ typedef struct mpm_health {
     int64_t outside_grid;         /* particle-steps whose adjacent nodes all lie off the grid */
     int64_t null_material;        /* particle-steps skipped for a missing material */
     int64_t plastic_points;       /* particle-steps whose return map yielded */
+    int64_t surrogate_calls;      /* particle-steps sent to the learned return map */
+    int64_t surrogate_fallbacks;  /* of those, answered by the exact map instead */
 } mpm_health;
 
 /* --- lifetime ------------------------------------------------------------ */
@@ -201,6 +203,21 @@ int mpm_step(mpm_engine *e, double dt, int32_t nsteps);
  * without the console report: steps of d_TimeIncrement_Maximum, the last one
  * clamped to the remainder; *finished = 0 while time < time_end, else 1. */
 int mpm_run(mpm_engine *e, double dt_total, int32_t *finished);
+/* Step phases timed by mpm_step_timed, in launch order. */
+enum {
+    MPM_PHASE_BIN = 0,          /* cell binning (scan, scatter, in-cell sort) */
+    MPM_PHASE_P2G_GRID = 1,     /* P2G + displacement control + grid integration */
+    MPM_PHASE_G2P_VELOCITY = 2, /* G2P pass 1 */
+    MPM_PHASE_REMAP_GRID = 3,   /* velocity remap + control + BC re-application */
+    MPM_PHASE_G2P_STRAIN = 4,   /* G2P pass 2 + return map + schedule */
+    MPM_PHASE_COUNT = 5
+};
+/* mpm_step with each phase bracketed by HIP events on the engine's stream;
+ * phase_ms receives the summed milliseconds per phase.  Replaces the
+ * reference's per-block wall-clock timers a_Runtime (PhysicsEngine.h:87,
+ * PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:487-790).
+ * Synchronises after every step. */
+int mpm_step_timed(mpm_engine *e, double dt, int32_t nsteps, double phase_ms[MPM_PHASE_COUNT]);
 /* Waits for the engine's stream. */
 int mpm_synchronize(mpm_engine *e);
 
@@ -227,6 +244,52 @@ int mpm_reset_health(mpm_engine *e);
 int mpm_constitutive_batch(int32_t device, const mpm_material *material, const double *backstress,
                            const double *stress, const double *dstrain, double *dstress,
                            double *dstrain_plastic, double *dbackstress, int64_t n);
+
+/* --- learned return-map surrogate (no counterpart in the reference) ---------- */
+
+/* A 13 -> 64 -> 64 -> 64 -> 7 SiLU MLP with BF16 weights and FP32
+ * accumulation on MFMA, standing in for
+ * ConstitutiveRelation::calculateIncrement_VonMisesHardening /
+ * calculateIncrement_Plastic (ConstitutiveRelation_calculateIncrement_*.cpp)
+ * at trial states outside the yield surface.  Inputs: sigma (6) and R scaled
+ * by a stress scale, the strain increment (6) by a strain scale, then
+ * standardised; outputs: the plastic correction of the stress increment
+ * (elastic trial increment minus the returned increment, 6, over the stress
+ * scale) and the equivalent plastic strain increment (over the strain
+ * scale), standardised.  Parameters: mpm_surrogate_param_count() FP32 values,
+ * W1[16][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64][16] b4[16]. */
+typedef struct mpm_surrogate mpm_surrogate;
+int64_t mpm_surrogate_param_count(void);
+/* Glorot-uniform initial weights drawn from seed. */
+int mpm_surrogate_create(mpm_surrogate **out, int32_t device, uint64_t seed);
+int mpm_surrogate_destroy(mpm_surrogate *s);
+int mpm_surrogate_set_params(mpm_surrogate *s, const float *params, int64_t count);
+int mpm_surrogate_get_params(mpm_surrogate *s, float *params, int64_t count);
+/* Scales and standardisation; in_lo / in_hi bound the standardised inputs
+ * (the training envelope): a point outside uses the exact map. */
+int mpm_surrogate_set_normalization(mpm_surrogate *s, double stress_scale, double strain_scale,
+                                    const float in_mean[13], const float in_std[13], const float in_lo[13],
+                                    const float in_hi[13], const float out_mean[7], const float out_std[7]);
+/* Standardised rows, host arrays: x n x 13 -> y n x 7. */
+int mpm_surrogate_forward(mpm_surrogate *s, const float *x, float *y, int64_t n);
+/* Gradient of mean((y - t)^2) over n rows and the 7 outputs. */
+int mpm_surrogate_gradient(mpm_surrogate *s, const float *x, const float *t, int64_t n, float *grad, float *loss);
+/* Copies a standardised training set to the device. */
+int mpm_surrogate_load_training_set(mpm_surrogate *s, const float *x, const float *t, int64_t n);
+/* Adam on random 64-row-aligned windows of `batch` rows; loss_history
+ * (optional) receives each step's batch loss. */
+int mpm_surrogate_train(mpm_surrogate *s, int32_t steps, int32_t batch, float lr, uint64_t seed, float *loss_history);
+
+/* Routes the return map of material `material` through the surrogate (NULL
+ * restores the exact map).  A prediction is accepted when its inputs lie in
+ * the training envelope and |sqrt(J2(sigma_new)) - (K0 + R/sqrt(2))| <=
+ * residual_tolerance * sigma_y; otherwise the exact map runs. */
+int mpm_set_surrogate(mpm_engine *e, mpm_surrogate *s, int32_t material, double residual_tolerance);
+/* Appends every yielding point of the exact map (sigma 6, dstrain 6, R,
+ * material index: 14 doubles) to a device buffer of `capacity` rows; 0 stops. */
+int mpm_set_sample_capture(mpm_engine *e, int64_t capacity);
+/* Copies up to `capacity` captured rows; *count = rows produced (may exceed the buffer). */
+int mpm_get_captured_samples(mpm_engine *e, double *rows, int64_t capacity, int64_t *count);
 
 #ifdef __cplusplus
 }

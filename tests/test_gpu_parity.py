@@ -122,7 +122,7 @@ def test_edge_cases():
 
 def test_empty_particle_set():
     from code_mpm_amd import Engine, scenes
-    scene = scenes.random_block(seed=9, n=4)
+    scene = scenes.random_block(seed=9, n=4, features=False)
     g = Engine(device=0)
     g.set_grid(scene.length, scene.cells)
     g.set_materials(scene.materials)
