@@ -105,6 +105,13 @@ SIGNATURES = {
     "mpm_set_time": (I32, [P, DBL, DBL, DBL]),
     "mpm_set_damping": (I32, [P, DBL, DBL]),
     "mpm_step": (I32, [P, DBL, I32]),
+    "mpm_step_graph": (I32, [P, I32]),
+    "mpm_set_instances": (I32, [P, I32, P, P, P]),
+    "mpm_get_instance_layout": (I32, [P, I32, PI64, PI64, PI64]),
+    "mpm_set_instance_timeline": (I32, [P, I32, P, P, I32]),
+    "mpm_set_instance_time": (I32, [P, I32, DBL, DBL, DBL]),
+    "mpm_get_instance_time": (I32, [P, I32, PDBL]),
+    "mpm_get_instance_monitors": (I32, [P, I32, ctypes.POINTER(Monitors)]),
     "mpm_run": (I32, [P, DBL, PI32]),
     "mpm_step_timed": (I32, [P, DBL, I32, PDBL]),
     "mpm_synchronize": (I32, [P]),

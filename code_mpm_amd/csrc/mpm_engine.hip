@@ -3,8 +3,12 @@
  *
  * The engine owns every device buffer (particles, nodes, cell bins) and one
  * HIP stream; the reference's PhysicsEngine owns the same objects as heap
- * AoS (PhysicsEngine.h:82-117).  Scalars that steer a step (time, dt, the
- * timeline velocity) stay on the host and reach the kernels as arguments.
+ * AoS (PhysicsEngine.h:82-117).  An engine holds one or more independent
+ * instances (scenes) laid out back to back in the same arrays; each step is
+ * one set of launches over all of them.  Per-instance geometry, step size,
+ * damping, time and timeline sit in a device table (Inst); time advances on
+ * the device, and the host keeps a mirror that adds the same dt in the same
+ * order.
  */
 #include <hip/hip_runtime.h>
 
@@ -13,6 +17,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -26,26 +31,40 @@ const int kFieldComponents[MPM_PF_COUNT] = {3, 3, 3, 1, 1, 1, 9, 6, 6, 6, 1, 1, 
 
 std::string g_create_error = "no error";
 
-/* TimeLine (TimeLine.h:27-46): piecewise-linear velocity; the last interval
- * [t_i, t_{i+1}) containing t wins; zero outside every interval. */
-struct HostTimeline {
-    std::vector<double> t;
-    std::vector<double> v;  /* 3 per point */
-    void velocity(double time, double out[3]) const {
-        out[0] = out[1] = out[2] = 0.0;
-        for (size_t i = 0; i + 1 < t.size(); i++) {
-            if (t[i] <= time && time < t[i + 1]) {
-                double s = (time - t[i]) / (t[i + 1] - t[i]);
-                for (int a = 0; a < 3; a++) out[a] = (1.0 - s) * v[3 * i + a] + (s)*v[3 * (i + 1) + a];
-            }
-        }
-    }
-};
-
 template <typename T>
 void dfree(T *&p) {
     if (p) (void)hipFree(p);
     p = nullptr;
+}
+
+/* Host-side state of one instance. */
+struct HostInst {
+    GridGeom g{};
+    int64_t node_off = 0, cell_off = 0, part_off = 0, npart = 0;
+    uint32_t tile_off = 0, tiles = 0;
+    double time = 0.0, dt_max = 1.0e-3, time_end = 10.0;
+    double damping = 0.0, mass_min = 0.0;
+    std::vector<double> tl; /* t, vx, vy, vz per point */
+};
+
+bool make_geom(const double L[3], const int32_t cells[3], GridGeom &g) {
+    for (int a = 0; a < 3; a++) {
+        if (cells[a] < 1 || !(L[a] > 0.0) || !std::isfinite(L[a])) return false;
+        g.L[a] = L[a];
+        g.h[a] = L[a] / (double)cells[a];
+        g.cells[a] = cells[a];
+        g.nn[a] = cells[a] + 1;
+        g.ec[a] = cells[a] + 2;
+    }
+    g.nnodes = (int64_t)g.nn[0] * g.nn[1] * g.nn[2];
+    g.necells = (int64_t)g.ec[0] * g.ec[1] * g.ec[2];
+    return true;
+}
+
+uint32_t tiles_of(const GridGeom &g) {
+    int64_t t = (int64_t)((g.nn[0] + TILE_X - 1) / TILE_X) * ((g.nn[1] + TILE_Y - 1) / TILE_Y) *
+                ((g.nn[2] + TILE_Z - 1) / TILE_Z);
+    return (uint32_t)t;
 }
 
 }  // namespace
@@ -55,15 +74,21 @@ struct mpm_engine {
     hipStream_t stream = nullptr;
     std::string err = "no error";
 
-    bool grid_set = false;
-    GridGeom g{};
+    /* instances */
+    std::vector<HostInst> inst;
+    int64_t nnodes = 0, ncells = 0;
+    uint32_t ntiles = 0;
+    Inst *dinst = nullptr;
+    uint32_t *dtile_off = nullptr, *dpinst = nullptr;
+    double *dtl = nullptr;
+    bool table_dirty = true;
+    bool layout_set = false;
 
     /* nodes */
     double *nm = nullptr, *np = nullptr, *nv = nullptr, *nf = nullptr, *na = nullptr, *nu = nullptr,
            *nfty = nullptr, *nfric = nullptr;
     uint8_t *nact = nullptr, *nfix = nullptr;
-    int64_t *d_fixed_y = nullptr;
-    int64_t nfy = 0;
+    std::vector<int64_t> fixed_y; /* host copy, ascending */
 
     /* particles */
     int64_t n = 0;
@@ -77,17 +102,15 @@ struct mpm_engine {
     uint32_t *bkey = nullptr, *bcount = nullptr, *bstart = nullptr, *bsorted = nullptr, *bblock = nullptr;
     bool keys_valid = false;
 
-    /* materials, lists, control */
+    /* materials, lists */
     MatTable mt{};
-    int64_t *d_mon[MPM_MON_COUNT] = {};
-    int64_t mon_count[MPM_MON_COUNT] = {};
-    HostTimeline timeline;
-    double time = 0.0, dt_max = 1.0e-3, time_end = 10.0;
-    double damping = 0.0, mass_min = 0.0;
+    std::vector<int64_t> mon[MPM_MON_COUNT];
     int64_t cycle = 0;
 
     unsigned long long *d_health = nullptr;
     double *d_mon_out = nullptr;
+    int64_t *d_monbuf = nullptr;
+    int64_t monbuf_len = 0;
     int64_t steps = 0;
 
     /* surrogate and sample capture */
@@ -103,6 +126,40 @@ struct mpm_engine {
     unsigned long long *capcount = nullptr;
     int64_t capcap = 0;
 
+    /* captured step graphs, keyed by step count */
+    std::map<int, hipGraphExec_t> graphs;
+
+    int fail(int code, const char *fmt, ...) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+    int hip(hipError_t st, const char *what) {
+        if (st == hipSuccess) return MPM_OK;
+        return fail(st == hipErrorOutOfMemory ? MPM_ERR_NOMEM : MPM_ERR_HIP, "%s: %s", what, hipGetErrorString(st));
+    }
+
+    void drop_graphs() {
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+    void free_nodes() {
+        dfree(nm); dfree(np); dfree(nv); dfree(nf); dfree(na); dfree(nu); dfree(nfty); dfree(nfric);
+        dfree(nact); dfree(nfix);
+        fixed_y.clear();
+        dfree(bcount); dfree(bstart); dfree(bblock);
+        dfree(dinst); dfree(dtile_off); dfree(dtl);
+    }
+    void free_particles() {
+        for (int f = 0; f < MPM_PF_COUNT; f++) dfree(pf[f]);
+        dfree(pmat); dfree(pdc_rank); dfree(pdc_axes); dfree(pdc_mult);
+        dfree(bkey); dfree(bsorted); dfree(dpinst);
+        for (int k = 0; k < MPM_MON_COUNT; k++) mon[k].clear();
+    }
     void free_queue() {
         dfree(qid); dfree(qde); dfree(qx); dfree(qy); dfree(qenv);
         qcap = 0;
@@ -138,36 +195,6 @@ struct mpm_engine {
         return a;
     }
 
-    int fail(int code, const char *fmt, ...) {
-        char buf[512];
-        va_list ap;
-        va_start(ap, fmt);
-        vsnprintf(buf, sizeof(buf), fmt, ap);
-        va_end(ap);
-        err = buf;
-        return code;
-    }
-    int hip(hipError_t st, const char *what) {
-        if (st == hipSuccess) return MPM_OK;
-        return fail(st == hipErrorOutOfMemory ? MPM_ERR_NOMEM : MPM_ERR_HIP, "%s: %s", what, hipGetErrorString(st));
-    }
-
-    void free_nodes() {
-        dfree(nm); dfree(np); dfree(nv); dfree(nf); dfree(na); dfree(nu); dfree(nfty); dfree(nfric);
-        dfree(nact); dfree(nfix); dfree(d_fixed_y);
-        nfy = 0;
-        dfree(bcount); dfree(bstart); dfree(bblock);
-    }
-    void free_particles() {
-        for (int f = 0; f < MPM_PF_COUNT; f++) dfree(pf[f]);
-        dfree(pmat); dfree(pdc_rank); dfree(pdc_axes); dfree(pdc_mult);
-        dfree(bkey); dfree(bsorted);
-        for (int k = 0; k < MPM_MON_COUNT; k++) {
-            dfree(d_mon[k]);
-            mon_count[k] = 0;
-        }
-    }
-
     PView pview() const {
         PView P;
         P.x = pf[MPM_PF_POSITION];
@@ -201,64 +228,310 @@ struct mpm_engine {
     BView bview() const {
         BView B;
         B.key = bkey; B.count = bcount; B.start = bstart; B.sorted = bsorted;
-        B.necells = g.necells;
+        B.necells = ncells;
         return B;
+    }
+    Batch batch() const {
+        Batch b;
+        b.inst = dinst;
+        b.tile_off = dtile_off;
+        b.pinst = inst.size() > 1 ? dpinst : nullptr;
+        b.tl = dtl;
+        b.ninst = (int)inst.size();
+        b.ntiles = ntiles;
+        b.nnodes = nnodes;
+        return b;
+    }
+
+    /* Allocates nodes, bins and the instance table for the current `inst`. */
+    int build_layout() {
+        (void)hipStreamSynchronize(stream);
+        drop_graphs();
+        free_nodes();
+        layout_set = false;
+        nnodes = ncells = 0;
+        ntiles = 0;
+        for (auto &I : inst) {
+            I.node_off = nnodes;
+            I.cell_off = ncells;
+            I.tile_off = ntiles;
+            I.tiles = tiles_of(I.g);
+            nnodes += I.g.nnodes;
+            ncells += I.g.necells;
+            if ((int64_t)ntiles + I.tiles >= (int64_t)0xFFFFFFFFll) return fail(MPM_ERR_RANGE, "too many node tiles");
+            ntiles += I.tiles;
+        }
+        if (ncells >= (int64_t)0xFFFFFFFFll) return fail(MPM_ERR_RANGE, "grid too large: %lld cells", (long long)ncells);
+        const size_t nn = (size_t)nnodes;
+        int rc = 0;
+        if ((rc = hip(hipMalloc(&nm, 8 * nn), "alloc node mass")) ||
+            (rc = hip(hipMalloc(&np, 24 * nn), "alloc node momentum")) ||
+            (rc = hip(hipMalloc(&nv, 24 * nn), "alloc node velocity")) ||
+            (rc = hip(hipMalloc(&nf, 24 * nn), "alloc node force")) ||
+            (rc = hip(hipMalloc(&na, 24 * nn), "alloc node f/m")) ||
+            (rc = hip(hipMalloc(&nu, 24 * nn), "alloc node p/m")) ||
+            (rc = hip(hipMalloc(&nfty, 8 * nn), "alloc node reaction")) ||
+            (rc = hip(hipMalloc(&nfric, 8 * nn), "alloc node friction")) ||
+            (rc = hip(hipMalloc(&nact, nn), "alloc node active")) ||
+            (rc = hip(hipMalloc(&nfix, nn), "alloc node fixed")) ||
+            (rc = hip(hipMalloc(&bcount, 4 * (size_t)ncells), "alloc cell counts")) ||
+            (rc = hip(hipMalloc(&bstart, 4 * ((size_t)ncells + 1)), "alloc cell starts")) ||
+            (rc = hip(hipMalloc(&bblock, 4 * (size_t)scan_block_count(ncells)), "alloc scan blocks")) ||
+            (rc = hip(hipMalloc(&dinst, sizeof(Inst) * inst.size()), "alloc instance table")) ||
+            (rc = hip(hipMalloc(&dtile_off, 4 * (inst.size() + 1)), "alloc tile offsets"))) {
+            free_nodes();
+            return rc;
+        }
+        if ((rc = hip(hipMemset(nact, 0, nn), "clear active")) || (rc = hip(hipMemset(nfix, 0, nn), "clear fixed")) ||
+            (rc = hip(hipMemset(nfric, 0, 8 * nn), "clear friction")) || (rc = hip(hipMemset(nm, 0, 8 * nn), "clear mass")))
+            return rc;
+        std::vector<uint32_t> to(inst.size() + 1);
+        for (size_t k = 0; k < inst.size(); k++) to[k] = inst[k].tile_off;
+        to[inst.size()] = ntiles;
+        if ((rc = hip(hipMemcpy(dtile_off, to.data(), 4 * to.size(), hipMemcpyHostToDevice), "upload tile offsets")))
+            return rc;
+        layout_set = true;
+        keys_valid = false;
+        table_dirty = true;
+        return MPM_OK;
+    }
+
+    /* Uploads timelines and the instance table (the host mirror is authoritative). */
+    int upload_table() {
+        if (!table_dirty) return MPM_OK;
+        int rc = hip(hipStreamSynchronize(stream), "table sync");
+        if (rc) return rc;
+        std::vector<double> tl;
+        std::vector<Inst> t(inst.size());
+        for (size_t k = 0; k < inst.size(); k++) {
+            const HostInst &H = inst[k];
+            Inst &I = t[k];
+            std::memset(&I, 0, sizeof(I));
+            I.g = H.g;
+            I.node_off = H.node_off;
+            I.cell_off = H.cell_off;
+            I.part_off = H.part_off;
+            I.npart = H.npart;
+            I.tile_off = H.tile_off;
+            I.tiles = H.tiles;
+            I.dt = H.dt_max;
+            I.damping = H.damping;
+            I.mass_min = H.mass_min;
+            I.time = H.time;
+            I.tl_off = (int32_t)(tl.size() / 4);
+            I.tl_count = (int32_t)(H.tl.size() / 4);
+            I.has_fext = has_fext ? 1 : 0;
+            tl.insert(tl.end(), H.tl.begin(), H.tl.end());
+        }
+        dfree(dtl);
+        if (!tl.empty()) {
+            if ((rc = hip(hipMalloc(&dtl, 8 * tl.size()), "alloc timelines"))) return rc;
+            if ((rc = hip(hipMemcpy(dtl, tl.data(), 8 * tl.size(), hipMemcpyHostToDevice), "upload timelines"))) return rc;
+        }
+        if ((rc = hip(hipMemcpy(dinst, t.data(), sizeof(Inst) * t.size(), hipMemcpyHostToDevice), "upload table")))
+            return rc;
+        table_dirty = false;
+        drop_graphs(); /* graphs hold the old timeline pointer */
+        return MPM_OK;
     }
 
     /* Cell keys and counts for the current positions (after an upload). */
     int ensure_keys() {
         if (keys_valid) return MPM_OK;
-        int rc = hip(hipMemsetAsync(bcount, 0, sizeof(uint32_t) * g.necells, stream), "clear cell counts");
+        int rc = hip(hipMemsetAsync(bcount, 0, sizeof(uint32_t) * ncells, stream), "clear cell counts");
         if (rc) return rc;
-        rc = hip(launch_keys(g, pview(), bview(), stream), "cell keys");
+        rc = hip(launch_keys(batch(), pview(), bview(), stream), "cell keys");
         if (rc) return rc;
         keys_valid = true;
         return MPM_OK;
     }
 
-    /* One step; with ev != nullptr an event is recorded before every phase
-     * and after the last (ev[0..PHASE_COUNT]). */
-    int step_once(double dt, hipEvent_t *ev = nullptr) {
-        StepParams sp;
-        sp.dt = dt;
-        sp.mass_min = mass_min;
-        sp.damping = damping;
-        timeline.velocity(time, sp.vt);
-        sp.has_fext = has_fext ? 1 : 0;
-        sp.timeline_nonempty = timeline.t.empty() ? 0 : 1;
-        int rc = ensure_keys();
-        if (rc) return rc;
+    /* The kernels of one step (no host work).  With ev != nullptr an event is
+     * recorded before every phase and after the last (ev[0..PHASE_COUNT]). */
+    int enqueue_step(hipEvent_t *ev = nullptr) {
+        const Batch b = batch();
+        int rc;
+        if ((rc = hip(launch_prepare(b, stream), "step prepare"))) return rc;
         if (ev && (rc = hip(hipEventRecord(ev[0], stream), "event"))) return rc;
-        rc = hip(launch_bin(g, pview(), bview(), bblock, stream), "cell binning");
-        if (rc) return rc;
+        if ((rc = hip(launch_bin(pview(), bview(), bblock, stream), "cell binning"))) return rc;
         const SurrogateArgs sa = sargs();
-        if (sur && (rc = ensure_queue())) return rc;
         for (int ph = PHASE_P2G_GRID; ph < PHASE_COUNT; ph++) {
             if (ev && (rc = hip(hipEventRecord(ev[ph], stream), "event"))) return rc;
             if (ph == PHASE_G2P_STRAIN && sur && (rc = hip(launch_zero_u32(qcount, 1, stream), "queue reset")))
                 return rc;
-            rc = hip(launch_phase(ph, g, pview(), nview(), bview(), sp, mt, d_health, sa, stream), "step kernels");
-            if (rc) return rc;
+            if ((rc = hip(launch_phase(ph, b, pview(), nview(), bview(), mt, d_health, sa, stream), "step kernels")))
+                return rc;
         }
         if (sur) {
             /* the network on the queued points, then acceptance / fallback */
             rc = hip(mlp_launch_forward(sur->params, sur->wbf16, qx, qy, n, stream, qcount), "surrogate forward");
-            if (!rc) rc = hip(launch_surrogate_post(pview(), sp, mt, d_health, sa, stream), "surrogate post");
+            if (!rc) rc = hip(launch_surrogate_post(b, pview(), mt, d_health, sa, stream), "surrogate post");
             if (rc) return rc;
         }
         if (ev && (rc = hip(hipEventRecord(ev[PHASE_COUNT], stream), "event"))) return rc;
-        keys_valid = true; /* g2p_strain wrote the next keys and counts */
-        time += dt;
+        return hip(launch_advance(b, stream), "step advance");
+    }
+
+    int begin_steps() {
+        if (!layout_set) return fail(MPM_ERR_STATE, "step before set_grid / set_instances");
+        int rc;
+        if ((rc = upload_table())) return rc;
+        if (sur && (rc = ensure_queue())) return rc;
+        return ensure_keys();
+    }
+
+    /* Host mirror of the device clock after one step. */
+    void advance_mirror(const double *dt0) {
+        for (size_t k = 0; k < inst.size(); k++) inst[k].time += (k == 0 && dt0) ? *dt0 : inst[k].dt_max;
         cycle++;
         steps++;
+    }
+
+    /* One eager step; dt0 (when given) replaces instance 0's dt_max. */
+    int step_once(const double *dt0, hipEvent_t *ev = nullptr) {
+        int rc = begin_steps();
+        if (rc) return rc;
+        if ((rc = hip(launch_set_dt(batch(), 0, dt0 ? *dt0 : inst[0].dt_max, stream), "set dt"))) return rc;
+        if ((rc = enqueue_step(ev))) return rc;
+        keys_valid = true; /* g2p_strain wrote the next keys and counts */
+        advance_mirror(dt0);
+        return MPM_OK;
+    }
+
+    int step_graph(int nsteps) {
+        int rc = begin_steps();
+        if (rc) return rc;
+        /* instance 0 back to its dt_max (outside any graph) */
+        if ((rc = hip(launch_set_dt(batch(), 0, inst[0].dt_max, stream), "set dt"))) return rc;
+        auto it = graphs.find(nsteps);
+        if (it == graphs.end()) {
+            if ((rc = hip(hipStreamSynchronize(stream), "pre-capture sync"))) return rc;
+            hipGraph_t graph;
+            if ((rc = hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture"))) return rc;
+            for (int s = 0; s < nsteps && !rc; s++) rc = enqueue_step();
+            hipError_t ce = hipStreamEndCapture(stream, &graph);
+            if (rc) return rc;
+            if ((rc = hip(ce, "end capture"))) return rc;
+            hipGraphExec_t exec;
+            rc = hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+            (void)hipGraphDestroy(graph);
+            if (rc) return rc;
+            it = graphs.emplace(nsteps, exec).first;
+        }
+        if ((rc = hip(hipGraphLaunch(it->second, stream), "graph launch"))) return rc;
+        keys_valid = true;
+        for (int s = 0; s < nsteps; s++) advance_mirror(nullptr);
+        return MPM_OK;
+    }
+
+    /* reportConsole sums over monitor entries in [p0, p1) and fixed-y nodes in [n0, n1). */
+    int monitors(int64_t p0, int64_t p1, int64_t n0, int64_t n1, double time, mpm_monitors *out) {
+        std::memset(out, 0, sizeof(*out));
+        out->time = time;
+        std::vector<int64_t> lists[MPM_MON_COUNT + 1];
+        for (int k = 0; k < MPM_MON_COUNT; k++)
+            for (int64_t q : mon[k])
+                if (q >= p0 && q < p1) lists[k].push_back(q);
+        for (int64_t q : fixed_y)
+            if (q >= n0 && q < n1) lists[MPM_MON_COUNT].push_back(q);
+        int64_t total = 0;
+        for (auto &l : lists) total += (int64_t)l.size();
+        int rc;
+        if ((rc = hip(hipStreamSynchronize(stream), "monitor sync"))) return rc;
+        if (total > monbuf_len) {
+            dfree(d_monbuf);
+            if ((rc = hip(hipMalloc(&d_monbuf, 8 * (size_t)total), "alloc monitor lists"))) return rc;
+            monbuf_len = total;
+        }
+        MonLists ml;
+        int64_t off = 0;
+        for (int k = 0; k <= MPM_MON_COUNT; k++) {
+            if (!lists[k].empty() &&
+                (rc = hip(hipMemcpy(d_monbuf + off, lists[k].data(), 8 * lists[k].size(), hipMemcpyHostToDevice),
+                          "upload monitor list")))
+                return rc;
+            if (k < MPM_MON_COUNT) {
+                ml.list[k] = d_monbuf + off;
+                ml.count[k] = (int64_t)lists[k].size();
+            } else {
+                ml.fixed_y = d_monbuf + off;
+                ml.nfy = (int64_t)lists[k].size();
+            }
+            off += (int64_t)lists[k].size();
+        }
+        if ((rc = hip(launch_monitors(pview(), nview(), nnodes, ml, d_mon_out, stream), "monitors"))) return rc;
+        double v[8];
+        if ((rc = hip(hipMemcpyAsync(v, d_mon_out, sizeof(v), hipMemcpyDeviceToHost, stream), "monitor copy")))
+            return rc;
+        if ((rc = hip(hipStreamSynchronize(stream), "monitor sync"))) return rc;
+        out->has_position = !lists[MPM_MON_DISPLACEMENT].empty();
+        out->has_momentum = !lists[MPM_MON_MOMENTUM].empty();
+        out->has_energy = !lists[MPM_MON_ENERGY].empty();
+        out->has_stress = !lists[MPM_MON_STRESS].empty();
+        out->position_y = out->has_position ? v[0] : 0.0;
+        out->force_y = v[1];
+        out->momentum_y = out->has_momentum ? v[2] : 0.0;
+        if (out->has_energy) {
+            out->kinetic_energy = v[3];
+            out->strain_energy = v[4];
+            out->plastic_energy = v[5];
+        }
+        if (out->has_stress) {
+            out->stress_y = v[6];
+            out->strain_y = v[7];
+        }
+        return MPM_OK;
+    }
+
+    int alloc_particles(int64_t count) {
+        (void)hipStreamSynchronize(stream);
+        drop_graphs();
+        free_particles();
+        free_queue();
+        n = count;
+        has_fext = false;
+        keys_valid = false;
+        table_dirty = true;
+        if (n == 0) return MPM_OK;
+        const size_t sn = (size_t)n;
+        int rc;
+        for (int f = 0; f < MPM_PF_COUNT; f++) {
+            if ((rc = hip(hipMalloc(&pf[f], 8 * sn * kFieldComponents[f]), "alloc particle field"))) {
+                free_particles();
+                n = 0;
+                return rc;
+            }
+            if ((rc = hip(hipMemset(pf[f], 0, 8 * sn * kFieldComponents[f]), "clear particle field"))) return rc;
+        }
+        if ((rc = hip(hipMalloc(&pmat, 4 * sn), "alloc material ids")) ||
+            (rc = hip(hipMalloc(&pdc_rank, 4 * sn), "alloc control ranks")) ||
+            (rc = hip(hipMalloc(&pdc_axes, sn), "alloc control axes")) ||
+            (rc = hip(hipMalloc(&pdc_mult, 4 * sn), "alloc control multipliers")) ||
+            (rc = hip(hipMalloc(&bkey, 4 * sn), "alloc keys")) ||
+            (rc = hip(hipMalloc(&bsorted, 4 * sn), "alloc sorted ids"))) {
+            free_particles();
+            n = 0;
+            return rc;
+        }
+        /* defaults of MaterialPoint_BC (MaterialPoint_BC.h:31-82) */
+        std::vector<double> F(9 * sn, 0.0);
+        for (int d = 0; d < 3; d++) std::fill(F.begin() + (4 * d) * sn, F.begin() + (4 * d + 1) * sn, 1.0);
+        std::vector<int32_t> minus1(sn, -1);
+        std::vector<float> one(sn, 1.0f);
+        if ((rc = hip(hipMemcpy(pf[MPM_PF_DEFORMATION_GRADIENT], F.data(), 8 * 9 * sn, hipMemcpyHostToDevice), "init F")) ||
+            (rc = hip(hipMemcpy(pmat, minus1.data(), 4 * sn, hipMemcpyHostToDevice), "init materials")) ||
+            (rc = hip(hipMemcpy(pdc_rank, minus1.data(), 4 * sn, hipMemcpyHostToDevice), "init control ranks")) ||
+            (rc = hip(hipMemset(pdc_axes, 0, sn), "init control axes")) ||
+            (rc = hip(hipMemcpy(pdc_mult, one.data(), 4 * sn, hipMemcpyHostToDevice), "init multipliers")))
+            return rc;
         return MPM_OK;
     }
 };
 
-#define CHECK_E(e)                                   \
-    do {                                             \
-        if (!(e)) return MPM_ERR_ARG;                \
-        if (hipSetDevice((e)->device) != hipSuccess) \
+#define CHECK_E(e)                                                                 \
+    do {                                                                           \
+        if (!(e)) return MPM_ERR_ARG;                                              \
+        if (hipSetDevice((e)->device) != hipSuccess)                               \
             return (e)->fail(MPM_ERR_HIP, "hipSetDevice(%d) failed", (e)->device); \
     } while (0)
 
@@ -287,6 +560,7 @@ int mpm_create(mpm_engine **out, int32_t device) {
         hipMalloc(&e->d_mon_out, sizeof(double) * 8) != hipSuccess ||
         hipMalloc(&e->qcount, sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&e->capcount, sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(e->capcount, 0, sizeof(unsigned long long)) != hipSuccess ||
         hipMemset(e->d_health, 0, sizeof(unsigned long long) * HEALTH_SLOTS) != hipSuccess) {
         g_create_error = "stream or buffer creation failed";
         mpm_destroy(e);
@@ -300,11 +574,13 @@ int mpm_destroy(mpm_engine *e) {
     if (!e) return MPM_OK;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    e->drop_graphs();
     e->free_nodes();
     e->free_particles();
+    e->free_queue();
     dfree(e->d_health);
     dfree(e->d_mon_out);
-    e->free_queue();
+    dfree(e->d_monbuf);
     dfree(e->qcount);
     dfree(e->cap);
     dfree(e->capcount);
@@ -318,63 +594,77 @@ const char *mpm_last_error(const mpm_engine *e) { return e ? e->err.c_str() : g_
 int mpm_set_grid(mpm_engine *e, const double length[3], const int32_t cells[3]) {
     CHECK_E(e);
     if (!length || !cells) return e->fail(MPM_ERR_ARG, "null grid argument");
-    for (int a = 0; a < 3; a++) {
-        if (cells[a] < 1) return e->fail(MPM_ERR_ARG, "cells[%d] = %d < 1", a, cells[a]);
-        if (!(length[a] > 0.0) || !std::isfinite(length[a])) return e->fail(MPM_ERR_ARG, "length[%d] not positive", a);
+    HostInst I;
+    if (!make_geom(length, cells, I.g)) return e->fail(MPM_ERR_ARG, "bad grid (cells >= 1, lengths > 0)");
+    if (e->inst.size() == 1) {
+        /* keep the scalar settings of a single-instance engine */
+        I.time = e->inst[0].time;
+        I.dt_max = e->inst[0].dt_max;
+        I.time_end = e->inst[0].time_end;
+        I.damping = e->inst[0].damping;
+        I.mass_min = e->inst[0].mass_min;
+        I.tl = e->inst[0].tl;
+    } else if (e->inst.size() > 1) {
+        /* leaving multi-instance mode: the particle arrays belonged to the batch */
+        e->alloc_particles(0);
     }
-    GridGeom g{};
-    for (int a = 0; a < 3; a++) {
-        g.L[a] = length[a];
-        g.h[a] = length[a] / (double)cells[a];
-        g.cells[a] = cells[a];
-        g.nn[a] = cells[a] + 1;
-        g.ec[a] = cells[a] + 2;
+    I.npart = e->n;
+    e->inst.assign(1, I);
+    dfree(e->dpinst);
+    return e->build_layout();
+}
+
+int mpm_set_instances(mpm_engine *e, int32_t count, const double *lengths, const int32_t *cells,
+                      const int64_t *particle_counts) {
+    CHECK_E(e);
+    if (count < 1 || !lengths || !cells || !particle_counts) return e->fail(MPM_ERR_ARG, "bad instance list");
+    std::vector<HostInst> v(count);
+    int64_t total = 0;
+    for (int k = 0; k < count; k++) {
+        if (!make_geom(lengths + 3 * k, cells + 3 * k, v[k].g)) return e->fail(MPM_ERR_ARG, "bad grid of instance %d", k);
+        if (particle_counts[k] < 0) return e->fail(MPM_ERR_ARG, "negative particle count");
+        v[k].part_off = total;
+        v[k].npart = particle_counts[k];
+        total += particle_counts[k];
     }
-    g.nnodes = (int64_t)g.nn[0] * g.nn[1] * g.nn[2];
-    g.necells = (int64_t)g.ec[0] * g.ec[1] * g.ec[2];
-    if (g.necells >= (int64_t)0xFFFFFFFFll) return e->fail(MPM_ERR_RANGE, "grid too large: %lld cells", (long long)g.necells);
-    (void)hipStreamSynchronize(e->stream);
-    e->free_nodes();
-    e->g = g;
-    const size_t nn = (size_t)g.nnodes;
-    int rc = 0;
-    if ((rc = e->hip(hipMalloc(&e->nm, 8 * nn), "alloc node mass")) ||
-        (rc = e->hip(hipMalloc(&e->np, 24 * nn), "alloc node momentum")) ||
-        (rc = e->hip(hipMalloc(&e->nv, 24 * nn), "alloc node velocity")) ||
-        (rc = e->hip(hipMalloc(&e->nf, 24 * nn), "alloc node force")) ||
-        (rc = e->hip(hipMalloc(&e->na, 24 * nn), "alloc node f/m")) ||
-        (rc = e->hip(hipMalloc(&e->nu, 24 * nn), "alloc node p/m")) ||
-        (rc = e->hip(hipMalloc(&e->nfty, 8 * nn), "alloc node reaction")) ||
-        (rc = e->hip(hipMalloc(&e->nfric, 8 * nn), "alloc node friction")) ||
-        (rc = e->hip(hipMalloc(&e->nact, nn), "alloc node active")) ||
-        (rc = e->hip(hipMalloc(&e->nfix, nn), "alloc node fixed")) ||
-        (rc = e->hip(hipMalloc(&e->bcount, 4 * (size_t)g.necells), "alloc cell counts")) ||
-        (rc = e->hip(hipMalloc(&e->bstart, 4 * ((size_t)g.necells + 1)), "alloc cell starts")) ||
-        (rc = e->hip(hipMalloc(&e->bblock, 4 * (size_t)scan_block_count(g.necells)), "alloc scan blocks"))) {
-        e->free_nodes();
-        return rc;
+    if (total >= (int64_t)0xFFFFFFFFll) return e->fail(MPM_ERR_RANGE, "too many particles");
+    e->inst = v;
+    int rc = e->build_layout();
+    if (rc) return rc;
+    if ((rc = e->alloc_particles(total))) return rc;
+    std::vector<uint32_t> pi((size_t)total);
+    for (int k = 0; k < count; k++)
+        std::fill(pi.begin() + v[k].part_off, pi.begin() + v[k].part_off + v[k].npart, (uint32_t)k);
+    if (total > 0) {
+        if ((rc = e->hip(hipMalloc(&e->dpinst, 4 * (size_t)total), "alloc particle instances"))) return rc;
+        if ((rc = e->hip(hipMemcpy(e->dpinst, pi.data(), 4 * (size_t)total, hipMemcpyHostToDevice), "upload instances")))
+            return rc;
     }
-    if ((rc = e->hip(hipMemsetAsync(e->nact, 0, nn, e->stream), "clear active")) ||
-        (rc = e->hip(hipMemsetAsync(e->nfix, 0, nn, e->stream), "clear fixed")) ||
-        (rc = e->hip(hipMemsetAsync(e->nfric, 0, 8 * nn, e->stream), "clear friction")) ||
-        (rc = e->hip(hipMemsetAsync(e->nm, 0, 8 * nn, e->stream), "clear mass")))
-        return rc;
-    e->grid_set = true;
-    e->keys_valid = false;
-    return e->hip(hipStreamSynchronize(e->stream), "set_grid sync");
+    return MPM_OK;
+}
+
+int mpm_get_instance_layout(const mpm_engine *e, int32_t instance, int64_t *particle_offset, int64_t *node_offset,
+                            int64_t *node_count) {
+    if (!e || instance < 0 || instance >= (int32_t)e->inst.size()) return MPM_ERR_ARG;
+    const HostInst &I = e->inst[instance];
+    if (particle_offset) *particle_offset = I.part_off;
+    if (node_offset) *node_offset = I.node_off;
+    if (node_count) *node_count = I.g.nnodes;
+    return MPM_OK;
 }
 
 int mpm_set_node_boundary(mpm_engine *e, const uint8_t *fixed_mask, const double *friction, int64_t n_nodes) {
     CHECK_E(e);
-    if (!e->grid_set) return e->fail(MPM_ERR_STATE, "set_node_boundary before set_grid");
-    if (n_nodes != e->g.nnodes) return e->fail(MPM_ERR_ARG, "node count %lld != %lld", (long long)n_nodes, (long long)e->g.nnodes);
+    if (!e->layout_set) return e->fail(MPM_ERR_STATE, "set_node_boundary before set_grid");
+    if (n_nodes != e->nnodes) return e->fail(MPM_ERR_ARG, "node count %lld != %lld", (long long)n_nodes, (long long)e->nnodes);
     const size_t nn = (size_t)n_nodes;
+    (void)hipStreamSynchronize(e->stream);
     int rc;
-    std::vector<int64_t> fy;
+    e->fixed_y.clear();
     if (fixed_mask) {
         if ((rc = e->hip(hipMemcpy(e->nfix, fixed_mask, nn, hipMemcpyHostToDevice), "upload fixed"))) return rc;
         for (size_t i = 0; i < nn; i++)
-            if (fixed_mask[i] & 2) fy.push_back((int64_t)i);
+            if (fixed_mask[i] & 2) e->fixed_y.push_back((int64_t)i);
     } else if ((rc = e->hip(hipMemset(e->nfix, 0, nn), "clear fixed"))) {
         return rc;
     }
@@ -382,13 +672,6 @@ int mpm_set_node_boundary(mpm_engine *e, const uint8_t *fixed_mask, const double
         if ((rc = e->hip(hipMemcpy(e->nfric, friction, 8 * nn, hipMemcpyHostToDevice), "upload friction"))) return rc;
     } else if ((rc = e->hip(hipMemset(e->nfric, 0, 8 * nn), "clear friction"))) {
         return rc;
-    }
-    dfree(e->d_fixed_y);
-    e->nfy = (int64_t)fy.size();
-    if (!fy.empty()) {
-        if ((rc = e->hip(hipMalloc(&e->d_fixed_y, 8 * fy.size()), "alloc fixed-y list"))) return rc;
-        if ((rc = e->hip(hipMemcpy(e->d_fixed_y, fy.data(), 8 * fy.size(), hipMemcpyHostToDevice), "upload fixed-y list")))
-            return rc;
     }
     return MPM_OK;
 }
@@ -407,52 +690,25 @@ int mpm_set_materials(mpm_engine *e, const mpm_material *materials, int32_t n) {
         mt.c0[i] = materials[i].hardening_c0;
         mt.c1[i] = materials[i].hardening_c1;
     }
+    (void)hipStreamSynchronize(e->stream);
     e->mt = mt;
+    e->drop_graphs(); /* graphs hold the material table by value */
     return MPM_OK;
 }
 
 int mpm_set_particle_count(mpm_engine *e, int64_t n) {
     CHECK_E(e);
     if (n < 0 || n >= (int64_t)0xFFFFFFFFll) return e->fail(MPM_ERR_RANGE, "particle count %lld unsupported", (long long)n);
-    (void)hipStreamSynchronize(e->stream);
-    e->free_particles();
-    e->n = n;
-    e->has_fext = false;
-    e->keys_valid = false;
-    if (n == 0) return MPM_OK;
-    const size_t sn = (size_t)n;
-    int rc;
-    for (int f = 0; f < MPM_PF_COUNT; f++) {
-        if ((rc = e->hip(hipMalloc(&e->pf[f], 8 * sn * kFieldComponents[f]), "alloc particle field"))) {
-            e->free_particles();
-            e->n = 0;
-            return rc;
-        }
-        if ((rc = e->hip(hipMemset(e->pf[f], 0, 8 * sn * kFieldComponents[f]), "clear particle field")))
-            return rc;
+    if (e->inst.size() > 1) {
+        if (n != e->n) return e->fail(MPM_ERR_ARG, "a multi-instance engine's particle count is set by set_instances");
+        return MPM_OK;
     }
-    if ((rc = e->hip(hipMalloc(&e->pmat, 4 * sn), "alloc material ids")) ||
-        (rc = e->hip(hipMalloc(&e->pdc_rank, 4 * sn), "alloc control ranks")) ||
-        (rc = e->hip(hipMalloc(&e->pdc_axes, sn), "alloc control axes")) ||
-        (rc = e->hip(hipMalloc(&e->pdc_mult, 4 * sn), "alloc control multipliers")) ||
-        (rc = e->hip(hipMalloc(&e->bkey, 4 * sn), "alloc keys")) ||
-        (rc = e->hip(hipMalloc(&e->bsorted, 4 * sn), "alloc sorted ids"))) {
-        e->free_particles();
-        e->n = 0;
-        return rc;
+    int rc = e->alloc_particles(n);
+    if (!rc && !e->inst.empty()) {
+        e->inst[0].part_off = 0;
+        e->inst[0].npart = n;
     }
-    /* defaults of MaterialPoint_BC (MaterialPoint_BC.h:31-82) */
-    std::vector<double> F(9 * sn, 0.0);
-    for (int d = 0; d < 3; d++) std::fill(F.begin() + (4 * d) * sn, F.begin() + (4 * d + 1) * sn, 1.0);
-    std::vector<int32_t> minus1(sn, -1);
-    std::vector<float> one(sn, 1.0f);
-    if ((rc = e->hip(hipMemcpy(e->pf[MPM_PF_DEFORMATION_GRADIENT], F.data(), 8 * 9 * sn, hipMemcpyHostToDevice), "init F")) ||
-        (rc = e->hip(hipMemcpy(e->pmat, minus1.data(), 4 * sn, hipMemcpyHostToDevice), "init materials")) ||
-        (rc = e->hip(hipMemcpy(e->pdc_rank, minus1.data(), 4 * sn, hipMemcpyHostToDevice), "init control ranks")) ||
-        (rc = e->hip(hipMemset(e->pdc_axes, 0, sn), "init control axes")) ||
-        (rc = e->hip(hipMemcpy(e->pdc_mult, one.data(), 4 * sn, hipMemcpyHostToDevice), "init multipliers")))
-        return rc;
-    return e->hip(hipStreamSynchronize(e->stream), "particle init sync");
+    return rc;
 }
 
 int mpm_get_particle_count(const mpm_engine *e, int64_t *n) {
@@ -476,6 +732,7 @@ int mpm_set_particle_field(mpm_engine *e, int32_t field, const double *src, int6
         /* N * 0 added to a sum never changes it, so an all-zero field is skipped */
         bool any = false;
         for (size_t i = 0; i < 3 * (size_t)n && !any; i++) any = (src[i] != 0.0);
+        if (any != e->has_fext) e->table_dirty = true;
         e->has_fext = any;
     }
     return MPM_OK;
@@ -533,60 +790,98 @@ int mpm_set_monitor_list(mpm_engine *e, int32_t kind, const int64_t *index, int6
     if (count < 0 || (count > 0 && !index)) return e->fail(MPM_ERR_ARG, "bad monitor list");
     for (int64_t k = 0; k < count; k++)
         if (index[k] < 0 || index[k] >= e->n) return e->fail(MPM_ERR_ARG, "monitor index out of range");
-    (void)hipStreamSynchronize(e->stream);
-    dfree(e->d_mon[kind]);
-    e->mon_count[kind] = count;
-    if (count == 0) return MPM_OK;
-    int rc = e->hip(hipMalloc(&e->d_mon[kind], 8 * (size_t)count), "alloc monitor list");
-    if (rc) return rc;
-    return e->hip(hipMemcpy(e->d_mon[kind], index, 8 * (size_t)count, hipMemcpyHostToDevice), "upload monitor list");
+    e->mon[kind].assign(index, index + count);
+    return MPM_OK;
+}
+
+int mpm_set_instance_timeline(mpm_engine *e, int32_t instance, const double *time, const double *v3, int32_t n) {
+    if (!e) return MPM_ERR_ARG;
+    if (instance < 0 || instance >= (int32_t)e->inst.size()) return e->fail(MPM_ERR_ARG, "bad instance");
+    if (n < 0 || (n > 0 && (!time || !v3))) return e->fail(MPM_ERR_ARG, "bad timeline");
+    std::vector<double> tl(4 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        tl[4 * i] = time[i];
+        tl[4 * i + 1] = v3[3 * i];
+        tl[4 * i + 2] = v3[3 * i + 1];
+        tl[4 * i + 3] = v3[3 * i + 2];
+    }
+    e->inst[instance].tl = tl;
+    e->table_dirty = true;
+    return MPM_OK;
 }
 
 int mpm_set_timeline(mpm_engine *e, const double *time, const double *v3, int32_t n) {
     if (!e) return MPM_ERR_ARG;
-    if (n < 0 || (n > 0 && (!time || !v3))) return e->fail(MPM_ERR_ARG, "bad timeline");
-    e->timeline.t.assign(time, time + n);
-    e->timeline.v.assign(v3, v3 + 3 * (size_t)n);
-    return MPM_OK;
-}
-
-int mpm_set_time(mpm_engine *e, double time, double dt_max, double time_end) {
-    if (!e) return MPM_ERR_ARG;
-    e->time = time;
-    e->dt_max = dt_max;
-    e->time_end = time_end;
-    return MPM_OK;
-}
-
-int mpm_set_damping(mpm_engine *e, double damping, double mass_minimum) {
-    if (!e) return MPM_ERR_ARG;
-    e->damping = damping;
-    e->mass_min = mass_minimum;
-    return MPM_OK;
-}
-
-int mpm_step(mpm_engine *e, double dt, int32_t nsteps) {
-    CHECK_E(e);
-    if (!e->grid_set) return e->fail(MPM_ERR_STATE, "step before set_grid");
-    if (nsteps < 0) return e->fail(MPM_ERR_ARG, "negative step count");
-    for (int s = 0; s < nsteps; s++) {
-        int rc = e->step_once(dt);
+    if (e->inst.empty()) return e->fail(MPM_ERR_STATE, "set_timeline before set_grid");
+    for (int32_t k = 0; k < (int32_t)e->inst.size(); k++) {
+        int rc = mpm_set_instance_timeline(e, k, time, v3, n);
         if (rc) return rc;
     }
     return MPM_OK;
 }
 
+int mpm_set_instance_time(mpm_engine *e, int32_t instance, double time, double dt_max, double time_end) {
+    if (!e) return MPM_ERR_ARG;
+    if (instance < 0 || instance >= (int32_t)e->inst.size()) return e->fail(MPM_ERR_ARG, "bad instance");
+    e->inst[instance].time = time;
+    e->inst[instance].dt_max = dt_max;
+    e->inst[instance].time_end = time_end;
+    e->table_dirty = true;
+    return MPM_OK;
+}
+
+int mpm_set_time(mpm_engine *e, double time, double dt_max, double time_end) {
+    if (!e) return MPM_ERR_ARG;
+    if (e->inst.empty()) return e->fail(MPM_ERR_STATE, "set_time before set_grid");
+    for (auto &I : e->inst) {
+        I.time = time;
+        I.dt_max = dt_max;
+        I.time_end = time_end;
+    }
+    e->table_dirty = true;
+    return MPM_OK;
+}
+
+int mpm_set_damping(mpm_engine *e, double damping, double mass_minimum) {
+    if (!e) return MPM_ERR_ARG;
+    if (e->inst.empty()) return e->fail(MPM_ERR_STATE, "set_damping before set_grid");
+    for (auto &I : e->inst) {
+        I.damping = damping;
+        I.mass_min = mass_minimum;
+    }
+    e->table_dirty = true;
+    return MPM_OK;
+}
+
+int mpm_step(mpm_engine *e, double dt, int32_t nsteps) {
+    CHECK_E(e);
+    if (nsteps < 0) return e->fail(MPM_ERR_ARG, "negative step count");
+    if (e->inst.size() > 1 && dt > 0.0)
+        return e->fail(MPM_ERR_ARG, "a multi-instance engine steps each instance by its own dt_max (pass dt <= 0)");
+    for (int s = 0; s < nsteps; s++) {
+        int rc = e->step_once(dt > 0.0 ? &dt : nullptr);
+        if (rc) return rc;
+    }
+    return MPM_OK;
+}
+
+int mpm_step_graph(mpm_engine *e, int32_t nsteps) {
+    CHECK_E(e);
+    if (nsteps < 1 || nsteps > 100000) return e->fail(MPM_ERR_ARG, "graph step count out of range");
+    return e->step_graph(nsteps);
+}
+
 int mpm_step_timed(mpm_engine *e, double dt, int32_t nsteps, double phase_ms[MPM_PHASE_COUNT]) {
     CHECK_E(e);
-    if (!e->grid_set) return e->fail(MPM_ERR_STATE, "step before set_grid");
     if (nsteps < 0 || !phase_ms) return e->fail(MPM_ERR_ARG, "bad timed-step arguments");
+    if (e->inst.size() > 1 && dt > 0.0) return e->fail(MPM_ERR_ARG, "multi-instance engines take dt <= 0");
     static_assert(MPM_PHASE_COUNT == PHASE_COUNT, "phase enums disagree");
     for (int k = 0; k < MPM_PHASE_COUNT; k++) phase_ms[k] = 0.0;
     hipEvent_t ev[PHASE_COUNT + 1] = {};
     int rc = MPM_OK;
     for (int k = 0; k <= PHASE_COUNT && !rc; k++) rc = e->hip(hipEventCreate(&ev[k]), "event create");
     for (int s = 0; s < nsteps && !rc; s++) {
-        rc = e->step_once(dt, ev);
+        rc = e->step_once(dt > 0.0 ? &dt : nullptr, ev);
         if (!rc) rc = e->hip(hipEventSynchronize(ev[PHASE_COUNT]), "event sync");
         for (int k = 0; k < PHASE_COUNT && !rc; k++) {
             float ms = 0.f;
@@ -601,16 +896,16 @@ int mpm_step_timed(mpm_engine *e, double dt, int32_t nsteps, double phase_ms[MPM
 
 int mpm_run(mpm_engine *e, double dt_total, int32_t *finished) {
     CHECK_E(e);
-    if (!e->grid_set) return e->fail(MPM_ERR_STATE, "run before set_grid");
+    if (e->inst.size() != 1) return e->fail(MPM_ERR_STATE, "mpm_run drives a single-instance engine");
     double acc = 0.0;
     while (acc < dt_total) {
-        double dt = e->dt_max;
-        if (e->dt_max > (dt_total - acc)) dt = dt_total - acc;
+        double dt = e->inst[0].dt_max;
+        if (e->inst[0].dt_max > (dt_total - acc)) dt = dt_total - acc;
         acc += dt;
-        int rc = e->step_once(dt);
+        int rc = e->step_once(&dt);
         if (rc) return rc;
     }
-    if (finished) *finished = (e->time < e->time_end) ? 0 : 1;
+    if (finished) *finished = (e->inst[0].time < e->inst[0].time_end) ? 0 : 1;
     return MPM_OK;
 }
 
@@ -621,22 +916,28 @@ int mpm_synchronize(mpm_engine *e) {
 
 int mpm_get_time(const mpm_engine *e, double *time, int64_t *cycle) {
     if (!e) return MPM_ERR_ARG;
-    if (time) *time = e->time;
+    if (time) *time = e->inst.empty() ? 0.0 : e->inst[0].time;
     if (cycle) *cycle = e->cycle;
+    return MPM_OK;
+}
+
+int mpm_get_instance_time(const mpm_engine *e, int32_t instance, double *time) {
+    if (!e || !time || instance < 0 || instance >= (int32_t)e->inst.size()) return MPM_ERR_ARG;
+    *time = e->inst[instance].time;
     return MPM_OK;
 }
 
 int mpm_get_node_count(const mpm_engine *e, int64_t *n) {
     if (!e || !n) return MPM_ERR_ARG;
-    *n = e->grid_set ? e->g.nnodes : 0;
+    *n = e->layout_set ? e->nnodes : 0;
     return MPM_OK;
 }
 
 int mpm_get_node_field(mpm_engine *e, int32_t field, double *dst, int64_t n) {
     CHECK_E(e);
-    if (!e->grid_set) return e->fail(MPM_ERR_STATE, "no grid");
+    if (!e->layout_set) return e->fail(MPM_ERR_STATE, "no grid");
     if (field < 0 || field >= MPM_NF_COUNT) return e->fail(MPM_ERR_ARG, "unknown node field %d", field);
-    if (n != e->g.nnodes || !dst) return e->fail(MPM_ERR_ARG, "node field length mismatch");
+    if (n != e->nnodes || !dst) return e->fail(MPM_ERR_ARG, "node field length mismatch");
     const int comps = (field == MPM_NF_MASS || field == MPM_NF_ACTIVE) ? 1 : 3;
     double *tmp = nullptr;
     size_t bytes = 8 * (size_t)n * comps;
@@ -652,39 +953,15 @@ int mpm_get_node_field(mpm_engine *e, int32_t field, double *dst, int64_t n) {
 int mpm_get_monitors(mpm_engine *e, mpm_monitors *out) {
     CHECK_E(e);
     if (!out) return e->fail(MPM_ERR_ARG, "null monitor output");
-    if (!e->grid_set) return e->fail(MPM_ERR_STATE, "no grid");
-    std::memset(out, 0, sizeof(*out));
-    out->time = e->time;
-    MonLists ml;
-    for (int k = 0; k < MPM_MON_COUNT; k++) {
-        ml.list[k] = e->d_mon[k];
-        ml.count[k] = e->mon_count[k];
-    }
-    ml.fixed_y = e->d_fixed_y;
-    ml.nfy = e->nfy;
-    int rc = e->hip(launch_monitors(e->g, e->pview(), e->nview(), ml, e->d_mon_out, e->stream), "monitors");
-    if (rc) return rc;
-    double v[8];
-    if ((rc = e->hip(hipMemcpyAsync(v, e->d_mon_out, sizeof(v), hipMemcpyDeviceToHost, e->stream), "monitor copy")))
-        return rc;
-    if ((rc = e->hip(hipStreamSynchronize(e->stream), "monitor sync"))) return rc;
-    out->has_position = e->mon_count[MPM_MON_DISPLACEMENT] > 0;
-    out->has_momentum = e->mon_count[MPM_MON_MOMENTUM] > 0;
-    out->has_energy = e->mon_count[MPM_MON_ENERGY] > 0;
-    out->has_stress = e->mon_count[MPM_MON_STRESS] > 0;
-    out->position_y = out->has_position ? v[0] : 0.0;
-    out->force_y = v[1];
-    out->momentum_y = out->has_momentum ? v[2] : 0.0;
-    if (out->has_energy) {
-        out->kinetic_energy = v[3];
-        out->strain_energy = v[4];
-        out->plastic_energy = v[5];
-    }
-    if (out->has_stress) {
-        out->stress_y = v[6];
-        out->strain_y = v[7];
-    }
-    return MPM_OK;
+    if (!e->layout_set) return e->fail(MPM_ERR_STATE, "no grid");
+    return e->monitors(0, e->n, 0, e->nnodes, e->inst[0].time, out);
+}
+
+int mpm_get_instance_monitors(mpm_engine *e, int32_t instance, mpm_monitors *out) {
+    CHECK_E(e);
+    if (!out || instance < 0 || instance >= (int32_t)e->inst.size()) return e->fail(MPM_ERR_ARG, "bad instance");
+    const HostInst &I = e->inst[instance];
+    return e->monitors(I.part_off, I.part_off + I.npart, I.node_off, I.node_off + I.g.nnodes, I.time, out);
 }
 
 int mpm_get_health(mpm_engine *e, mpm_health *out) {
@@ -704,6 +981,14 @@ int mpm_get_health(mpm_engine *e, mpm_health *out) {
     return MPM_OK;
 }
 
+int mpm_reset_health(mpm_engine *e) {
+    CHECK_E(e);
+    e->steps = 0;
+    int rc = e->hip(hipStreamSynchronize(e->stream), "health sync");
+    if (rc) return rc;
+    return e->hip(hipMemset(e->d_health, 0, sizeof(unsigned long long) * HEALTH_SLOTS), "health reset");
+}
+
 int mpm_set_surrogate(mpm_engine *e, mpm_surrogate *s, int32_t material, double residual_tolerance) {
     CHECK_E(e);
     if (s && s->device != e->device) return e->fail(MPM_ERR_ARG, "surrogate lives on another device");
@@ -711,6 +996,7 @@ int mpm_set_surrogate(mpm_engine *e, mpm_surrogate *s, int32_t material, double 
     if (s && !(residual_tolerance >= 0.0)) return e->fail(MPM_ERR_ARG, "negative residual tolerance");
     (void)hipStreamSynchronize(e->stream);
     if (s) (void)hipStreamSynchronize(s->stream);
+    e->drop_graphs();
     e->sur = s;
     e->sur_material = s ? material : -1;
     e->sur_tol = residual_tolerance;
@@ -722,6 +1008,7 @@ int mpm_set_sample_capture(mpm_engine *e, int64_t capacity) {
     CHECK_E(e);
     if (capacity < 0) return e->fail(MPM_ERR_ARG, "negative capture capacity");
     (void)hipStreamSynchronize(e->stream);
+    e->drop_graphs();
     dfree(e->cap);
     e->capcap = 0;
     int rc = e->hip(hipMemset(e->capcount, 0, sizeof(unsigned long long)), "capture reset");
@@ -744,14 +1031,6 @@ int mpm_get_captured_samples(mpm_engine *e, double *rows, int64_t capacity, int6
         rc = e->hip(hipMemcpy(rows, e->cap, sizeof(double) * 14 * (size_t)std::min(have, capacity), hipMemcpyDeviceToHost),
                     "capture copy");
     return rc;
-}
-
-int mpm_reset_health(mpm_engine *e) {
-    CHECK_E(e);
-    e->steps = 0;
-    int rc = e->hip(hipStreamSynchronize(e->stream), "health sync");
-    if (rc) return rc;
-    return e->hip(hipMemset(e->d_health, 0, sizeof(unsigned long long) * HEALTH_SLOTS), "health reset");
 }
 
 int mpm_constitutive_batch(int32_t device, const mpm_material *material, const double *backstress, const double *stress,

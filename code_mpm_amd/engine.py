@@ -121,6 +121,87 @@ class Engine:
         self.console_interval = scene.console_interval
         self.console_last = 0.0
 
+    def load_batch(self, scenes_):
+        """Upload several independent scenes as one batch: one set of launches
+        per step for all of them (the BESO driver's stream of independent
+        worlds, Main.cpp:209-336, run side by side).  Materials equal by value
+        are shared; control and monitor lists are offset to global indices."""
+        scenes_ = list(scenes_)
+        L = np.ascontiguousarray([sc.length for sc in scenes_], np.float64).ravel()
+        C = np.ascontiguousarray([sc.cells for sc in scenes_], np.int32).ravel()
+        N = np.ascontiguousarray([sc.n for sc in scenes_], np.int64)
+        self._check(self._lib.mpm_set_instances(self._h, len(scenes_), ptr(L), ptr(C), ptr(N)), "set_instances")
+        n = ctypes.c_int64()
+        self._lib.mpm_get_node_count(self._h, ctypes.byref(n))
+        self.nnodes = n.value
+        self.n = int(N.sum())
+        mats, key_of = [], {}
+        mat_ids = []
+        for sc in scenes_:
+            remap = []
+            for m in sc.materials:
+                key = tuple(sorted((k, float(v)) for k, v in m.items()))
+                if key not in key_of:
+                    key_of[key] = len(mats)
+                    mats.append(m)
+                remap.append(key_of[key])
+            remap = np.asarray(remap + [-1], np.int32)
+            mat_ids.append(np.where(sc.material >= 0, remap[np.clip(sc.material, 0, len(remap) - 2)], -1))
+        self.set_materials(mats)
+        self.set_node_boundary(np.concatenate([sc.fixed for sc in scenes_]),
+                               np.concatenate([sc.friction for sc in scenes_]))
+        names = set()
+        for sc in scenes_:
+            names |= set(sc.fields)
+        from .capi import PF as _PF
+        for name in names:
+            nc = _PF[name][1]
+            parts = []
+            for sc in scenes_:
+                if name in sc.fields:
+                    parts.append(np.asarray(sc.fields[name], np.float64).reshape(nc, sc.n))
+                else:
+                    d = np.zeros((nc, sc.n))
+                    if name == "DEFORMATION_GRADIENT":
+                        d[[0, 4, 8]] = 1.0
+                    parts.append(d)
+            self.set_particle(name, np.concatenate(parts, axis=1))
+        self.set_particle_material(np.concatenate(mat_ids).astype(np.int32))
+        offs = np.concatenate([[0], np.cumsum(N)[:-1]])
+        self.set_displacement_control(np.concatenate([sc.dc_index + o for sc, o in zip(scenes_, offs)]),
+                                      np.concatenate([sc.dc_axes for sc in scenes_]),
+                                      np.concatenate([sc.dc_mult for sc in scenes_]))
+        kinds = set()
+        for sc in scenes_:
+            kinds |= set(sc.monitors)
+        for kind in kinds:
+            self.set_monitor_list(kind, np.concatenate(
+                [np.asarray(sc.monitors.get(kind, []), np.int64) + o for sc, o in zip(scenes_, offs)]))
+        for k, sc in enumerate(scenes_):
+            t = np.ascontiguousarray(sc.timeline_t, np.float64)
+            v = np.ascontiguousarray(sc.timeline_v, np.float64).reshape(-1, 3)
+            self._check(self._lib.mpm_set_instance_timeline(self._h, k, ptr(t), ptr(v), len(t)), "instance timeline")
+            self._check(self._lib.mpm_set_instance_time(self._h, k, float(sc.time0), float(sc.dt), float(sc.time_end)),
+                        "instance time")
+        self.set_damping(scenes_[0].damping, scenes_[0].mass_min)
+        self.batch = scenes_
+        self.scene = None
+        return offs
+
+    def instance_monitors(self, k):
+        m = capi.Monitors()
+        self._check(self._lib.mpm_get_instance_monitors(self._h, int(k), ctypes.byref(m)), "instance monitors")
+        return m.as_dict()
+
+    def instance_time(self, k):
+        t = ctypes.c_double()
+        self._check(self._lib.mpm_get_instance_time(self._h, int(k), ctypes.byref(t)), "instance time")
+        return t.value
+
+    def step_graph(self, nsteps):
+        """nsteps steps replayed from a HIP graph (recorded on first use)."""
+        self._check(self._lib.mpm_step_graph(self._h, int(nsteps)), "step_graph")
+
     # -- the step -------------------------------------------------------------
     def step(self, dt, nsteps=1):
         self._check(self._lib.mpm_step(self._h, float(dt), int(nsteps)), "step")

@@ -191,12 +191,42 @@ int mpm_set_time(mpm_engine *e, double time, double dt_max, double time_end);
 /* d_DampingCoefficient and d_Mass_Minimum (PhysicsEngine.h:83-84). */
 int mpm_set_damping(mpm_engine *e, double damping, double mass_minimum);
 
+/* --- batches of independent instances (no counterpart in the reference) ---- */
+
+/* Replaces the grid and particle set by `count` independent instances, each
+ * a separate world as set up by one initializeWorld_* call (the BESO driver
+ * builds one engine per iteration, Main.cpp:227-232).  Instance k has grid
+ * lengths[3k..3k+2] / cells[3k..3k+2] and particle_counts[k] particles; all
+ * instances share the particle, node and material arrays back to back
+ * (particles of instance k start at the sum of the earlier counts, nodes
+ * likewise, see mpm_get_instance_layout) and step together, one launch per
+ * phase for all of them.  Particle, node-boundary and control uploads then
+ * take the concatenated arrays; control and monitor lists use global
+ * particle indices. */
+int mpm_set_instances(mpm_engine *e, int32_t count, const double *lengths, const int32_t *cells,
+                      const int64_t *particle_counts);
+int mpm_get_instance_layout(const mpm_engine *e, int32_t instance, int64_t *particle_offset, int64_t *node_offset,
+                            int64_t *node_count);
+/* Per-instance TimeLine, d_Time / d_TimeIncrement_Maximum / d_TimeEnd. */
+int mpm_set_instance_timeline(mpm_engine *e, int32_t instance, const double *time, const double *v3, int32_t n);
+int mpm_set_instance_time(mpm_engine *e, int32_t instance, double time, double dt_max, double time_end);
+int mpm_get_instance_time(const mpm_engine *e, int32_t instance, double *time);
+/* reportConsole values of one instance. */
+int mpm_get_instance_monitors(mpm_engine *e, int32_t instance, mpm_monitors *out);
+
 /* --- the hot path ---------------------------------------------------------- */
 
+/* nsteps steps; each instance advances by its own dt_max.  The first call
+ * for a given nsteps records the steps' kernels in a HIP graph (time and
+ * the timeline velocity advance on the device, so no host work is needed
+ * between steps); later calls replay it.  Set-up calls invalidate it. */
+int mpm_step_graph(mpm_engine *e, int32_t nsteps);
 /* nsteps steps of size dt, each the body of the while loop of
  * runSimulation_Classic_DoublePass_MPLocks
  * (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:462-803):
- * time += dt and cycle += 1 after each. Asynchronous on the engine's stream. */
+ * time += dt and cycle += 1 after each. Asynchronous on the engine's stream.
+ * A multi-instance engine takes dt <= 0 and steps each instance by its own
+ * dt_max. */
 int mpm_step(mpm_engine *e, double dt, int32_t nsteps);
 /* runSimulation_Classic_DoublePass_MPLocks(dTimeIncrement_Total)
  * (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:439-815)
