@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--order", default="cell", choices=["cell", "particle"],
+                    help="node summation order (cell: fast, particle: the reference's one-thread order)")
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="first run one device copy of this many GiB (the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -102,7 +104,7 @@ def main():
 
     t_setup = time.perf_counter()
     scene = scenes.honeycomb_crush()
-    eng = Engine(device=0)
+    eng = Engine(device=0, order=args.order)
     eng.load_scene(scene)
     print(f"# scene {scene.name}: {scene.n} particles, setup {time.perf_counter() - t_setup:.1f} s",
           file=sys.stderr, flush=True)
@@ -154,7 +156,7 @@ def main():
         "config": {"workload": "J2 honeycomb crush, 512x512x32 cells, 8.0M particles, exact FP64 return map "
                                "(von Mises hardening), platen 10 m/s, seed 1",
                    "particles": scene.n, "active_nodes": active, "plastic_fraction": round(plastic_frac, 5),
-                   "return_map": "exact"},
+                   "return_map": "exact", "summation_order": args.order},
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAME[dominant], "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes": dom_bytes,

@@ -25,6 +25,7 @@ PF = dict(
 )
 NF = dict(MASS=(0, 1), MOMENTUM=(1, 3), VELOCITY=(2, 3), FORCE=(3, 3), FORCE_TEMP=(4, 3), ACTIVE=(5, 1))
 MON = dict(DISPLACEMENT=0, MOMENTUM=1, ENERGY=2, STRESS=3)
+ORDER = dict(particle=0, cell=1)
 PHASES = ("bin", "p2g_grid", "g2p_velocity", "remap_grid", "g2p_strain")
 
 
@@ -106,6 +107,7 @@ SIGNATURES = {
     "mpm_set_damping": (I32, [P, DBL, DBL]),
     "mpm_step": (I32, [P, DBL, I32]),
     "mpm_step_graph": (I32, [P, I32]),
+    "mpm_set_summation_order": (I32, [P, I32]),
     "mpm_set_instances": (I32, [P, I32, P, P, P]),
     "mpm_get_instance_layout": (I32, [P, I32, PI64, PI64, PI64]),
     "mpm_set_instance_timeline": (I32, [P, I32, P, P, I32]),

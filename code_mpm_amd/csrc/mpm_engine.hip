@@ -99,8 +99,15 @@ struct mpm_engine {
     bool has_fext = false;
 
     /* bins */
-    uint32_t *bkey = nullptr, *bcount = nullptr, *bstart = nullptr, *bsorted = nullptr, *bblock = nullptr;
+    uint32_t *bkey = nullptr, *bcount = nullptr, *bstart = nullptr, *bsorted = nullptr;
+    uint32_t *bslot = nullptr, *bocc = nullptr, *bnocc = nullptr;
+    uint64_t *bblock = nullptr;
     bool keys_valid = false;
+
+    /* node summation order and the cell-order partial sums */
+    int order = ORDER_CELL;
+    CellParts parts{};
+    int64_t parts_cap = 0;
 
     /* materials, lists */
     MatTable mt{};
@@ -151,8 +158,49 @@ struct mpm_engine {
         dfree(nm); dfree(np); dfree(nv); dfree(nf); dfree(na); dfree(nu); dfree(nfty); dfree(nfric);
         dfree(nact); dfree(nfix);
         fixed_y.clear();
-        dfree(bcount); dfree(bstart); dfree(bblock);
+        dfree(bcount); dfree(bstart); dfree(bblock); dfree(bslot); dfree(bnocc);
         dfree(dinst); dfree(dtile_off); dfree(dtl);
+        free_parts();
+    }
+    void free_parts() {
+        dfree(parts.m); dfree(parts.p); dfree(parts.f); dfree(parts.dc); dfree(bocc);
+        parts = CellParts{};
+        parts_cap = 0;
+        occ_cap = 0;
+    }
+    /* The occupied-cell list written by every scan: occupied cells never
+     * exceed min(cells, particles). */
+    int64_t occ_cap = 0;
+    int ensure_occ() {
+        int64_t cap = std::max<int64_t>(1, std::min<int64_t>(ncells, n));
+        if (cap == occ_cap && bocc) return MPM_OK;
+        dfree(bocc);
+        occ_cap = 0;
+        int rc = hip(hipMalloc(&bocc, 4 * (size_t)cap), "alloc occupied cells");
+        if (rc) return rc;
+        occ_cap = cap;
+        drop_graphs();
+        return MPM_OK;
+    }
+    int ensure_parts() {
+        int64_t cap = occ_cap;
+        if (cap == parts_cap && parts.m) return MPM_OK;
+        dfree(parts.m); dfree(parts.p); dfree(parts.f); dfree(parts.dc);
+        parts = CellParts{};
+        parts_cap = 0;
+        const size_t c8 = 8 * (size_t)cap;
+        int rc;
+        if ((rc = hip(hipMalloc(&parts.m, 8 * c8), "alloc partials")) ||
+            (rc = hip(hipMalloc(&parts.p, 24 * c8), "alloc partials")) ||
+            (rc = hip(hipMalloc(&parts.f, 24 * c8), "alloc partials")) ||
+            (rc = hip(hipMalloc(&parts.dc, 12 * c8), "alloc partials"))) {
+            free_parts();
+            return rc;
+        }
+        parts.cap8 = (int64_t)c8;
+        parts_cap = cap;
+        drop_graphs();
+        return MPM_OK;
     }
     void free_particles() {
         for (int f = 0; f < MPM_PF_COUNT; f++) dfree(pf[f]);
@@ -228,6 +276,7 @@ struct mpm_engine {
     BView bview() const {
         BView B;
         B.key = bkey; B.count = bcount; B.start = bstart; B.sorted = bsorted;
+        B.slot = bslot; B.occ = bocc; B.nocc = bnocc;
         B.necells = ncells;
         return B;
     }
@@ -276,7 +325,9 @@ struct mpm_engine {
             (rc = hip(hipMalloc(&nfix, nn), "alloc node fixed")) ||
             (rc = hip(hipMalloc(&bcount, 4 * (size_t)ncells), "alloc cell counts")) ||
             (rc = hip(hipMalloc(&bstart, 4 * ((size_t)ncells + 1)), "alloc cell starts")) ||
-            (rc = hip(hipMalloc(&bblock, 4 * (size_t)scan_block_count(ncells)), "alloc scan blocks")) ||
+            (rc = hip(hipMalloc(&bblock, 8 * (size_t)scan_block_count(ncells)), "alloc scan blocks")) ||
+            (rc = hip(hipMalloc(&bslot, 4 * (size_t)ncells), "alloc cell slots")) ||
+            (rc = hip(hipMalloc(&bnocc, 4), "alloc occupied count")) ||
             (rc = hip(hipMalloc(&dinst, sizeof(Inst) * inst.size()), "alloc instance table")) ||
             (rc = hip(hipMalloc(&dtile_off, 4 * (inst.size() + 1)), "alloc tile offsets"))) {
             free_nodes();
@@ -359,7 +410,8 @@ struct mpm_engine {
             if (ev && (rc = hip(hipEventRecord(ev[ph], stream), "event"))) return rc;
             if (ph == PHASE_G2P_STRAIN && sur && (rc = hip(launch_zero_u32(qcount, 1, stream), "queue reset")))
                 return rc;
-            if ((rc = hip(launch_phase(ph, b, pview(), nview(), bview(), mt, d_health, sa, stream), "step kernels")))
+            if ((rc = hip(launch_phase(ph, b, pview(), nview(), bview(), parts, order, mt, d_health, sa, stream),
+                          "step kernels")))
                 return rc;
         }
         if (sur) {
@@ -377,6 +429,8 @@ struct mpm_engine {
         int rc;
         if ((rc = upload_table())) return rc;
         if (sur && (rc = ensure_queue())) return rc;
+        if ((rc = ensure_occ())) return rc;
+        if (order == ORDER_CELL && (rc = ensure_parts())) return rc;
         return ensure_keys();
     }
 
@@ -488,6 +542,7 @@ struct mpm_engine {
         drop_graphs();
         free_particles();
         free_queue();
+        free_parts();
         n = count;
         has_fext = false;
         keys_valid = false;
@@ -862,6 +917,15 @@ int mpm_step(mpm_engine *e, double dt, int32_t nsteps) {
         int rc = e->step_once(dt > 0.0 ? &dt : nullptr);
         if (rc) return rc;
     }
+    return MPM_OK;
+}
+
+int mpm_set_summation_order(mpm_engine *e, int32_t order) {
+    CHECK_E(e);
+    if (order != MPM_ORDER_PARTICLE && order != MPM_ORDER_CELL) return e->fail(MPM_ERR_ARG, "unknown order %d", order);
+    (void)hipStreamSynchronize(e->stream);
+    e->drop_graphs();
+    e->order = order == MPM_ORDER_CELL ? ORDER_CELL : ORDER_PARTICLE;
     return MPM_OK;
 }
 

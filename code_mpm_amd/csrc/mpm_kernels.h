@@ -65,13 +65,30 @@ struct NView {
     double *friction;
 };
 
-/* Cell binning: key per particle, count / start per ext cell, ids sorted by cell. */
+/* Cell binning: key per particle, count / start per ext cell, ids sorted by
+ * cell; slot = rank of an occupied cell among occupied cells, occ = the
+ * occupied cells in ascending order, nocc = their number (device). */
 struct BView {
     uint32_t *key;
     uint32_t *count;
     uint32_t *start;   /* necells + 1 */
     uint32_t *sorted;
+    uint32_t *slot;    /* necells, valid for occupied cells */
+    uint32_t *occ;     /* up to min(necells, particles) */
+    uint32_t *nocc;
     int64_t necells;
+};
+
+/* Node summation order. */
+enum { ORDER_PARTICLE = 0, ORDER_CELL = 1 };
+
+/* Per (occupied cell, corner) partial sums of the cell order, component
+ * stride cap8 = 8 x occupied-cell capacity: P2G m, p3, f3; the remap reuses
+ * f for velocity; dc = id of the latest controlled particle per axis. */
+struct CellParts {
+    double *m, *p, *f;
+    uint32_t *dc;
+    int64_t cap8;
 };
 
 /* One instance of a batch (a single scene is a batch of one): geometry,
@@ -108,12 +125,13 @@ struct MonLists {
 
 hipError_t launch_keys(const Batch &bt, const PView &P, const BView &B, hipStream_t s);
 int64_t scan_block_count(int64_t necells);
-hipError_t launch_bin(const PView &P, const BView &B, uint32_t *block_sum, hipStream_t s);
+hipError_t launch_bin(const PView &P, const BView &B, uint64_t *block_sum, hipStream_t s);
 /* Step phases after binning, in launch order (PHASE_BIN is launch_bin). */
 enum { PHASE_BIN = 0, PHASE_P2G_GRID = 1, PHASE_G2P_VELOCITY = 2, PHASE_REMAP_GRID = 3, PHASE_G2P_STRAIN = 4,
        PHASE_COUNT = 5 };
 hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView &N, const BView &B,
-                        const MatTable &mt, unsigned long long *health, const SurrogateArgs &sg, hipStream_t s);
+                        const CellParts &C, int order, const MatTable &mt, unsigned long long *health,
+                        const SurrogateArgs &sg, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t s);
 hipError_t launch_surrogate_post(const Batch &bt, const PView &P, const MatTable &mt, unsigned long long *health,
                                  const SurrogateArgs &sg, hipStream_t s);

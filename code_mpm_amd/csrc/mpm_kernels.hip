@@ -119,69 +119,85 @@ __global__ __launch_bounds__(BLOCK) void k_keys(Batch bt, PView P, BView B) {
 #define SCAN_ITEMS 16
 #define SCAN_TILE (BLOCK * SCAN_ITEMS)
 
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *lds, uint32_t &total) {
+/* Cell counts are scanned as one 64-bit value per cell: particle count in
+ * the low word, 1 if the cell is occupied in the high word.  The exclusive
+ * prefix gives the cell's first slot in the sorted id list (low word) and
+ * its rank among occupied cells (high word). */
+__device__ __forceinline__ uint64_t pack_count(uint32_t c) { return (uint64_t)c | ((uint64_t)(c != 0u) << 32); }
+
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t *lds, uint64_t &total) {
     /* wave-level inclusive scan, then across the 4 waves */
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t x = v;
+    uint64_t x = v;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, 64);
+        uint64_t y = __shfl_up(x, o, 64);
         if (lane >= o) x += y;
     }
     if (lane == 63) lds[wave] = x;
     __syncthreads();
-    uint32_t off = 0;
+    uint64_t off = 0;
     for (int w = 0; w < wave; w++) off += lds[w];
     total = lds[0] + lds[1] + lds[2] + lds[3];
     __syncthreads();
     return off + x - v;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_scan_reduce(const uint32_t *count, int64_t n, uint32_t *block_sum) {
-    __shared__ uint32_t lds[4];
+__global__ __launch_bounds__(BLOCK) void k_scan_reduce(const uint32_t *count, int64_t n, uint64_t *block_sum) {
+    __shared__ uint64_t lds[4];
     int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-    uint32_t s = 0;
+    uint64_t s = 0;
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; j++)
-        if (base + j < n) s += count[base + j];
-    uint32_t total;
+        if (base + j < n) s += pack_count(count[base + j]);
+    uint64_t total;
     block_exclusive_scan(s, lds, total);
     if (threadIdx.x == 0) block_sum[blockIdx.x] = total;
 }
 
 /* One block scans all block sums in place (exclusive) in chunks of BLOCK. */
-__global__ __launch_bounds__(BLOCK) void k_scan_blocks(uint32_t *block_sum, int64_t nblocks) {
-    __shared__ uint32_t lds[4];
-    uint32_t carry = 0;
+__global__ __launch_bounds__(BLOCK) void k_scan_blocks(uint64_t *block_sum, int64_t nblocks) {
+    __shared__ uint64_t lds[4];
+    uint64_t carry = 0;
     for (int64_t base = 0; base < nblocks; base += BLOCK) {
         int64_t i = base + threadIdx.x;
-        uint32_t v = (i < nblocks) ? block_sum[i] : 0u;
-        uint32_t total;
-        uint32_t ex = block_exclusive_scan(v, lds, total);
+        uint64_t v = (i < nblocks) ? block_sum[i] : 0u;
+        uint64_t total;
+        uint64_t ex = block_exclusive_scan(v, lds, total);
         if (i < nblocks) block_sum[i] = carry + ex;
         carry += total;
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_scan_apply(const uint32_t *count, int64_t n, const uint32_t *block_sum,
-                                                     uint32_t *start) {
-    __shared__ uint32_t lds[4];
+__global__ __launch_bounds__(BLOCK) void k_scan_apply(const uint32_t *count, int64_t n, const uint64_t *block_sum,
+                                                     uint32_t *start, uint32_t *slot, uint32_t *occ, uint32_t *nocc) {
+    __shared__ uint64_t lds[4];
     int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
-    uint32_t s = 0;
+    uint64_t s = 0;
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; j++) {
         v[j] = (base + j < n) ? count[base + j] : 0u;
-        s += v[j];
+        s += pack_count(v[j]);
     }
-    uint32_t total;
-    uint32_t ex = block_exclusive_scan(s, lds, total) + block_sum[blockIdx.x];
+    uint64_t total;
+    uint64_t ex = block_exclusive_scan(s, lds, total) + block_sum[blockIdx.x];
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; j++) {
-        if (base + j < n) start[base + j] = ex;
-        ex += v[j];
+        if (base + j < n) {
+            start[base + j] = (uint32_t)ex;
+            if (v[j]) {
+                uint32_t r = (uint32_t)(ex >> 32);
+                slot[base + j] = r;
+                occ[r] = (uint32_t)(base + j);
+            }
+        }
+        ex += pack_count(v[j]);
     }
-    if (base + SCAN_ITEMS >= n && base < n) start[n] = ex; /* the thread holding the last cell */
+    if (base + SCAN_ITEMS >= n && base < n) { /* the thread holding the last cell */
+        start[n] = (uint32_t)ex;
+        *nocc = (uint32_t)(ex >> 32);
+    }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_scatter(PView P, BView B) {
@@ -277,6 +293,70 @@ __device__ __forceinline__ void dc_consider(DCWin &w, const PView &P, uint32_t i
             }
 }
 
+/* DC override then runSimulation_IntegrateGrid (DP:90-142) on one node's
+ * sums, in registers; stores m, p, f, f/m and the reaction. */
+__device__ __forceinline__ void node_integrate(const Inst &I, const NView &Nd, int64_t node, size_t nn, double m,
+                                               double p[3], double f[3], const DCWin &w) {
+    double v[3] = {0.0, 0.0, 0.0};
+    /* runSimulation_DisplacementControl, DP:143-174 */
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        if (w.rank[a] >= 0) {
+            v[a] = w.v[a];
+            p[a] = m * w.v[a];
+            f[a] = 0.0;
+        }
+    /* runSimulation_IntegrateGrid, DP:90-142 */
+    const double dt = I.dt;
+    {
+        double pp = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+        if (sqrt(pp) > 1.0e-24) {
+            double inv = 1.0 / sqrt(pp);
+            double nx = p[0] * inv, ny = p[1] * inv, nz = p[2] * inv;
+            double sc = I.damping * fabs(f[0] * nx + f[1] * ny + f[2] * nz);
+            f[0] -= sc * nx;
+            f[1] -= sc * ny;
+            f[2] -= sc * nz;
+        }
+    }
+    const uint8_t fixed = Nd.fixed[node];
+    double fty = 0.0;
+    if (fixed & 1) {
+        double pty = p[1] + f[1] * dt;
+        if (fabs(pty * dt) > 1.0e-24 && f[0] > 0.0) {
+            double ffy = Nd.friction[node] * glm_sign(-pty) * fabs(f[0]);
+            if (fabs(ffy * dt) > fabs(pty)) ffy *= fabs(pty / dt) / fabs(ffy);
+            f[1] += ffy;
+        }
+        v[0] = 0.0;
+        p[0] = 0.0;
+        f[0] = 0.0;
+    }
+    if (fixed & 2) {
+        v[1] = 0.0;
+        p[1] = 0.0;
+        fty = 0.0 + f[1];
+        f[1] = 0.0;
+    }
+    if (fixed & 4) {
+        v[2] = 0.0;
+        p[2] = 0.0;
+        f[2] = 0.0;
+    }
+    p[0] += f[0] * dt;
+    p[1] += f[1] * dt;
+    p[2] += f[2] * dt;
+    Nd.active[node] = 1;
+    Nd.m[node] = m;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        Nd.p[a * nn + node] = p[a];
+        Nd.f[a * nn + node] = f[a];
+        Nd.a[a * nn + node] = f[a] / m; /* hoisted f/m of DP:508 */
+    }
+    Nd.fty[node] = fty;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd, BView B) {
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const Inst &I = bt.inst[inst_of_tile(bt, tile)];
@@ -327,65 +407,7 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd,
         merge_advance(M, B, id);
         id = merge_min(M);
     }
-    double v[3] = {0.0, 0.0, 0.0};
-    /* runSimulation_DisplacementControl, DP:143-174 */
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-        if (w.rank[a] >= 0) {
-            v[a] = w.v[a];
-            p[a] = m * w.v[a];
-            f[a] = 0.0;
-        }
-    /* runSimulation_IntegrateGrid, DP:90-142 */
-    const double dt = I.dt;
-    {
-        double pp = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
-        if (sqrt(pp) > 1.0e-24) {
-            double inv = 1.0 / sqrt(pp);
-            double nx = p[0] * inv, ny = p[1] * inv, nz = p[2] * inv;
-            double sc = I.damping * fabs(f[0] * nx + f[1] * ny + f[2] * nz);
-            f[0] -= sc * nx;
-            f[1] -= sc * ny;
-            f[2] -= sc * nz;
-        }
-    }
-    const uint8_t fixed = Nd.fixed[node];
-    double fty = 0.0;
-    if (fixed & 1) {
-        double pty = p[1] + f[1] * dt;
-        if (fabs(pty * dt) > 1.0e-24 && f[0] > 0.0) {
-            double ffy = Nd.friction[node] * glm_sign(-pty) * fabs(f[0]);
-            if (fabs(ffy * dt) > fabs(pty)) ffy *= fabs(pty / dt) / fabs(ffy);
-            f[1] += ffy;
-        }
-        v[0] = 0.0;
-        p[0] = 0.0;
-        f[0] = 0.0;
-    }
-    if (fixed & 2) {
-        v[1] = 0.0;
-        p[1] = 0.0;
-        fty = 0.0 + f[1];
-        f[1] = 0.0;
-    }
-    if (fixed & 4) {
-        v[2] = 0.0;
-        p[2] = 0.0;
-        f[2] = 0.0;
-    }
-    p[0] += f[0] * dt;
-    p[1] += f[1] * dt;
-    p[2] += f[2] * dt;
-    const size_t nn = (size_t)bt.nnodes;
-    Nd.active[node] = 1;
-    Nd.m[node] = m;
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        Nd.p[a * nn + node] = p[a];
-        Nd.f[a * nn + node] = f[a];
-        Nd.a[a * nn + node] = f[a] / m; /* hoisted f/m of DP:508 */
-    }
-    Nd.fty[node] = fty;
+    node_integrate(I, Nd, node, (size_t)bt.nnodes, m, p, f, w);
 }
 
 /* G2P pass 1: v_p += N * (f_i / m_i) * dt over the adjacent nodes in
@@ -431,6 +453,29 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_velocity(Batch bt, PView P, NView
     P.v[2 * n + i] = v[2];
 }
 
+/* Second DC override (DP:568) and BC re-application (DP:573-599) on the
+ * remapped velocity; stores v, the final p, zeroed forces and p/m. */
+__device__ __forceinline__ void node_remap_finish(const NView &Nd, int64_t node, size_t nn, double m, double v[3],
+                                                  const DCWin &w) {
+    double p[3] = {Nd.p[node], Nd.p[nn + node], Nd.p[2 * nn + node]};
+    const uint8_t fixed = Nd.fixed[node];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        if (w.rank[a] >= 0) {
+            v[a] = w.v[a];
+            p[a] = m * w.v[a];
+            Nd.f[a * nn + node] = 0.0;
+        }
+        if ((fixed >> a) & 1) {
+            v[a] = 0.0;
+            p[a] = 0.0;
+        }
+        Nd.v[a * nn + node] = v[a];
+        Nd.p[a * nn + node] = p[a];
+        Nd.u[a * nn + node] = p[a] / m;
+    }
+}
+
 /* Velocity remap (DP:539-564) gathered per node in ascending particle
  * order, then the second displacement-control override (DP:568) and the BC
  * re-application (DP:573-599). Stores the final v, p, the zeroed force
@@ -467,22 +512,164 @@ __global__ __launch_bounds__(BLOCK) void k_remap_grid(Batch bt, PView P, NView N
         dc_consider(w, P, id);
         merge_advance(M, B, id);
     }
-    double p[3] = {Nd.p[node], Nd.p[nn + node], Nd.p[2 * nn + node]};
-    const uint8_t fixed = Nd.fixed[node];
+    node_remap_finish(Nd, node, nn, m, v, w);
+}
+
+/* ---- cell order: per (occupied cell, corner) partial sums, then nodes ----
+ *
+ * A node's sum is the sum, in a fixed order of its 8 adjacent cells, of
+ * per-cell partial sums, each accumulated over the cell's particles in
+ * ascending id.  Thread t handles corner t % 8 of occupied cell t / 8: the 8
+ * lanes of a cell read the same particles, so loads stay coalesced.  The
+ * cell -> corner mapping: local cell c = ext - 1, corner bits (x, y, z) =
+ * (k >> 2, k >> 1, k) & 1, node = c + corner; the node sees that cell as its
+ * (7 - k)-th neighbour. */
+
+template <bool REMAP>
+__global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NView Nd, BView B, CellParts C) {
+    const uint32_t nocc = *B.nocc;
+    const size_t n = P.n, cap = (size_t)C.cap8;
+    for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < (uint64_t)nocc * 8u;
+         t += (uint64_t)gridDim.x * BLOCK) {
+        const uint32_t slot = (uint32_t)(t >> 3);
+        const int corner = (int)(t & 7u);
+        const uint32_t c = B.occ[slot];
+        const uint32_t s0 = B.start[c], s1 = B.start[c + 1];
+        const Inst &I = bt.inst[inst_of_particle(bt, B.sorted[s0])];
+        const GridGeom &g = I.g;
+        const uint32_t e = c - (uint32_t)I.cell_off;
+        const int ex = (int)(e % (uint32_t)g.ec[0]);
+        const int ey = (int)((e / (uint32_t)g.ec[0]) % (uint32_t)g.ec[1]);
+        const int ez = (int)(e / ((uint32_t)g.ec[0] * (uint32_t)g.ec[1]));
+        const int ix = ex - 1 + (corner >> 2), iy = ey - 1 + ((corner >> 1) & 1), iz = ez - 1 + (corner & 1);
+        if (ix < 0 || iy < 0 || iz < 0 || ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) continue;
+        const double xn[3] = {node_coord(ix, g.L[0], g.cells[0]), node_coord(iy, g.L[1], g.cells[1]),
+                              node_coord(iz, g.L[2], g.cells[2])};
+        int brank[3] = {-1, -1, -1};
+        uint32_t bid[3] = {MPM_NONE, MPM_NONE, MPM_NONE};
+        const size_t j = (size_t)t;
+        if (!REMAP) {
+            double m = 0.0, p[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0};
+            for (uint32_t q = s0; q < s1; q++) {
+                const uint32_t id = B.sorted[q];
+                AxisShape ax = axis_shape(P.x[id], xn[0], g.h[0]);
+                AxisShape ay = axis_shape(P.x[n + id], xn[1], g.h[1]);
+                AxisShape az = axis_shape(P.x[2 * n + id], xn[2], g.h[2]);
+                double N, gx, gy, gz;
+                shape_full(ax, ay, az, g.h, N, gx, gy, gz);
+                double mp = P.m[id];
+                m += N * mp;
+                p[0] += N * (mp * P.v[id]);
+                p[1] += N * (mp * P.v[n + id]);
+                p[2] += N * (mp * P.v[2 * n + id]);
+                double V = P.V[id];
+                const double *sg = P.sig;
+                double s0_ = sg[id], s1_ = sg[n + id], s2_ = sg[2 * n + id], s3_ = sg[3 * n + id],
+                       s4_ = sg[4 * n + id], s5_ = sg[5 * n + id];
+                f[0] += -V * (gx * s0_ + gy * s3_ + gz * s5_);
+                f[1] += -V * (gy * s1_ + gx * s3_ + gz * s4_);
+                f[2] += -V * (gz * s2_ + gx * s5_ + gy * s4_);
+                if (I.has_fext) {
+                    f[0] += N * P.fext[id];
+                    f[1] += N * P.fext[n + id];
+                    f[2] += N * P.fext[2 * n + id];
+                }
+                int r = P.dc_rank[id];
+                if (r >= 0) {
+                    uint8_t axm = P.dc_axes[id];
+                    for (int a = 0; a < 3; a++)
+                        if (((axm >> a) & 1) && r > brank[a]) {
+                            brank[a] = r;
+                            bid[a] = id;
+                        }
+                }
+            }
+            C.m[j] = m;
+            for (int a = 0; a < 3; a++) {
+                C.p[a * cap + j] = p[a];
+                C.f[a * cap + j] = f[a];
+            }
+        } else {
+            const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
+            const double mn = Nd.m[node];
+            const bool massive = mn > I.mass_min;
+            double v[3] = {0.0, 0.0, 0.0};
+            for (uint32_t q = s0; q < s1; q++) {
+                const uint32_t id = B.sorted[q];
+                if (massive) {
+                    double N = axis_shape(P.x[id], xn[0], g.h[0]).s * axis_shape(P.x[n + id], xn[1], g.h[1]).s *
+                               axis_shape(P.x[2 * n + id], xn[2], g.h[2]).s;
+                    double mp = P.m[id];
+                    v[0] += N * (mp * P.v[id]) / mn;
+                    v[1] += N * (mp * P.v[n + id]) / mn;
+                    v[2] += N * (mp * P.v[2 * n + id]) / mn;
+                }
+                int r = P.dc_rank[id];
+                if (r >= 0) {
+                    uint8_t axm = P.dc_axes[id];
+                    for (int a = 0; a < 3; a++)
+                        if (((axm >> a) & 1) && r > brank[a]) {
+                            brank[a] = r;
+                            bid[a] = id;
+                        }
+                }
+            }
+            for (int a = 0; a < 3; a++) C.f[a * cap + j] = v[a];
+        }
+        for (int a = 0; a < 3; a++) C.dc[a * cap + j] = bid[a];
+    }
+}
+
+/* Node side of the cell order: adds the 8 partials in neighbour order
+ * k = 0..7 (x outer, z inner), then the same grid update as the particle
+ * order (P2G) or the same remap finish (REMAP). */
+template <bool REMAP>
+__global__ __launch_bounds__(BLOCK) void k_node_cells(Batch bt, PView P, NView Nd, BView B, CellParts C) {
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const Inst &I = bt.inst[inst_of_tile(bt, tile)];
+    const GridGeom &g = I.g;
+    int ix, iy, iz;
+    node_of_tile(g, tile - I.tile_off, ix, iy, iz);
+    if (ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) return;
+    const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
+    const size_t nn = (size_t)bt.nnodes, cap = (size_t)C.cap8, n = P.n;
+    if (REMAP && !Nd.active[node]) return;
+    double m = 0.0, p[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0};
+    DCWin w;
+    w.rank[0] = w.rank[1] = w.rank[2] = -1;
+    w.v[0] = w.v[1] = w.v[2] = 0.0;
+    bool any = false;
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        if (w.rank[a] >= 0) {
-            v[a] = w.v[a];
-            p[a] = m * w.v[a];
-            Nd.f[a * nn + node] = 0.0;
+    for (int k = 0; k < 8; k++) {
+        const uint32_t c = (uint32_t)I.cell_off + (uint32_t)(ix + (k >> 2)) +
+                           (uint32_t)g.ec[0] * ((uint32_t)(iy + ((k >> 1) & 1)) + (uint32_t)g.ec[1] * (uint32_t)(iz + (k & 1)));
+        if (B.start[c] == B.start[c + 1]) continue;
+        any = true;
+        const size_t j = (size_t)B.slot[c] * 8u + (size_t)(7 - k);
+        if (!REMAP) {
+            m += C.m[j];
+            for (int a = 0; a < 3; a++) p[a] += C.p[a * cap + j];
         }
-        if ((fixed >> a) & 1) {
-            v[a] = 0.0;
-            p[a] = 0.0;
+        for (int a = 0; a < 3; a++) f[a] += C.f[a * cap + j];
+        for (int a = 0; a < 3; a++) {
+            uint32_t id = C.dc[a * cap + j];
+            if (id != MPM_NONE) {
+                int r = P.dc_rank[id];
+                if (r > w.rank[a]) {
+                    w.rank[a] = r;
+                    w.v[a] = P.v[(size_t)a * n + id];
+                }
+            }
         }
-        Nd.v[a * nn + node] = v[a];
-        Nd.p[a * nn + node] = p[a];
-        Nd.u[a * nn + node] = p[a] / m;
+    }
+    if (!REMAP) {
+        if (!any) {
+            Nd.active[node] = 0;
+            return;
+        }
+        node_integrate(I, Nd, node, nn, m, p, f, w);
+    } else {
+        node_remap_finish(Nd, node, nn, Nd.m[node], f, w);
     }
 }
 
@@ -950,27 +1137,48 @@ hipError_t launch_keys(const Batch &bt, const PView &P, const BView &B, hipStrea
 
 int64_t scan_block_count(int64_t necells) { return (necells + SCAN_TILE - 1) / SCAN_TILE; }
 
-hipError_t launch_bin(const PView &P, const BView &B, uint32_t *block_sum, hipStream_t s) {
+hipError_t launch_bin(const PView &P, const BView &B, uint64_t *block_sum, hipStream_t s) {
     int64_t nb = scan_block_count(B.necells);
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(BLOCK), 0, s, B.count, B.necells, block_sum);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(BLOCK), 0, s, block_sum, nb);
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(BLOCK), 0, s, B.count, B.necells, block_sum, B.start);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(BLOCK), 0, s, B.count, B.necells, block_sum, B.start,
+                       B.slot, B.occ, B.nocc);
     if (P.n > 0) hipLaunchKernelGGL(k_scatter, dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, P, B);
     hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(B.necells, BLOCK)), dim3(BLOCK), 0, s, B);
     return hipGetLastError();
 }
 
+static unsigned partial_grid(int64_t n) {
+    int64_t b = blocks_for(8 * std::max<int64_t>(n, 1), BLOCK);
+    return (unsigned)std::min<int64_t>(b, 8192);
+}
+
 hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView &N, const BView &B,
-                        const MatTable &mt, unsigned long long *health, const SurrogateArgs &sg, hipStream_t s) {
+                        const CellParts &C, int order, const MatTable &mt, unsigned long long *health,
+                        const SurrogateArgs &sg, hipStream_t s) {
     switch (phase) {
     case PHASE_P2G_GRID:
-        if (bt.ntiles) hipLaunchKernelGGL(k_p2g_grid, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B);
+        if (!bt.ntiles) break;
+        if (order == ORDER_CELL) {
+            if (P.n > 0)
+                hipLaunchKernelGGL(k_cell_partials<false>, dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt, P, N, B, C);
+            hipLaunchKernelGGL(k_node_cells<false>, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
+        } else {
+            hipLaunchKernelGGL(k_p2g_grid, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B);
+        }
         break;
     case PHASE_G2P_VELOCITY:
         if (P.n > 0) hipLaunchKernelGGL(k_g2p_velocity, dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, bt, P, N);
         break;
     case PHASE_REMAP_GRID:
-        if (bt.ntiles) hipLaunchKernelGGL(k_remap_grid, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B);
+        if (!bt.ntiles) break;
+        if (order == ORDER_CELL) {
+            if (P.n > 0)
+                hipLaunchKernelGGL(k_cell_partials<true>, dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt, P, N, B, C);
+            hipLaunchKernelGGL(k_node_cells<true>, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
+        } else {
+            hipLaunchKernelGGL(k_remap_grid, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B);
+        }
         break;
     case PHASE_G2P_STRAIN:
         if (P.n > 0) {

@@ -25,18 +25,25 @@ from .capi import MON, NF, PF, check, ptr
 class Engine:
     """One simulation on one GPU (one HIP stream)."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, order="cell"):
         self._lib = capi.load()
         h = ctypes.c_void_p()
         rc = self._lib.mpm_create(ctypes.byref(h), int(device))
         check(rc, None, "mpm_create")
         self._h = h
+        self.set_order(order)
         self.n = 0
         self.nnodes = 0
         self.console_interval = None
         self.console_last = -1.0e12
         self.console_log = []
         self.scene = None
+
+    def set_order(self, order):
+        """Node summation order: "particle" (the reference's one-thread order,
+        bit for bit) or "cell" (per-cell partial sums in a fixed order; fast)."""
+        self._check(self._lib.mpm_set_summation_order(self._h, capi.ORDER[order]), "set_order")
+        self.order = order
 
     # -- lifetime ---------------------------------------------------------
     def close(self):

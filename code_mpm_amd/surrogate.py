@@ -32,13 +32,18 @@ def hooke_increment(E, nu, de):
     return out
 
 
-def make_training_set(material, rows, device=0):
+def make_training_set(material, rows, device=0, yielding_only=True):
     """rows (k, >=13): sigma6, de6, R.  Targets come from the device's exact
-    return map.  Returns dict with raw features, raw targets, scales."""
+    return map.  Only rows whose trial state yields are kept by default: the
+    engine sends only those to the network.  Returns dict with raw features,
+    raw targets, scales."""
     sig = np.ascontiguousarray(rows[:, 0:6])
     de = np.ascontiguousarray(rows[:, 6:12])
     R = np.ascontiguousarray(rows[:, 12])
     ds, dp, dR = capi.constitutive_batch(material, R, sig, de, device=device)
+    if yielding_only:
+        keep = np.any(dp != 0.0, axis=1)
+        sig, de, R, ds, dp, dR = sig[keep], de[keep], R[keep], ds[keep], dp[keep], dR[keep]
     E, nu, sy = material["elastic_modulus"], material["poisson_ratio"], material["yield_stress"]
     s_sig, s_eps = sy, sy / E
     trial = hooke_increment(E, nu, de)
@@ -140,11 +145,12 @@ class Surrogate:
         return hist
 
 
-def synthetic_states(material, n, seed=0, strain_scale=None, r_max=None):
-    """Return-map inputs spread over the plastic regime: stresses inside or on
-    the yield surface (random deviatoric direction, radius up to the current
-    yield radius, random pressure), strain increments of size ~strain_scale,
-    hardening variable in [0, r_max].  Rows: sigma6, de6, R."""
+def synthetic_states(material, n, seed=0, strain_scale=None, r_max=None, on_surface=0.8):
+    """Return-map inputs spread over the plastic regime: stresses on the
+    yield surface (fraction on_surface, as after a plastic step) or just
+    inside it (random deviatoric direction, random pressure), strain
+    increments of size ~strain_scale, hardening variable in [0, r_max].
+    Rows: sigma6, de6, R."""
     rng = np.random.default_rng(seed)
     E, sy = material["elastic_modulus"], material["yield_stress"]
     if strain_scale is None:
@@ -156,7 +162,8 @@ def synthetic_states(material, n, seed=0, strain_scale=None, r_max=None):
     d = rng.normal(0, 1, (n, 6))
     d[:, :3] -= d[:, :3].mean(1, keepdims=True)
     j2 = 0.5 * np.sum(d[:, :3] ** 2, 1) + np.sum(d[:, 3:] ** 2, 1)
-    d *= (K * rng.uniform(0.7, 1.0, n) / np.sqrt(j2))[:, None]
+    radius = np.where(rng.uniform(size=n) < on_surface, 1.0, rng.uniform(0.98, 1.0, n))
+    d *= (K * radius / np.sqrt(j2))[:, None]
     p = rng.normal(0, 0.5 * sy, n)
     sig = d.copy()
     sig[:, :3] += p[:, None]

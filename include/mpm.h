@@ -214,6 +214,21 @@ int mpm_get_instance_time(const mpm_engine *e, int32_t instance, double *time);
 /* reportConsole values of one instance. */
 int mpm_get_instance_monitors(mpm_engine *e, int32_t instance, mpm_monitors *out);
 
+/* --- node summation order -------------------------------------------------- */
+
+/* Both orders are deterministic and use no floating-point atomics.
+ * MPM_ORDER_PARTICLE: each node adds its particles' contributions in
+ *   ascending particle index, the order of the reference's scatters run by
+ *   one thread (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:48-89,
+ *   539-564); results equal that run bit for bit.
+ * MPM_ORDER_CELL (default): each node adds 8 per-cell partial sums in a
+ *   fixed neighbour order, each partial taken over the cell's particles in
+ *   ascending index; the same terms in another association, so results
+ *   differ from the particle order only by rounding, and much faster on the
+ *   GPU (neighbouring lanes read the same particles). */
+enum { MPM_ORDER_PARTICLE = 0, MPM_ORDER_CELL = 1 };
+int mpm_set_summation_order(mpm_engine *e, int32_t order);
+
 /* --- the hot path ---------------------------------------------------------- */
 
 /* nsteps steps; each instance advances by its own dt_max.  The first call

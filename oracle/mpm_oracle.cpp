@@ -385,6 +385,104 @@ struct oracle_engine {
         }
     }
 
+    /* ---- cell summation order (the engine's MPM_ORDER_CELL) ----
+     * Node sums are taken as sum over the node's 8 neighbour cells k = 0..7
+     * (x outer, z inner; cell = node - 1 + k per axis) of per-cell partial
+     * sums over the cell's particles in ascending index.  Same terms as the
+     * reference's scatter, another association. */
+    int order = MPM_ORDER_PARTICLE;
+    std::vector<uint32_t> bin_start, bin_ids;
+    int ec[3] = {0, 0, 0};
+
+    void build_bins() {
+        for (int a = 0; a < 3; a++) ec[a] = cells[a] + 2;
+        size_t necells = (size_t)ec[0] * ec[1] * ec[2];
+        std::vector<long> key(particles.size(), -1);
+        bin_start.assign(necells + 1, 0);
+        for (size_t i = 0; i < particles.size(); i++) {
+            const V3 &x = particles[i]->x;
+            double q[3] = {x.x / h.x, x.y / h.y, x.z / h.z};
+            long e[3];
+            bool ok = true;
+            for (int a = 0; a < 3; a++) {
+                double f = std::floor(q[a]);
+                if (!(f >= -1.0 && f <= (double)cells[a])) ok = false;
+                else e[a] = (long)f + 1;
+            }
+            if (!ok) continue;
+            key[i] = e[0] + ec[0] * (e[1] + (long)ec[1] * e[2]);
+            bin_start[key[i] + 1]++;
+        }
+        for (size_t c = 0; c < necells; c++) bin_start[c + 1] += bin_start[c];
+        bin_ids.assign(bin_start[necells], 0);
+        std::vector<uint32_t> fill(bin_start.begin(), bin_start.end() - 1);
+        for (size_t i = 0; i < particles.size(); i++)
+            if (key[i] >= 0) bin_ids[fill[key[i]]++] = (uint32_t)i; /* ascending index within a cell */
+    }
+
+    const AGP *agp_for(const Particle *p, unsigned node) const {
+        for (const AGP &a : p->agp)
+            if (a.index == node) return &a;
+        return nullptr;
+    }
+
+    void m2g_cells() {
+#pragma omp for
+        for (long ni = 0; ni < (long)nodes.size(); ni++) {
+            Node *n = nodes[ni];
+            if (!n->active) continue;
+            int ix = (int)(ni % nn[0]), iy = (int)((ni / nn[0]) % nn[1]), iz = (int)(ni / ((long)nn[0] * nn[1]));
+            for (int k = 0; k < 8; k++) {
+                long c = (ix + (k >> 2)) + ec[0] * ((iy + ((k >> 1) & 1)) + (long)ec[1] * (iz + (k & 1)));
+                if (bin_start[c] == bin_start[c + 1]) continue;
+                double sm = 0.0;
+                V3 sp, sf;
+                for (uint32_t q = bin_start[c]; q < bin_start[c + 1]; q++) {
+                    const Particle *p = particles[bin_ids[q]];
+                    const AGP *a = agp_for(p, (unsigned)ni);
+                    double N = a->N;
+                    V3 g = a->g;
+                    sm += N * p->m;
+                    sp.x += N * (p->m * p->v.x);
+                    sp.y += N * (p->m * p->v.y);
+                    sp.z += N * (p->m * p->v.z);
+                    double V = p->V;
+                    sf.x += -V * (g.x * p->sig[0] + g.y * p->sig[3] + g.z * p->sig[5]);
+                    sf.y += -V * (g.y * p->sig[1] + g.x * p->sig[3] + g.z * p->sig[4]);
+                    sf.z += -V * (g.z * p->sig[2] + g.x * p->sig[5] + g.y * p->sig[4]);
+                    sf.x += N * p->fext.x;
+                    sf.y += N * p->fext.y;
+                    sf.z += N * p->fext.z;
+                }
+                n->m += sm;
+                n->p.x += sp.x; n->p.y += sp.y; n->p.z += sp.z;
+                n->f.x += sf.x; n->f.y += sf.y; n->f.z += sf.z;
+            }
+        }
+    }
+
+    void remap_cells() {
+#pragma omp for
+        for (long ni = 0; ni < (long)nodes.size(); ni++) {
+            Node *n = nodes[ni];
+            if (!n->active || !(n->m > mass_min)) continue;
+            int ix = (int)(ni % nn[0]), iy = (int)((ni / nn[0]) % nn[1]), iz = (int)(ni / ((long)nn[0] * nn[1]));
+            for (int k = 0; k < 8; k++) {
+                long c = (ix + (k >> 2)) + ec[0] * ((iy + ((k >> 1) & 1)) + (long)ec[1] * (iz + (k & 1)));
+                if (bin_start[c] == bin_start[c + 1]) continue;
+                V3 sv;
+                for (uint32_t q = bin_start[c]; q < bin_start[c + 1]; q++) {
+                    const Particle *p = particles[bin_ids[q]];
+                    double N = agp_for(p, (unsigned)ni)->N;
+                    sv.x += N * (p->m * p->v.x) / n->m;
+                    sv.y += N * (p->m * p->v.y) / n->m;
+                    sv.z += N * (p->m * p->v.z) / n->m;
+                }
+                n->v.x += sv.x; n->v.y += sv.y; n->v.z += sv.z;
+            }
+        }
+    }
+
     /* runSimulation_DisplacementControl, DP:143-174. */
     void displacement_control() {
 #pragma omp for
@@ -607,7 +705,13 @@ struct oracle_engine {
 #pragma omp barrier
             find_agps();
 #pragma omp barrier
-            m2g(nt);
+            if (order == MPM_ORDER_CELL) {
+#pragma omp single
+                build_bins();
+                m2g_cells();
+            } else {
+                m2g(nt);
+            }
 #pragma omp barrier
             displacement_control();
 #pragma omp barrier
@@ -615,7 +719,10 @@ struct oracle_engine {
 #pragma omp barrier
             g2p_velocity(dt);
 #pragma omp barrier
-            remap(nt);
+            if (order == MPM_ORDER_CELL)
+                remap_cells();
+            else
+                remap(nt);
 #pragma omp barrier
             displacement_control();
 #pragma omp barrier
@@ -806,6 +913,12 @@ int oracle_set_damping(oracle_engine *e, double damping, double mass_min) {
     if (!e) return MPM_ERR_ARG;
     e->damping = damping;
     e->mass_min = mass_min;
+    return MPM_OK;
+}
+
+int oracle_set_order(oracle_engine *e, int32_t order) {
+    if (!e || (order != MPM_ORDER_PARTICLE && order != MPM_ORDER_CELL)) return MPM_ERR_ARG;
+    e->order = order;
     return MPM_OK;
 }
 

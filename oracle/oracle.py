@@ -49,6 +49,7 @@ def load():
         "oracle_set_time": [P, DBL, DBL, DBL],
         "oracle_set_damping": [P, DBL, DBL],
         "oracle_step": [P, DBL, I32, I32],
+        "oracle_set_order": [P, I32],
         "oracle_run": [P, DBL, I32, ctypes.POINTER(I32)],
         "oracle_get_time": [P, ctypes.POINTER(DBL), ctypes.POINTER(I64)],
         "oracle_get_node_count": [P, ctypes.POINTER(I64)],
@@ -84,11 +85,12 @@ class OracleEngine:
     """The reference's DoublePass step on the CPU (single thread = the
     reference's summation order; more threads = its OpenMP lock scatter)."""
 
-    def __init__(self, threads=1):
+    def __init__(self, threads=1, order="particle"):
         self._lib = load()
         h = ctypes.c_void_p()
         _ok(self._lib.oracle_create(ctypes.byref(h)), "create")
         self._h = h
+        _ok(self._lib.oracle_set_order(self._h, {"particle": 0, "cell": 1}[order]), "order")
         self.threads = int(threads)
         self.n = 0
         self.nnodes = 0

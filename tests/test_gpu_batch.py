@@ -21,7 +21,7 @@ def test_batch_instances_match_separate_runs_and_oracle():
     from code_mpm_amd import Engine
     from oracle.oracle import OracleEngine
     sc = _scenes()
-    batch = Engine()
+    batch = Engine(order="particle")
     offs = batch.load_batch(sc)
     steps = 12
     batch.step(0.0, steps)  # each instance by its own dt_max

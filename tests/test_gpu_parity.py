@@ -15,12 +15,15 @@ PARTICLE_FIELDS = ["POSITION", "VELOCITY", "MASS", "VOLUME", "DEFORMATION_GRADIE
 NODE_FIELDS = ["MASS", "MOMENTUM", "VELOCITY", "FORCE", "FORCE_TEMP", "ACTIVE"]
 
 
-def _pair(scene):
+ORDERS = ["particle", "cell"]
+
+
+def _pair(scene, order="particle"):
     from code_mpm_amd import Engine
     from oracle.oracle import OracleEngine
-    g = Engine(device=0)
+    g = Engine(device=0, order=order)
     g.load_scene(scene)
-    c = OracleEngine(threads=1)
+    c = OracleEngine(threads=1, order=order)
     c.load_scene(scene)
     return g, c
 
@@ -42,11 +45,12 @@ def _assert_monitors(g, c, label):
         assert mg[k] == mc[k], f"{label}: monitor {k}: {mg[k]!r} vs {mc[k]!r}"
 
 
+@pytest.mark.parametrize("order", ORDERS)
 @pytest.mark.parametrize("seed", [0, 1, 2, 3])
-def test_random_block_per_step(seed):
+def test_random_block_per_step(seed, order):
     from code_mpm_amd import scenes
     scene = scenes.random_block(seed=seed)
-    g, c = _pair(scene)
+    g, c = _pair(scene, order)
     for step in range(1, 13):
         g.step(scene.dt, 1)
         c.step(scene.dt, 1)
@@ -58,11 +62,12 @@ def test_random_block_per_step(seed):
     assert hg["outside_grid"] > 0 and hg["null_material"] > 0 and hg["plastic_points"] > 0
 
 
-def test_random_block_dense_many_steps():
+@pytest.mark.parametrize("order", ORDERS)
+def test_random_block_dense_many_steps(order):
     """Denser cells (tens of particles per cell) over 100 steps."""
     from code_mpm_amd import scenes
     scene = scenes.random_block(seed=11, cells=(4, 4, 3), n=2500)
-    g, c = _pair(scene)
+    g, c = _pair(scene, order)
     g.step(scene.dt, 100)
     c.step(scene.dt, 100)
     _assert_same(g, c, "dense 100 steps")
@@ -80,10 +85,11 @@ def test_run_clamps_last_step():
     _assert_same(g, c, "run")
 
 
-def test_elastic_bar_1000_steps():
+@pytest.mark.parametrize("order", ORDERS)
+def test_elastic_bar_1000_steps(order):
     from code_mpm_amd import scenes
     scene = scenes.elastic_bar()
-    g, c = _pair(scene)
+    g, c = _pair(scene, order)
     for k in range(10):
         g.step(scene.dt, 100)
         c.step(scene.dt, 100)
@@ -91,10 +97,11 @@ def test_elastic_bar_1000_steps():
     _assert_same(g, c, "bar 1000 steps")
 
 
-def test_honeycomb_small_crush():
+@pytest.mark.parametrize("order", ORDERS)
+def test_honeycomb_small_crush(order):
     from code_mpm_amd import scenes
     scene = scenes.honeycomb_crush(cells=(64, 64, 8), steps=300)
-    g, c = _pair(scene)
+    g, c = _pair(scene, order)
     for k in range(3):
         g.step(scene.dt, 100)
         c.step(scene.dt, 100)
@@ -102,19 +109,20 @@ def test_honeycomb_small_crush():
     _assert_same(g, c, "honeycomb 300 steps")
 
 
-def test_edge_cases():
+@pytest.mark.parametrize("order", ORDERS)
+def test_edge_cases(order):
     from code_mpm_amd import scenes
     # every particle off the grid
     scene = scenes.random_block(seed=7, n=50, features=False)
     scene.fields["POSITION"] = scene.fields["POSITION"] + 10.0
-    g, c = _pair(scene)
+    g, c = _pair(scene, order)
     g.step(scene.dt, 3)
     c.step(scene.dt, 3)
     _assert_same(g, c, "all outside")
     assert g.health()["outside_grid"] == 150
     # a single particle
     scene = scenes.random_block(seed=8, n=1, features=False)
-    g, c = _pair(scene)
+    g, c = _pair(scene, order)
     g.step(scene.dt, 5)
     c.step(scene.dt, 5)
     _assert_same(g, c, "single particle")
@@ -165,3 +173,36 @@ def test_sqrt_and_division_rounding():
     b = oracle.constitutive_batch(mat, R, sig, de)
     assert np.array_equal(a[0], b[0])
     assert np.array_equal(a[1], b[1])
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_cell_order_tracks_reference_order():
+    """The fast cell order against the reference's one-thread particle order
+    (the oracle): per step within 1e-12 relative on grid and particle fields,
+    and over 1000 steps within 1e-9 relative on stress, displacement and the
+    force-displacement monitors."""
+    from code_mpm_amd import Engine, scenes
+    from oracle.oracle import OracleEngine
+    scene = scenes.elastic_bar()
+    g = Engine(order="cell")
+    g.load_scene(scene)
+    c = OracleEngine(threads=1, order="particle")
+    c.load_scene(scene)
+    x0 = scene.fields["POSITION"]
+    for k in range(1, 1001):
+        g.step(scene.dt, 1)
+        c.step(scene.dt, 1)
+        if k <= 3:
+            for name in ("POSITION", "VELOCITY", "STRESS"):
+                assert _rel(g.get_particle(name), c.get_particle(name)) < 1e-12, (k, name)
+            for name in ("MASS", "MOMENTUM", "VELOCITY"):
+                assert _rel(g.get_node(name), c.get_node(name)) < 1e-12, (k, name)
+        if k % 250 == 0:
+            assert _rel(g.get_particle("STRESS"), c.get_particle("STRESS")) < 1e-9, k
+            assert _rel(g.get_particle("POSITION") - x0, c.get_particle("POSITION") - x0) < 1e-9, k
+            mg, mc = g.monitors(), c.monitors()
+            for key in ("position_y", "momentum_y", "kinetic_energy", "strain_energy"):
+                assert abs(mg[key] - mc[key]) <= 1e-9 * abs(mc[key]) + 1e-300, (k, key)
