@@ -108,6 +108,9 @@ SIGNATURES = {
     "mpm_step": (I32, [P, DBL, I32]),
     "mpm_step_graph": (I32, [P, I32]),
     "mpm_set_summation_order": (I32, [P, I32]),
+    "mpm_set_tile_shape": (I32, [P, PI32]),
+    "mpm_get_tile_shape": (I32, [P, PI32]),
+    "mpm_autotune": (I32, [P, I32, PI32, PDBL]),
     "mpm_set_instances": (I32, [P, I32, P, P, P]),
     "mpm_get_instance_layout": (I32, [P, I32, PI64, PI64, PI64]),
     "mpm_set_instance_timeline": (I32, [P, I32, P, P, I32]),

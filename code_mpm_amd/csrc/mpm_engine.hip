@@ -61,11 +61,16 @@ bool make_geom(const double L[3], const int32_t cells[3], GridGeom &g) {
     return true;
 }
 
-uint32_t tiles_of(const GridGeom &g) {
-    int64_t t = (int64_t)((g.nn[0] + TILE_X - 1) / TILE_X) * ((g.nn[1] + TILE_Y - 1) / TILE_Y) *
-                ((g.nn[2] + TILE_Z - 1) / TILE_Z);
+uint32_t tiles_of(const GridGeom &g, const int shape[3]) {
+    int64_t t = (int64_t)((g.nn[0] + shape[0] - 1) / shape[0]) * ((g.nn[1] + shape[1] - 1) / shape[1]) *
+                ((g.nn[2] + shape[2] - 1) / shape[2]);
     return (uint32_t)t;
 }
+
+/* Node tile shapes the autotuner tries (all NODE_BLOCK nodes). */
+const int kTileShapes[][3] = {{8, 8, 4}, {16, 4, 4}, {4, 8, 8}, {16, 16, 1}, {32, 8, 1},
+                              {64, 4, 1}, {8, 4, 8}, {4, 4, 16}, {16, 8, 2}, {32, 4, 2}};
+const int kNumTileShapes = sizeof(kTileShapes) / sizeof(kTileShapes[0]);
 
 }  // namespace
 
@@ -103,6 +108,9 @@ struct mpm_engine {
     uint32_t *bslot = nullptr, *bocc = nullptr, *bnocc = nullptr;
     uint64_t *bblock = nullptr;
     bool keys_valid = false;
+
+    /* node tile shape */
+    int tile[3] = {8, 8, 4};
 
     /* node summation order and the cell-order partial sums */
     int order = ORDER_CELL;
@@ -289,7 +297,26 @@ struct mpm_engine {
         b.ninst = (int)inst.size();
         b.ntiles = ntiles;
         b.nnodes = nnodes;
+        b.tx = tile[0];
+        b.ty = tile[1];
+        b.tz = tile[2];
         return b;
+    }
+
+    /* Tile offsets of every instance for the current tile shape. */
+    int layout_tiles() {
+        ntiles = 0;
+        for (auto &I : inst) {
+            I.tile_off = ntiles;
+            I.tiles = tiles_of(I.g, tile);
+            if ((int64_t)ntiles + I.tiles >= (int64_t)0xFFFFFFFFll) return fail(MPM_ERR_RANGE, "too many node tiles");
+            ntiles += I.tiles;
+        }
+        std::vector<uint32_t> to(inst.size() + 1);
+        for (size_t k = 0; k < inst.size(); k++) to[k] = inst[k].tile_off;
+        to[inst.size()] = ntiles;
+        table_dirty = true;
+        return hip(hipMemcpy(dtile_off, to.data(), 4 * to.size(), hipMemcpyHostToDevice), "upload tile offsets");
     }
 
     /* Allocates nodes, bins and the instance table for the current `inst`. */
@@ -299,16 +326,11 @@ struct mpm_engine {
         free_nodes();
         layout_set = false;
         nnodes = ncells = 0;
-        ntiles = 0;
         for (auto &I : inst) {
             I.node_off = nnodes;
             I.cell_off = ncells;
-            I.tile_off = ntiles;
-            I.tiles = tiles_of(I.g);
             nnodes += I.g.nnodes;
             ncells += I.g.necells;
-            if ((int64_t)ntiles + I.tiles >= (int64_t)0xFFFFFFFFll) return fail(MPM_ERR_RANGE, "too many node tiles");
-            ntiles += I.tiles;
         }
         if (ncells >= (int64_t)0xFFFFFFFFll) return fail(MPM_ERR_RANGE, "grid too large: %lld cells", (long long)ncells);
         const size_t nn = (size_t)nnodes;
@@ -336,11 +358,7 @@ struct mpm_engine {
         if ((rc = hip(hipMemset(nact, 0, nn), "clear active")) || (rc = hip(hipMemset(nfix, 0, nn), "clear fixed")) ||
             (rc = hip(hipMemset(nfric, 0, 8 * nn), "clear friction")) || (rc = hip(hipMemset(nm, 0, 8 * nn), "clear mass")))
             return rc;
-        std::vector<uint32_t> to(inst.size() + 1);
-        for (size_t k = 0; k < inst.size(); k++) to[k] = inst[k].tile_off;
-        to[inst.size()] = ntiles;
-        if ((rc = hip(hipMemcpy(dtile_off, to.data(), 4 * to.size(), hipMemcpyHostToDevice), "upload tile offsets")))
-            return rc;
+        if ((rc = layout_tiles())) return rc;
         layout_set = true;
         keys_valid = false;
         table_dirty = true;
@@ -449,6 +467,84 @@ struct mpm_engine {
         if ((rc = enqueue_step(ev))) return rc;
         keys_valid = true; /* g2p_strain wrote the next keys and counts */
         advance_mirror(dt0);
+        return MPM_OK;
+    }
+
+    /* Mean milliseconds of `reps` launches of one phase. */
+    int time_phase(int phase, int reps, const SurrogateArgs &sa, hipEvent_t e0, hipEvent_t e1, double &ms) {
+        int rc;
+        if ((rc = hip(hipEventRecord(e0, stream), "event"))) return rc;
+        for (int r = 0; r < reps; r++)
+            if ((rc = hip(launch_phase(phase, batch(), pview(), nview(), bview(), parts, order, mt, d_health, sa, stream),
+                          "autotune phase")))
+                return rc;
+        if ((rc = hip(hipEventRecord(e1, stream), "event")) || (rc = hip(hipEventSynchronize(e1), "event sync")))
+            return rc;
+        float f = 0.f;
+        if ((rc = hip(hipEventElapsedTime(&f, e0, e1), "event time"))) return rc;
+        ms = f / reps;
+        return MPM_OK;
+    }
+    int set_tile(const int shape[3]) {
+        int rc = hip(hipStreamSynchronize(stream), "tile sync");
+        if (rc) return rc;
+        tile[0] = shape[0];
+        tile[1] = shape[1];
+        tile[2] = shape[2];
+        drop_graphs();
+        if ((rc = layout_tiles())) return rc;
+        if ((rc = upload_table())) return rc;
+        /* the table upload cleared vt: recompute it from the device clock */
+        return hip(launch_prepare(batch(), stream), "step prepare");
+    }
+    /* One step in which the node phases (P2G + grid update, remap + BC),
+     * which only read particle and bin state and overwrite node state, are
+     * timed under every candidate tile shape; the fastest shape is kept and
+     * the step completes exactly as an ordinary one. */
+    int autotune_step(int reps, double *best_ms) {
+        int rc = begin_steps();
+        if (rc) return rc;
+        if ((rc = hip(launch_set_dt(batch(), 0, inst[0].dt_max, stream), "set dt"))) return rc;
+        hipEvent_t e0, e1;
+        if ((rc = hip(hipEventCreate(&e0), "event")) || (rc = hip(hipEventCreate(&e1), "event"))) return rc;
+        const SurrogateArgs sa = sargs();
+        std::vector<double> t_p2g(kNumTileShapes, 0.0), t_remap(kNumTileShapes, 0.0);
+        rc = hip(launch_prepare(batch(), stream), "step prepare");
+        if (!rc) rc = hip(launch_bin(pview(), bview(), bblock, stream), "cell binning");
+        for (int k = 0; k < kNumTileShapes && !rc; k++) {
+            rc = set_tile(kTileShapes[k]);
+            if (!rc) rc = time_phase(PHASE_P2G_GRID, reps, sa, e0, e1, t_p2g[k]);
+        }
+        /* G2P pass 1 must see the node state of one P2G pass: redo it with the current shape */
+        double once = 0.0;
+        if (!rc) rc = time_phase(PHASE_P2G_GRID, 1, sa, e0, e1, once);
+        if (!rc) rc = hip(launch_phase(PHASE_G2P_VELOCITY, batch(), pview(), nview(), bview(), parts, order, mt, d_health,
+                                       sa, stream), "g2p velocity");
+        for (int k = 0; k < kNumTileShapes && !rc; k++) {
+            rc = set_tile(kTileShapes[k]);
+            if (!rc) rc = time_phase(PHASE_REMAP_GRID, reps, sa, e0, e1, t_remap[k]);
+        }
+        int best = 0;
+        for (int k = 1; k < kNumTileShapes; k++)
+            if (t_p2g[k] + t_remap[k] < t_p2g[best] + t_remap[best]) best = k;
+        if (!rc) rc = set_tile(kTileShapes[best]);
+        if (!rc) rc = time_phase(PHASE_REMAP_GRID, 1, sa, e0, e1, once);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+        if (sur && (rc = hip(launch_zero_u32(qcount, 1, stream), "queue reset"))) return rc;
+        if ((rc = hip(launch_phase(PHASE_G2P_STRAIN, batch(), pview(), nview(), bview(), parts, order, mt, d_health, sa,
+                                   stream), "g2p strain")))
+            return rc;
+        if (sur) {
+            rc = hip(mlp_launch_forward(sur->params, sur->wbf16, qx, qy, n, stream, qcount), "surrogate forward");
+            if (!rc) rc = hip(launch_surrogate_post(batch(), pview(), mt, d_health, sa, stream), "surrogate post");
+            if (rc) return rc;
+        }
+        if ((rc = hip(launch_advance(batch(), stream), "step advance"))) return rc;
+        keys_valid = true;
+        advance_mirror(nullptr);
+        if (best_ms) *best_ms = t_p2g[best] + t_remap[best];
         return MPM_OK;
     }
 
@@ -927,6 +1023,30 @@ int mpm_set_summation_order(mpm_engine *e, int32_t order) {
     e->drop_graphs();
     e->order = order == MPM_ORDER_CELL ? ORDER_CELL : ORDER_PARTICLE;
     return MPM_OK;
+}
+
+int mpm_set_tile_shape(mpm_engine *e, const int32_t shape[3]) {
+    CHECK_E(e);
+    if (!shape || shape[0] < 1 || shape[1] < 1 || shape[2] < 1 || shape[0] * shape[1] * shape[2] != NODE_BLOCK)
+        return e->fail(MPM_ERR_ARG, "tile shape must multiply to %d", NODE_BLOCK);
+    if (!e->layout_set) return e->fail(MPM_ERR_STATE, "no grid");
+    int s[3] = {shape[0], shape[1], shape[2]};
+    return e->set_tile(s);
+}
+
+int mpm_get_tile_shape(const mpm_engine *e, int32_t shape[3]) {
+    if (!e || !shape) return MPM_ERR_ARG;
+    for (int a = 0; a < 3; a++) shape[a] = e->tile[a];
+    return MPM_OK;
+}
+
+int mpm_autotune(mpm_engine *e, int32_t reps, int32_t shape[3], double *node_ms) {
+    CHECK_E(e);
+    if (reps < 1 || reps > 100) return e->fail(MPM_ERR_ARG, "reps outside [1, 100]");
+    int rc = e->autotune_step(reps, node_ms);
+    if (!rc && shape)
+        for (int a = 0; a < 3; a++) shape[a] = e->tile[a];
+    return rc;
 }
 
 int mpm_step_graph(mpm_engine *e, int32_t nsteps) {

@@ -10,10 +10,9 @@
 #include "mpm_device.h"
 #include "mpm_mlp.h"
 
-/* Node tile of one 256-thread block: 8 x 8 x 4 nodes, x fastest. */
-#define TILE_X 8
-#define TILE_Y 8
-#define TILE_Z 4
+/* Node tiles: one 256-thread block covers tx x ty x tz nodes (x fastest);
+ * the shape is chosen at run time (default 8 x 8 x 4, or by the autotuner). */
+#define NODE_BLOCK 256
 
 enum { HEALTH_OUTSIDE = 0, HEALTH_NULL = 1, HEALTH_PLASTIC = 2, HEALTH_NONFINITE = 3, HEALTH_SURROGATE_CALLS = 4,
        HEALTH_SURROGATE_FALLBACKS = 5, HEALTH_SLOTS = 6 };
@@ -114,6 +113,7 @@ struct Batch {
     int ninst;
     uint32_t ntiles;
     int64_t nnodes;           /* all instances: component stride of node arrays */
+    int tx, ty, tz;           /* node tile shape, tx * ty * tz == NODE_BLOCK */
 };
 
 struct MonLists {

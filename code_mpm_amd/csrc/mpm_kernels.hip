@@ -262,14 +262,16 @@ __device__ __forceinline__ void merge_advance(Merge8 &M, const BView &B, uint32_
     }
 }
 
-__device__ __forceinline__ void node_of_tile(const GridGeom &g, uint32_t tile, int &ix, int &iy, int &iz) {
-    int tnx = (g.nn[0] + TILE_X - 1) / TILE_X, tny = (g.nn[1] + TILE_Y - 1) / TILE_Y;
+__device__ __forceinline__ void node_of_tile(const Batch &bt, const GridGeom &g, uint32_t tile, int &ix, int &iy,
+                                             int &iz) {
+    const int TX = bt.tx, TY = bt.ty;
+    int tnx = (g.nn[0] + TX - 1) / TX, tny = (g.nn[1] + TY - 1) / TY;
     int tx = (int)(tile % (uint32_t)tnx);
     int ty = (int)((tile / (uint32_t)tnx) % (uint32_t)tny);
     int tz = (int)(tile / ((uint32_t)tnx * (uint32_t)tny));
-    ix = tx * TILE_X + (int)(threadIdx.x % TILE_X);
-    iy = ty * TILE_Y + (int)((threadIdx.x / TILE_X) % TILE_Y);
-    iz = tz * TILE_Z + (int)(threadIdx.x / (TILE_X * TILE_Y));
+    ix = tx * TX + (int)(threadIdx.x % TX);
+    iy = ty * TY + (int)((threadIdx.x / TX) % TY);
+    iz = tz * bt.tz + (int)(threadIdx.x / (TX * TY));
 }
 
 /* Track, per axis, the displacement-controlled particle latest in the
@@ -362,7 +364,7 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd,
     const Inst &I = bt.inst[inst_of_tile(bt, tile)];
     const GridGeom &g = I.g;
     int ix, iy, iz;
-    node_of_tile(g, tile - I.tile_off, ix, iy, iz);
+    node_of_tile(bt, g, tile - I.tile_off, ix, iy, iz);
     if (ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) return;
     const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
     const size_t n = P.n;
@@ -485,7 +487,7 @@ __global__ __launch_bounds__(BLOCK) void k_remap_grid(Batch bt, PView P, NView N
     const Inst &I = bt.inst[inst_of_tile(bt, tile)];
     const GridGeom &g = I.g;
     int ix, iy, iz;
-    node_of_tile(g, tile - I.tile_off, ix, iy, iz);
+    node_of_tile(bt, g, tile - I.tile_off, ix, iy, iz);
     if (ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) return;
     const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
     if (!Nd.active[node]) return;
@@ -629,7 +631,7 @@ __global__ __launch_bounds__(BLOCK) void k_node_cells(Batch bt, PView P, NView N
     const Inst &I = bt.inst[inst_of_tile(bt, tile)];
     const GridGeom &g = I.g;
     int ix, iy, iz;
-    node_of_tile(g, tile - I.tile_off, ix, iy, iz);
+    node_of_tile(bt, g, tile - I.tile_off, ix, iy, iz);
     if (ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) return;
     const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
     const size_t nn = (size_t)bt.nnodes, cap = (size_t)C.cap8, n = P.n;

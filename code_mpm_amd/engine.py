@@ -205,6 +205,23 @@ class Engine:
         self._check(self._lib.mpm_get_instance_time(self._h, int(k), ctypes.byref(t)), "instance time")
         return t.value
 
+    def set_tile_shape(self, shape):
+        a = np.ascontiguousarray(shape, np.int32)
+        self._check(self._lib.mpm_set_tile_shape(self._h, a.ctypes.data_as(capi.PI32)), "set_tile_shape")
+
+    def tile_shape(self):
+        a = np.zeros(3, np.int32)
+        self._check(self._lib.mpm_get_tile_shape(self._h, a.ctypes.data_as(capi.PI32)), "get_tile_shape")
+        return tuple(int(v) for v in a)
+
+    def autotune(self, reps=3):
+        """One dt_max step that also picks the fastest node tile shape."""
+        a = np.zeros(3, np.int32)
+        ms = ctypes.c_double()
+        self._check(self._lib.mpm_autotune(self._h, int(reps), a.ctypes.data_as(capi.PI32), ctypes.byref(ms)),
+                    "autotune")
+        return tuple(int(v) for v in a), ms.value
+
     def step_graph(self, nsteps):
         """nsteps steps replayed from a HIP graph (recorded on first use)."""
         self._check(self._lib.mpm_step_graph(self._h, int(nsteps)), "step_graph")

@@ -229,6 +229,20 @@ int mpm_get_instance_monitors(mpm_engine *e, int32_t instance, mpm_monitors *out
 enum { MPM_ORDER_PARTICLE = 0, MPM_ORDER_CELL = 1 };
 int mpm_set_summation_order(mpm_engine *e, int32_t order);
 
+/* --- node tiles and the autotuner (no counterpart in the reference) -------- */
+
+/* Nodes are processed in tiles of shape[0] x shape[1] x shape[2] = 256 nodes
+ * per workgroup (default 8 x 8 x 4). */
+int mpm_set_tile_shape(mpm_engine *e, const int32_t shape[3]);
+int mpm_get_tile_shape(const mpm_engine *e, int32_t shape[3]);
+/* Performs one step during which the two node phases are launched `reps`
+ * times under each candidate tile shape (they read particle and bin state
+ * and overwrite node state, so repeating them changes nothing) and keeps
+ * the shape with the least total time; the step's results equal an
+ * ordinary mpm_step of dt_max.  Returns the chosen shape and its node-phase
+ * milliseconds. */
+int mpm_autotune(mpm_engine *e, int32_t reps, int32_t shape[3], double *node_ms);
+
 /* --- the hot path ---------------------------------------------------------- */
 
 /* nsteps steps; each instance advances by its own dt_max.  The first call

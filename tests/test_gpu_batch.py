@@ -68,3 +68,23 @@ def test_batch_graph_replay_matches_eager():
             assert np.array_equal(eager.get_particle(name), graph.get_particle(name)), (rep, name)
         # eager work between replays
         graph.health()
+
+
+@pytest.mark.parametrize("order", ["particle", "cell"])
+def test_autotuned_step_equals_ordinary_step(order):
+    from code_mpm_amd import Engine, scenes
+    sc = scenes.honeycomb_crush(cells=(48, 48, 8), steps=100)
+    a, b = Engine(order=order), Engine(order=order)
+    a.load_scene(sc)
+    b.load_scene(sc)
+    a.step(sc.dt, 3)
+    b.step(sc.dt, 2)
+    shape, ms = b.autotune(reps=2)
+    assert shape[0] * shape[1] * shape[2] == 256 and ms > 0
+    for name in FIELDS:
+        assert np.array_equal(a.get_particle(name), b.get_particle(name)), name
+    assert a.monitors() == b.monitors()
+    a.step(sc.dt, 5)
+    b.step(sc.dt, 5)   # with the tuned shape
+    for name in FIELDS:
+        assert np.array_equal(a.get_particle(name), b.get_particle(name)), name
