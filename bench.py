@@ -35,24 +35,31 @@ PEAK_HBM_GBS = 8000.0
 PER_PARTICLE = {"g2p_strain": 72 * 8 + 5, "p2g_grid": 14 * 8, "g2p_velocity": 9 * 8, "remap_grid": 7 * 8,
                 "bin": 0}
 PER_NODE = {"g2p_strain": 7 * 8, "p2g_grid": 20 * 8, "g2p_velocity": 4 * 8, "remap_grid": 10 * 8, "bin": 0}
-KERNEL_NAME = {"g2p_strain": "k_g2p_strain", "p2g_grid": "k_p2g_grid", "g2p_velocity": "k_g2p_velocity",
-               "remap_grid": "k_remap_grid", "bin": "k_scan_*/k_scatter/k_cell_sort"}
+# kernels of each phase, per summation order (names as rocprofv3 reports them)
+KERNELS = {
+    "cell": {"bin": ["k_scan_reduce", "k_scan_blocks", "k_scan_apply", "k_scatter", "k_cell_sort"],
+             "p2g_grid": ["k_cell_partials<false>", "k_node_cells<false>"], "g2p_velocity": ["k_g2p_velocity"],
+             "remap_grid": ["k_cell_partials<true>", "k_node_cells<true>"], "g2p_strain": ["k_g2p_strain<0>"]},
+    "particle": {"bin": ["k_scan_reduce", "k_scan_blocks", "k_scan_apply", "k_scatter", "k_cell_sort"],
+                 "p2g_grid": ["k_p2g_grid"], "g2p_velocity": ["k_g2p_velocity"], "remap_grid": ["k_remap_grid"],
+                 "g2p_strain": ["k_g2p_strain<0>"]},
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_honeycomb8m.txt")
 
 
-def read_traffic(kernel):
-    """HBM bytes per launch measured with rocprofv3 --pmc, if a summary is committed."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_honeycomb8m_traffic.txt")
-    if not os.path.exists(path):
-        return None
-    vals = {}
-    for line in open(path):
-        if ":" in line and not line.startswith("#"):
-            k, v = line.split(":", 1)
-            vals[k.strip()] = v.strip()
-    try:
-        return float(vals["hbm_bytes_per_launch"])
-    except (KeyError, ValueError):
-        return None
+def read_traffic():
+    """HBM bytes per launch per kernel, from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a copy of known size)."""
+    out = {}
+    if not os.path.exists(TRAFFIC_FILE):
+        return out
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("#") or "hbm_bytes_per_launch[" not in line:
+            continue
+        tail = line.split("hbm_bytes_per_launch[", 1)[1]
+        name, val = tail.split("]:", 1)
+        out[name.strip()] = float(val)
+    return out
 
 
 def cpu_baseline(scene, threads, steps=20, warmup=3):
@@ -79,6 +86,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--order", default="cell", choices=["cell", "particle"],
                     help="node summation order (cell: fast, particle: the reference's one-thread order)")
+    ap.add_argument("--variant-steps", type=int, default=10,
+                    help="timed steps of the contact and single-pass loops on the same state (0: skip)")
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="first run one device copy of this many GiB (the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -126,19 +135,39 @@ def main():
     health = eng.health()
     steps_total = args.warmup + args.steps + args.profile_steps
     plastic_frac = health["plastic_points"] / max(1, scene.n * steps_total)
+    kernels = KERNELS[args.order]
+    measured = read_traffic() if args.order == "cell" else {}
     per_kernel = {}
     for k, ms in avg_ms.items():
         b = PER_PARTICLE[k] * scene.n + PER_NODE[k] * active
-        per_kernel[k] = dict(ms=round(ms, 4), algorithmic_bytes=b, gbs=round(b / (ms * 1e-3) / 1e9, 1) if b else None)
-    dominant = max(avg_ms, key=avg_ms.get)
-    if PER_PARTICLE[dominant] == 0:
-        dominant = max((k for k in avg_ms if PER_PARTICLE[k]), key=avg_ms.get)
+        t = [measured.get(kn) for kn in kernels[k]]
+        per_kernel[k] = dict(ms=round(ms, 4), kernels=kernels[k], algorithmic_bytes=b,
+                             gbs=round(b / (ms * 1e-3) / 1e9, 1) if b else None,
+                             traffic=round(sum(t)) if t and all(x is not None for x in t) else None)
+    # the dominant single kernel (a phase made of one kernel), so the figure
+    # matches one line of the rocprofv3 kernel summary
+    single = [k for k in avg_ms if len(kernels[k]) == 1 and PER_PARTICLE[k]]
+    dominant = max(single, key=avg_ms.get)
     dom_bytes = per_kernel[dominant]["algorithmic_bytes"]
     achieved = dom_bytes / (avg_ms[dominant] * 1e-3) / 1e9
-    traffic = read_traffic(KERNEL_NAME[dominant])
+    traffic = per_kernel[dominant]["traffic"]
     step_bytes = 768 * scene.n + 272 * active
     for k, v in per_kernel.items():
-        print(f"# {k:13s} {v['ms']:8.3f} ms  {v['gbs']} GB/s", file=sys.stderr)
+        print(f"# {k:13s} {v['ms']:8.3f} ms  {v['gbs']} GB/s  traffic {v['traffic']}", file=sys.stderr)
+
+    variants = {}
+    for var in (("doublepass_contact", "singlepass") if args.variant_steps > 0 else ()):
+        eng.set_variant(var)
+        eng.step(scene.dt, 2)
+        eng.synchronize()
+        t1 = time.perf_counter()
+        eng.step(scene.dt, args.variant_steps)
+        eng.synchronize()
+        el = time.perf_counter() - t1
+        variants[var] = {"value": round(scene.n * args.variant_steps / el, 1), "unit": "particle-steps/s",
+                         "ms_per_step": round(1e3 * el / args.variant_steps, 4), "steps": args.variant_steps}
+        print(f"# {var}: {variants[var]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    eng.set_variant("doublepass")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -157,12 +186,13 @@ def main():
                                "(von Mises hardening), platen 10 m/s, seed 1",
                    "particles": scene.n, "active_nodes": active, "plastic_fraction": round(plastic_frac, 5),
                    "return_map": "exact", "summation_order": args.order},
-        "roofline": {"bound": "hbm", "kernel": KERNEL_NAME[dominant], "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": kernels[dominant][0], "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes": dom_bytes,
                      "avg_launch_ms": round(avg_ms[dominant], 4)},
         "step_effective_gbs": round(step_bytes / (elapsed / args.steps) / 1e9, 1),
-        "phases_ms": {k: v["ms"] for k, v in per_kernel.items()},
+        "phases": per_kernel,
+        "variants": variants,
         "cpu_baseline": cpu,
         "gpu_over_cpu": round(value / cpu["value"], 1) if cpu else None,
     }

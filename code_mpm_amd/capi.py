@@ -25,6 +25,7 @@ PF = dict(
 )
 NF = dict(MASS=(0, 1), MOMENTUM=(1, 3), VELOCITY=(2, 3), FORCE=(3, 3), FORCE_TEMP=(4, 3), ACTIVE=(5, 1))
 MON = dict(DISPLACEMENT=0, MOMENTUM=1, ENERGY=2, STRESS=3)
+VARIANT = dict(doublepass=0, doublepass_contact=1, singlepass=2)
 ORDER = dict(particle=0, cell=1)
 PHASES = ("bin", "p2g_grid", "g2p_velocity", "remap_grid", "g2p_strain")
 
@@ -108,6 +109,8 @@ SIGNATURES = {
     "mpm_step": (I32, [P, DBL, I32]),
     "mpm_step_graph": (I32, [P, I32]),
     "mpm_set_summation_order": (I32, [P, I32]),
+    "mpm_set_variant": (I32, [P, I32]),
+    "mpm_get_variant": (I32, [P, PI32]),
     "mpm_set_tile_shape": (I32, [P, PI32]),
     "mpm_get_tile_shape": (I32, [P, PI32]),
     "mpm_autotune": (I32, [P, I32, PI32, PDBL]),

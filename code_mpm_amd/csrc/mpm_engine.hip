@@ -114,6 +114,7 @@ struct mpm_engine {
 
     /* node summation order and the cell-order partial sums */
     int order = ORDER_CELL;
+    int variant = VAR_DP;
     CellParts parts{};
     int64_t parts_cap = 0;
 
@@ -300,7 +301,19 @@ struct mpm_engine {
         b.tx = tile[0];
         b.ty = tile[1];
         b.tz = tile[2];
+        b.variant = variant;
         return b;
+    }
+
+    /* Phase order of the step loop: DoublePass and its contact variant run
+     * P2G + grid, G2P velocity, remap + BC, G2P strain; the single pass
+     * runs its velocity sum + grid update (the remap phase) before its one
+     * grid-to-particle pass (velocity, then strain). */
+    void phase_order(int ph[4]) const {
+        ph[0] = PHASE_P2G_GRID;
+        ph[1] = variant == VAR_SP ? PHASE_REMAP_GRID : PHASE_G2P_VELOCITY;
+        ph[2] = variant == VAR_SP ? PHASE_G2P_VELOCITY : PHASE_REMAP_GRID;
+        ph[3] = PHASE_G2P_STRAIN;
     }
 
     /* Tile offsets of every instance for the current tile shape. */
@@ -416,7 +429,8 @@ struct mpm_engine {
     }
 
     /* The kernels of one step (no host work).  With ev != nullptr an event is
-     * recorded before every phase and after the last (ev[0..PHASE_COUNT]). */
+     * recorded before the binning (ev[0]), before each later phase in launch
+     * order (ev[1..4], see phase_order) and after the last (ev[PHASE_COUNT]). */
     int enqueue_step(hipEvent_t *ev = nullptr) {
         const Batch b = batch();
         int rc;
@@ -424,8 +438,11 @@ struct mpm_engine {
         if (ev && (rc = hip(hipEventRecord(ev[0], stream), "event"))) return rc;
         if ((rc = hip(launch_bin(pview(), bview(), bblock, stream), "cell binning"))) return rc;
         const SurrogateArgs sa = sargs();
-        for (int ph = PHASE_P2G_GRID; ph < PHASE_COUNT; ph++) {
-            if (ev && (rc = hip(hipEventRecord(ev[ph], stream), "event"))) return rc;
+        int order_of[4];
+        phase_order(order_of);
+        for (int q = 0; q < 4; q++) {
+            const int ph = order_of[q];
+            if (ev && (rc = hip(hipEventRecord(ev[1 + q], stream), "event"))) return rc;
             if (ph == PHASE_G2P_STRAIN && sur && (rc = hip(launch_zero_u32(qcount, 1, stream), "queue reset")))
                 return rc;
             if ((rc = hip(launch_phase(ph, b, pview(), nview(), bview(), parts, order, mt, d_health, sa, stream),
@@ -470,14 +487,21 @@ struct mpm_engine {
         return MPM_OK;
     }
 
-    /* Mean milliseconds of `reps` launches of one phase. */
-    int time_phase(int phase, int reps, const SurrogateArgs &sa, hipEvent_t e0, hipEvent_t e1, double &ms) {
+    /* Mean milliseconds of `reps` launches of one phase (or of the phase
+     * pair phase, phase2 when phase2 >= 0). */
+    int time_phase(int phase, int reps, const SurrogateArgs &sa, hipEvent_t e0, hipEvent_t e1, double &ms,
+                   int phase2 = -1) {
         int rc;
         if ((rc = hip(hipEventRecord(e0, stream), "event"))) return rc;
-        for (int r = 0; r < reps; r++)
+        for (int r = 0; r < reps; r++) {
             if ((rc = hip(launch_phase(phase, batch(), pview(), nview(), bview(), parts, order, mt, d_health, sa, stream),
                           "autotune phase")))
                 return rc;
+            if (phase2 >= 0 &&
+                (rc = hip(launch_phase(phase2, batch(), pview(), nview(), bview(), parts, order, mt, d_health, sa,
+                                       stream), "autotune phase")))
+                return rc;
+        }
         if ((rc = hip(hipEventRecord(e1, stream), "event")) || (rc = hip(hipEventSynchronize(e1), "event sync")))
             return rc;
         float f = 0.f;
@@ -511,24 +535,44 @@ struct mpm_engine {
         std::vector<double> t_p2g(kNumTileShapes, 0.0), t_remap(kNumTileShapes, 0.0);
         rc = hip(launch_prepare(batch(), stream), "step prepare");
         if (!rc) rc = hip(launch_bin(pview(), bview(), bblock, stream), "cell binning");
-        for (int k = 0; k < kNumTileShapes && !rc; k++) {
-            rc = set_tile(kTileShapes[k]);
-            if (!rc) rc = time_phase(PHASE_P2G_GRID, reps, sa, e0, e1, t_p2g[k]);
-        }
-        /* G2P pass 1 must see the node state of one P2G pass: redo it with the current shape */
         double once = 0.0;
-        if (!rc) rc = time_phase(PHASE_P2G_GRID, 1, sa, e0, e1, once);
-        if (!rc) rc = hip(launch_phase(PHASE_G2P_VELOCITY, batch(), pview(), nview(), bview(), parts, order, mt, d_health,
-                                       sa, stream), "g2p velocity");
-        for (int k = 0; k < kNumTileShapes && !rc; k++) {
-            rc = set_tile(kTileShapes[k]);
-            if (!rc) rc = time_phase(PHASE_REMAP_GRID, reps, sa, e0, e1, t_remap[k]);
-        }
         int best = 0;
-        for (int k = 1; k < kNumTileShapes; k++)
-            if (t_p2g[k] + t_remap[k] < t_p2g[best] + t_remap[best]) best = k;
-        if (!rc) rc = set_tile(kTileShapes[best]);
-        if (!rc) rc = time_phase(PHASE_REMAP_GRID, 1, sa, e0, e1, once);
+        if (variant == VAR_SP) {
+            /* the single pass's grid update reads the P2G sums and updates
+             * them in place: time it after a fresh P2G each time */
+            for (int k = 0; k < kNumTileShapes && !rc; k++) {
+                double pair = 0.0;
+                rc = set_tile(kTileShapes[k]);
+                if (!rc) rc = time_phase(PHASE_P2G_GRID, reps, sa, e0, e1, t_p2g[k]);
+                if (!rc) rc = time_phase(PHASE_P2G_GRID, reps, sa, e0, e1, pair, PHASE_REMAP_GRID);
+                t_remap[k] = pair - t_p2g[k];
+            }
+            for (int k = 1; k < kNumTileShapes; k++)
+                if (t_p2g[k] + t_remap[k] < t_p2g[best] + t_remap[best]) best = k;
+            if (!rc) rc = set_tile(kTileShapes[best]);
+            if (!rc) rc = time_phase(PHASE_P2G_GRID, 1, sa, e0, e1, once, PHASE_REMAP_GRID);
+            if (!rc) rc = hip(launch_phase(PHASE_G2P_VELOCITY, batch(), pview(), nview(), bview(), parts, order, mt,
+                                           d_health, sa, stream), "g2p velocity");
+        } else {
+            /* both node phases only read particle and bin state and
+             * overwrite node state */
+            for (int k = 0; k < kNumTileShapes && !rc; k++) {
+                rc = set_tile(kTileShapes[k]);
+                if (!rc) rc = time_phase(PHASE_P2G_GRID, reps, sa, e0, e1, t_p2g[k]);
+            }
+            /* G2P pass 1 must see the node state of one P2G pass: redo it with the current shape */
+            if (!rc) rc = time_phase(PHASE_P2G_GRID, 1, sa, e0, e1, once);
+            if (!rc) rc = hip(launch_phase(PHASE_G2P_VELOCITY, batch(), pview(), nview(), bview(), parts, order, mt,
+                                           d_health, sa, stream), "g2p velocity");
+            for (int k = 0; k < kNumTileShapes && !rc; k++) {
+                rc = set_tile(kTileShapes[k]);
+                if (!rc) rc = time_phase(PHASE_REMAP_GRID, reps, sa, e0, e1, t_remap[k]);
+            }
+            for (int k = 1; k < kNumTileShapes; k++)
+                if (t_p2g[k] + t_remap[k] < t_p2g[best] + t_remap[best]) best = k;
+            if (!rc) rc = set_tile(kTileShapes[best]);
+            if (!rc) rc = time_phase(PHASE_REMAP_GRID, 1, sa, e0, e1, once);
+        }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (rc) return rc;
@@ -1016,6 +1060,24 @@ int mpm_step(mpm_engine *e, double dt, int32_t nsteps) {
     return MPM_OK;
 }
 
+int mpm_set_variant(mpm_engine *e, int32_t variant) {
+    CHECK_E(e);
+    if (variant < MPM_VARIANT_DOUBLEPASS || variant > MPM_VARIANT_SINGLEPASS)
+        return e->fail(MPM_ERR_ARG, "unknown variant %d", variant);
+    static_assert(MPM_VARIANT_DOUBLEPASS == VAR_DP && MPM_VARIANT_DOUBLEPASS_CONTACT == VAR_DPC &&
+                      MPM_VARIANT_SINGLEPASS == VAR_SP, "variant codes disagree");
+    (void)hipStreamSynchronize(e->stream);
+    e->drop_graphs();
+    e->variant = variant;
+    return MPM_OK;
+}
+
+int mpm_get_variant(const mpm_engine *e, int32_t *variant) {
+    if (!e || !variant) return MPM_ERR_ARG;
+    *variant = e->variant;
+    return MPM_OK;
+}
+
 int mpm_set_summation_order(mpm_engine *e, int32_t order) {
     CHECK_E(e);
     if (order != MPM_ORDER_PARTICLE && order != MPM_ORDER_CELL) return e->fail(MPM_ERR_ARG, "unknown order %d", order);
@@ -1067,10 +1129,12 @@ int mpm_step_timed(mpm_engine *e, double dt, int32_t nsteps, double phase_ms[MPM
     for (int s = 0; s < nsteps && !rc; s++) {
         rc = e->step_once(dt > 0.0 ? &dt : nullptr, ev);
         if (!rc) rc = e->hip(hipEventSynchronize(ev[PHASE_COUNT]), "event sync");
+        int order_of[4];
+        e->phase_order(order_of);
         for (int k = 0; k < PHASE_COUNT && !rc; k++) {
             float ms = 0.f;
             rc = e->hip(hipEventElapsedTime(&ms, ev[k], ev[k + 1]), "event time");
-            phase_ms[k] += ms;
+            phase_ms[k == 0 ? PHASE_BIN : order_of[k - 1]] += ms;
         }
     }
     for (int k = 0; k <= PHASE_COUNT; k++)

@@ -80,6 +80,8 @@ struct BView {
 
 /* Node summation order. */
 enum { ORDER_PARTICLE = 0, ORDER_CELL = 1 };
+/* step loops: DoublePass, DoublePass_Contact, SinglePass (same codes as MPM_VARIANT_*) */
+enum { VAR_DP = 0, VAR_DPC = 1, VAR_SP = 2 };
 
 /* Per (occupied cell, corner) partial sums of the cell order, component
  * stride cap8 = 8 x occupied-cell capacity: P2G m, p3, f3; the remap reuses
@@ -114,6 +116,7 @@ struct Batch {
     uint32_t ntiles;
     int64_t nnodes;           /* all instances: component stride of node arrays */
     int tx, ty, tz;           /* node tile shape, tx * ty * tz == NODE_BLOCK */
+    int variant;              /* VAR_* */
 };
 
 struct MonLists {

@@ -282,10 +282,12 @@ struct DCWin {
     double v[3];
 };
 
-__device__ __forceinline__ void dc_consider(DCWin &w, const PView &P, uint32_t id) {
+/* A controlled particle claims its flagged axes (DoublePass, DP:143-174) or
+ * all three (DPC:283-297, SP:251-268); the highest list rank wins. */
+__device__ __forceinline__ void dc_consider(DCWin &w, const PView &P, uint32_t id, bool all_axes) {
     int r = P.dc_rank[id];
     if (r < 0) return;
-    uint8_t ax = P.dc_axes[id];
+    uint8_t ax = all_axes ? (uint8_t)7 : P.dc_axes[id];
 #pragma unroll
     for (int a = 0; a < 3; a++)
         if ((ax >> a) & 1)
@@ -359,6 +361,148 @@ __device__ __forceinline__ void node_integrate(const Inst &I, const NView &Nd, i
     Nd.fty[node] = fty;
 }
 
+/* Grid update of the contact loop on one node's sums, DPC:242-297:
+ * damping along p (as DP:100-101), p += f dt, the fixed components zeroed
+ * with the y reaction taken first (no friction), then displacement control
+ * on all axes.  Stores m, p, f, f/m and the reaction. */
+__device__ __forceinline__ void node_integrate_contact(const Inst &I, const NView &Nd, int64_t node, size_t nn,
+                                                       double m, double p[3], double f[3], const DCWin &w) {
+    const double dt = I.dt;
+    {
+        double pp = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+        if (sqrt(pp) > 1.0e-24) {
+            double inv = 1.0 / sqrt(pp);
+            double nx = p[0] * inv, ny = p[1] * inv, nz = p[2] * inv;
+            double sc = I.damping * fabs(f[0] * nx + f[1] * ny + f[2] * nz);
+            f[0] -= sc * nx;
+            f[1] -= sc * ny;
+            f[2] -= sc * nz;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) p[a] += f[a] * dt;
+    const uint8_t fixed = Nd.fixed[node];
+    double fty = 0.0;
+    if (fixed & 1) {
+        p[0] = 0.0;
+        f[0] = 0.0;
+    }
+    if (fixed & 2) {
+        p[1] = 0.0;
+        fty = 0.0 + f[1];
+        f[1] = 0.0;
+    }
+    if (fixed & 4) {
+        p[2] = 0.0;
+        f[2] = 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        if (w.rank[a] >= 0) {
+            p[a] = m * w.v[a];
+            f[a] = 0.0;
+        }
+    Nd.active[node] = 1;
+    Nd.m[node] = m;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        Nd.p[a * nn + node] = p[a];
+        Nd.f[a * nn + node] = f[a];
+        Nd.a[a * nn + node] = f[a] / m;
+    }
+    Nd.fty[node] = fty;
+}
+
+/* Single pass, first half of the grid work (SP:108-204): the node keeps
+ * the sums; momentum only where the mass exceeds the minimum (SP:170-174). */
+__device__ __forceinline__ void node_store_singlepass(const Inst &I, const NView &Nd, int64_t node, size_t nn,
+                                                      double m, double p[3], double f[3]) {
+    const bool massive = m > I.mass_min;
+    Nd.active[node] = 1;
+    Nd.m[node] = m;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        Nd.p[a * nn + node] = massive ? p[a] : 0.0;
+        Nd.f[a * nn + node] = f[a];
+    }
+}
+
+/* Single pass, grid update on the mass-weighted velocity v (SP:210-268):
+ * damping along v with |f|, v += f / m dt above the mass minimum, p += f dt,
+ * fixed components zeroed with the y reaction taken first, then
+ * displacement control on all axes.  Reads the sums stored by
+ * node_store_singlepass; stores v, p, f, f/m, p/m and the reaction. */
+__device__ __forceinline__ void node_update_singlepass(const Inst &I, const NView &Nd, int64_t node, size_t nn,
+                                                       double m, double v[3], const DCWin &w) {
+    const double dt = I.dt;
+    double p[3] = {Nd.p[node], Nd.p[nn + node], Nd.p[2 * nn + node]};
+    double f[3] = {Nd.f[node], Nd.f[nn + node], Nd.f[2 * nn + node]};
+    {
+        double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        if (sqrt(vv) > 1.0e-9) {
+            double lf = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+            double inv = 1.0 / sqrt(vv);
+            double nx = v[0] * inv, ny = v[1] * inv, nz = v[2] * inv;
+            double sc = I.damping * lf;
+            f[0] -= sc * nx;
+            f[1] -= sc * ny;
+            f[2] -= sc * nz;
+        }
+    }
+    if (m > I.mass_min) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) v[a] += f[a] / m * dt;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) p[a] += f[a] * dt;
+    const uint8_t fixed = Nd.fixed[node];
+    double fty = 0.0;
+    if (fixed & 1) {
+        v[0] = 0.0;
+        p[0] = 0.0;
+        f[0] = 0.0;
+    }
+    if (fixed & 2) {
+        v[1] = 0.0;
+        p[1] = 0.0;
+        fty = 0.0 + f[1];
+        f[1] = 0.0;
+    }
+    if (fixed & 4) {
+        v[2] = 0.0;
+        p[2] = 0.0;
+        f[2] = 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        if (w.rank[a] >= 0) {
+            v[a] = w.v[a];
+            p[a] = m * w.v[a];
+            f[a] = 0.0;
+        }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        Nd.v[a * nn + node] = v[a];
+        Nd.p[a * nn + node] = p[a];
+        Nd.f[a * nn + node] = f[a];
+        Nd.a[a * nn + node] = f[a] / m;
+        Nd.u[a * nn + node] = p[a] / m;
+    }
+    Nd.fty[node] = fty;
+}
+
+/* The node side of the particle-to-grid phase for each loop. */
+__device__ __forceinline__ void node_p2g_finish(const Batch &bt, const Inst &I, const NView &Nd, int64_t node,
+                                                double m, double p[3], double f[3], const DCWin &w) {
+    const size_t nn = (size_t)bt.nnodes;
+    if (bt.variant == VAR_DPC)
+        node_integrate_contact(I, Nd, node, nn, m, p, f, w);
+    else if (bt.variant == VAR_SP)
+        node_store_singlepass(I, Nd, node, nn, m, p, f);
+    else
+        node_integrate(I, Nd, node, nn, m, p, f, w);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd, BView B) {
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const Inst &I = bt.inst[inst_of_tile(bt, tile)];
@@ -381,7 +525,14 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd,
     DCWin w;
     w.rank[0] = w.rank[1] = w.rank[2] = -1;
     w.v[0] = w.v[1] = w.v[2] = 0.0;
+    const bool all_axes = bt.variant != VAR_DP, skip_dc = bt.variant == VAR_SP;
     while (id != MPM_NONE) {
+        if (skip_dc && P.dc_rank[id] >= 0) { /* SP:116-117, 157-158 */
+            dc_consider(w, P, id, all_axes);
+            merge_advance(M, B, id);
+            id = merge_min(M);
+            continue;
+        }
         /* runSimulation_M2G body, DP:69-84 */
         AxisShape ax = axis_shape(P.x[id], xn[0], g.h[0]);
         AxisShape ay = axis_shape(P.x[n + id], xn[1], g.h[1]);
@@ -405,11 +556,11 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd,
             f[1] += N * P.fext[n + id];
             f[2] += N * P.fext[2 * n + id];
         }
-        dc_consider(w, P, id);
+        dc_consider(w, P, id, all_axes);
         merge_advance(M, B, id);
         id = merge_min(M);
     }
-    node_integrate(I, Nd, node, (size_t)bt.nnodes, m, p, f, w);
+    node_p2g_finish(bt, I, Nd, node, m, p, f, w);
 }
 
 /* G2P pass 1: v_p += N * (f_i / m_i) * dt over the adjacent nodes in
@@ -417,6 +568,7 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd,
 __global__ __launch_bounds__(BLOCK) void k_g2p_velocity(Batch bt, PView P, NView Nd) {
     uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= P.n) return;
+    if (bt.variant != VAR_DP && P.dc_rank[i] >= 0) return; /* DPC:309-310, SP:279-280 */
     const Inst &I = bt.inst[inst_of_particle(bt, i)];
     const GridGeom &g = I.g;
     const size_t n = P.n, nn = (size_t)bt.nnodes;
@@ -478,6 +630,34 @@ __device__ __forceinline__ void node_remap_finish(const NView &Nd, int64_t node,
     }
 }
 
+/* Contact loop: BC re-applied (DPC:380-411), then displacement control on
+ * all axes (DPC:413-433), so control wins over fixed components. */
+__device__ __forceinline__ void node_remap_finish_contact(const NView &Nd, int64_t node, size_t nn, double m,
+                                                          double v[3], const DCWin &w) {
+    double p[3] = {Nd.p[node], Nd.p[nn + node], Nd.p[2 * nn + node]};
+    const uint8_t fixed = Nd.fixed[node];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        if ((fixed >> a) & 1) {
+            v[a] = 0.0;
+            p[a] = 0.0;
+        }
+        if (w.rank[a] >= 0) {
+            v[a] = w.v[a];
+            p[a] = m * w.v[a];
+            Nd.f[a * nn + node] = 0.0;
+        }
+        Nd.v[a * nn + node] = v[a];
+        Nd.p[a * nn + node] = p[a];
+        Nd.u[a * nn + node] = p[a] / m;
+    }
+}
+
+/* The node side of the remap phase for each loop (the single pass runs it
+ * before its only grid-to-particle pass, as its velocity sum). */
+__device__ __forceinline__ void node_remap_dispatch(const Batch &bt, const Inst &I, const NView &Nd, int64_t node,
+                                                    double m, double v[3], const DCWin &w);
+
 /* Velocity remap (DP:539-564) gathered per node in ascending particle
  * order, then the second displacement-control override (DP:568) and the BC
  * re-application (DP:573-599). Stores the final v, p, the zeroed force
@@ -502,8 +682,9 @@ __global__ __launch_bounds__(BLOCK) void k_remap_grid(Batch bt, PView P, NView N
     DCWin w;
     w.rank[0] = w.rank[1] = w.rank[2] = -1;
     w.v[0] = w.v[1] = w.v[2] = 0.0;
+    const bool all_axes = bt.variant != VAR_DP, skip_dc = bt.variant != VAR_DP;
     for (uint32_t id = merge_min(M); id != MPM_NONE; id = merge_min(M)) {
-        if (massive) {
+        if (massive && !(skip_dc && P.dc_rank[id] >= 0)) { /* DPC:358-359, SP:157-158 */
             double N = axis_shape(P.x[id], xn[0], g.h[0]).s * axis_shape(P.x[n + id], xn[1], g.h[1]).s *
                        axis_shape(P.x[2 * n + id], xn[2], g.h[2]).s;
             double mp = P.m[id];
@@ -511,10 +692,21 @@ __global__ __launch_bounds__(BLOCK) void k_remap_grid(Batch bt, PView P, NView N
             v[1] += N * (mp * P.v[n + id]) / m;
             v[2] += N * (mp * P.v[2 * n + id]) / m;
         }
-        dc_consider(w, P, id);
+        dc_consider(w, P, id, all_axes);
         merge_advance(M, B, id);
     }
-    node_remap_finish(Nd, node, nn, m, v, w);
+    node_remap_dispatch(bt, I, Nd, node, m, v, w);
+}
+
+__device__ __forceinline__ void node_remap_dispatch(const Batch &bt, const Inst &I, const NView &Nd, int64_t node,
+                                                    double m, double v[3], const DCWin &w) {
+    const size_t nn = (size_t)bt.nnodes;
+    if (bt.variant == VAR_DPC)
+        node_remap_finish_contact(Nd, node, nn, m, v, w);
+    else if (bt.variant == VAR_SP)
+        node_update_singlepass(I, Nd, node, nn, m, v, w);
+    else
+        node_remap_finish(Nd, node, nn, m, v, w);
 }
 
 /* ---- cell order: per (occupied cell, corner) partial sums, then nodes ----
@@ -550,10 +742,22 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
         int brank[3] = {-1, -1, -1};
         uint32_t bid[3] = {MPM_NONE, MPM_NONE, MPM_NONE};
         const size_t j = (size_t)t;
+        const bool all_axes = bt.variant != VAR_DP;
         if (!REMAP) {
             double m = 0.0, p[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0};
+            const bool skip_dc = bt.variant == VAR_SP;
             for (uint32_t q = s0; q < s1; q++) {
                 const uint32_t id = B.sorted[q];
+                const int r = P.dc_rank[id];
+                if (r >= 0) {
+                    uint8_t axm = all_axes ? (uint8_t)7 : P.dc_axes[id];
+                    for (int a = 0; a < 3; a++)
+                        if (((axm >> a) & 1) && r > brank[a]) {
+                            brank[a] = r;
+                            bid[a] = id;
+                        }
+                    if (skip_dc) continue; /* SP:116-117, 157-158 */
+                }
                 AxisShape ax = axis_shape(P.x[id], xn[0], g.h[0]);
                 AxisShape ay = axis_shape(P.x[n + id], xn[1], g.h[1]);
                 AxisShape az = axis_shape(P.x[2 * n + id], xn[2], g.h[2]);
@@ -576,15 +780,6 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
                     f[1] += N * P.fext[n + id];
                     f[2] += N * P.fext[2 * n + id];
                 }
-                int r = P.dc_rank[id];
-                if (r >= 0) {
-                    uint8_t axm = P.dc_axes[id];
-                    for (int a = 0; a < 3; a++)
-                        if (((axm >> a) & 1) && r > brank[a]) {
-                            brank[a] = r;
-                            bid[a] = id;
-                        }
-                }
             }
             C.m[j] = m;
             for (int a = 0; a < 3; a++) {
@@ -596,9 +791,11 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
             const double mn = Nd.m[node];
             const bool massive = mn > I.mass_min;
             double v[3] = {0.0, 0.0, 0.0};
+            const bool skip_dc = bt.variant != VAR_DP;
             for (uint32_t q = s0; q < s1; q++) {
                 const uint32_t id = B.sorted[q];
-                if (massive) {
+                const int r = P.dc_rank[id];
+                if (massive && !(skip_dc && r >= 0)) { /* DPC:358-359, SP:157-158 */
                     double N = axis_shape(P.x[id], xn[0], g.h[0]).s * axis_shape(P.x[n + id], xn[1], g.h[1]).s *
                                axis_shape(P.x[2 * n + id], xn[2], g.h[2]).s;
                     double mp = P.m[id];
@@ -606,9 +803,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
                     v[1] += N * (mp * P.v[n + id]) / mn;
                     v[2] += N * (mp * P.v[2 * n + id]) / mn;
                 }
-                int r = P.dc_rank[id];
                 if (r >= 0) {
-                    uint8_t axm = P.dc_axes[id];
+                    uint8_t axm = all_axes ? (uint8_t)7 : P.dc_axes[id];
                     for (int a = 0; a < 3; a++)
                         if (((axm >> a) & 1) && r > brank[a]) {
                             brank[a] = r;
@@ -669,9 +865,9 @@ __global__ __launch_bounds__(BLOCK) void k_node_cells(Batch bt, PView P, NView N
             Nd.active[node] = 0;
             return;
         }
-        node_integrate(I, Nd, node, nn, m, p, f, w);
+        node_p2g_finish(bt, I, Nd, node, m, p, f, w);
     } else {
-        node_remap_finish(Nd, node, nn, Nd.m[node], f, w);
+        node_remap_dispatch(bt, I, Nd, node, Nd.m[node], f, w);
     }
 }
 
@@ -711,6 +907,34 @@ __device__ __forceinline__ uint32_t finish_update(const PView &P, size_t i, cons
     return bad;
 }
 
+/* Stress, plastic strain, energies and backstress of the contact and
+ * single-pass loops (DPC:514-524, SP:351-362): energies accumulate
+ * sigma_new[k] * de[k] * V one component at a time; no history maximum or
+ * time integral.  The strain was already updated. */
+__device__ __forceinline__ uint32_t finish_update_new_stress(const PView &P, size_t i, double sig[6],
+                                                             const CRResult &cr, double R, double V,
+                                                             const double de[6]) {
+    const size_t n = P.n;
+#pragma unroll
+    for (int k = 0; k < 6; k++) sig[k] += cr.dsig[k];
+#pragma unroll
+    for (int k = 0; k < 6; k++) P.sig[k * n + i] = sig[k];
+#pragma unroll
+    for (int k = 0; k < 6; k++) P.epsp[k * n + i] += cr.depsp[k];
+    double Ws = P.Ws[i];
+#pragma unroll
+    for (int k = 0; k < 6; k++) Ws += sig[k] * de[k] * V;
+    P.Ws[i] = Ws;
+    double Wp = P.Wp[i];
+#pragma unroll
+    for (int k = 0; k < 6; k++) Wp += sig[k] * cr.depsp[k] * V;
+    P.Wp[i] = Wp;
+    P.R[i] = R + cr.dR;
+    uint32_t bad = 0;
+    for (int k = 0; k < 6; k++) bad |= !isfinite(sig[k]);
+    return bad;
+}
+
 /* Trial state of the radial return: J2 of sigma + C:de and the yield radius
  * (the first half of ConstitutiveRelation_calculateIncrement_{PerfectlyPlastic,
  * VonMisesHardening}.cpp, identical expressions). */
@@ -738,7 +962,7 @@ __device__ __forceinline__ void trial_state(const MatTable &mt, int m, double R,
  *   MODE_SURROGATE  points of the surrogate's material whose trial state
  *                   yields are queued for the MLP; everything else, and
  *                   all kinematics, is finished here. */
-template <int MODE>
+template <int MODE, int VAR>
 __global__ __launch_bounds__(BLOCK) void k_g2p_strain(Batch bt, PView P, NView Nd, BView B, MatTable mt,
                                                      unsigned long long *health, SurrogateArgs sg) {
     uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
@@ -764,136 +988,199 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_strain(Batch bt, PView P, NView N
 #pragma unroll
             for (int r = 0; r < 3; r++) L[c][r] = 0.0;
         const double dt = I.dt;
-        int touched = 0;
         bool ok[3][2];
 #pragma unroll
         for (int a = 0; a < 3; a++)
 #pragma unroll
             for (int o = 0; o < 2; o++)
                 ok[a][o] = o >= lo[a] && o < hi[a] && b[a] + o >= 0 && b[a] + o <= g.nn[a] - 1;
-        /* corners in GridPoint_Mediator order: x outer, z inner */
+        f_outside = !((ok[0][0] || ok[0][1]) && (ok[1][0] || ok[1][1]) && (ok[2][0] || ok[2][1]));
+        /* the contact and single-pass loops leave controlled particles out
+         * of the grid-to-particle pass (DPC:443-444, SP:279-280) */
+        const bool ctl = VAR != VAR_DP && P.dc_rank[i] >= 0;
+        if (!ctl) {
+            /* corners in GridPoint_Mediator order: x outer, z inner */
 #pragma unroll
-        for (int cc = 0; cc < 8; cc++) {
-            const int ox = cc >> 2, oy = (cc >> 1) & 1, oz = cc & 1;
-            if (!(ok[0][ox] && ok[1][oy] && ok[2][oz])) continue;
-            touched++;
-            size_t node = (size_t)I.node_off + (size_t)(b[0] + ox) +
-                          (size_t)g.nn[0] * ((size_t)(b[1] + oy) + (size_t)g.nn[1] * (b[2] + oz));
-            double N, gr[3];
-            shape_full(as[0][ox], as[1][oy], as[2][oz], g.h, N, gr[0], gr[1], gr[2]);
-            if (Nd.m[node] > I.mass_min) {
-                x[0] += N * Nd.u[node] * dt;
-                x[1] += N * Nd.u[nn + node] * dt;
-                x[2] += N * Nd.u[2 * nn + node] * dt;
+            for (int cc = 0; cc < 8; cc++) {
+                const int ox = cc >> 2, oy = (cc >> 1) & 1, oz = cc & 1;
+                if (!(ok[0][ox] && ok[1][oy] && ok[2][oz])) continue;
+                size_t node = (size_t)I.node_off + (size_t)(b[0] + ox) +
+                              (size_t)g.nn[0] * ((size_t)(b[1] + oy) + (size_t)g.nn[1] * (b[2] + oz));
+                double N, gr[3];
+                shape_full(as[0][ox], as[1][oy], as[2][oz], g.h, N, gr[0], gr[1], gr[2]);
+                if (Nd.m[node] > I.mass_min) {
+                    x[0] += N * Nd.u[node] * dt;
+                    x[1] += N * Nd.u[nn + node] * dt;
+                    x[2] += N * Nd.u[2 * nn + node] * dt;
+                }
+                double vv[3] = {Nd.v[node], Nd.v[nn + node], Nd.v[2 * nn + node]};
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+#pragma unroll
+                    for (int r = 0; r < 3; r++) L[c][r] += vv[r] * gr[c];
             }
-            double vv[3] = {Nd.v[node], Nd.v[nn + node], Nd.v[2 * nn + node]};
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-#pragma unroll
-                for (int r = 0; r < 3; r++) L[c][r] += vv[r] * gr[c];
-        }
-        f_outside = (touched == 0);
-        P.x[i] = x[0];
-        P.x[n + i] = x[1];
-        P.x[2 * n + i] = x[2];
-        const int mat = P.mat[i];
-        if (mat < 0 || mat >= mt.n) {
-            f_null = 1;
-            for (int k = 0; k < 6; k++) f_nonfinite |= !isfinite(P.sig[k * n + i]);
-        } else {
+            P.x[i] = x[0];
+            P.x[n + i] = x[1];
+            P.x[2 * n + i] = x[2];
+            const int mat = P.mat[i];
+            const bool valid = mat >= 0 && mat < mt.n;
             double dF[3][3];
 #pragma unroll
             for (int c = 0; c < 3; c++)
 #pragma unroll
                 for (int r = 0; r < 3; r++) dF[c][r] = (c == r ? 1.0 : 0.0) + L[c][r] * dt;
-            double E[3][3];
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-#pragma unroll
-                for (int r = 0; r < 3; r++) E[c][r] = 0.5 * (dF[r][c] + dF[c][r]) - (c == r ? 1.0 : 0.0);
             double de[6];
-            de[0] = E[0][0];
-            de[1] = E[1][1];
-            de[2] = E[2][2];
-            de[3] = E[0][1] + E[1][0];
-            de[4] = E[1][2] + E[2][1];
-            de[5] = E[2][0] + E[0][2];
-            /* F = F * dF (DP:242), V = det(F) V0 (DP:244-245) */
-            double F[3][3], Fn[3][3];
+            if (VAR == VAR_DP) {
+                double E[3][3];
 #pragma unroll
-            for (int c = 0; c < 3; c++)
+                for (int c = 0; c < 3; c++)
 #pragma unroll
-                for (int r = 0; r < 3; r++) F[c][r] = P.F[(3 * c + r) * n + i];
+                    for (int r = 0; r < 3; r++) E[c][r] = 0.5 * (dF[r][c] + dF[c][r]) - (c == r ? 1.0 : 0.0);
+                de[0] = E[0][0];
+                de[1] = E[1][1];
+                de[2] = E[2][2];
+                de[3] = E[0][1] + E[1][0];
+                de[4] = E[1][2] + E[2][1];
+                de[5] = E[2][0] + E[0][2];
+            } else {
+                /* DPC:469-477, SP:310-318: straight from dF */
+                de[0] = dF[0][0] - 1.0;
+                de[1] = dF[1][1] - 1.0;
+                de[2] = dF[2][2] - 1.0;
+                de[3] = dF[0][1] + dF[1][0];
+                de[4] = dF[1][2] + dF[2][1];
+                de[5] = dF[2][0] + dF[0][2];
+            }
+            /* kinematics: F and V.  DP: F = F dF after the material check
+             * (DP:242-245); DPC / SP: F += (L F) dt before it (DPC:464-467,
+             * SP:305-308) */
+            double V = P.V[i];
+            if (VAR != VAR_DP || valid) {
+                double F[3][3], Fn[3][3];
 #pragma unroll
-            for (int c = 0; c < 3; c++)
+                for (int c = 0; c < 3; c++)
 #pragma unroll
-                for (int r = 0; r < 3; r++) Fn[c][r] = F[0][r] * dF[c][0] + F[1][r] * dF[c][1] + F[2][r] * dF[c][2];
+                    for (int r = 0; r < 3; r++) F[c][r] = P.F[(3 * c + r) * n + i];
+                if (VAR == VAR_DP) {
 #pragma unroll
-            for (int c = 0; c < 3; c++)
+                    for (int c = 0; c < 3; c++)
 #pragma unroll
-                for (int r = 0; r < 3; r++) P.F[(3 * c + r) * n + i] = Fn[c][r];
-            double det = +Fn[0][0] * (Fn[1][1] * Fn[2][2] - Fn[2][1] * Fn[1][2]) -
-                         Fn[1][0] * (Fn[0][1] * Fn[2][2] - Fn[2][1] * Fn[0][2]) +
-                         Fn[2][0] * (Fn[0][1] * Fn[1][2] - Fn[1][1] * Fn[0][2]);
-            const double V = det * P.V0[i];
-            P.V[i] = V;
+                        for (int r = 0; r < 3; r++)
+                            Fn[c][r] = F[0][r] * dF[c][0] + F[1][r] * dF[c][1] + F[2][r] * dF[c][2];
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 3; c++)
+#pragma unroll
+                        for (int r = 0; r < 3; r++)
+                            Fn[c][r] = F[c][r] + (L[0][r] * F[c][0] + L[1][r] * F[c][1] + L[2][r] * F[c][2]) * dt;
+                }
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+#pragma unroll
+                    for (int r = 0; r < 3; r++) P.F[(3 * c + r) * n + i] = Fn[c][r];
+                double det = +Fn[0][0] * (Fn[1][1] * Fn[2][2] - Fn[2][1] * Fn[1][2]) -
+                             Fn[1][0] * (Fn[0][1] * Fn[2][2] - Fn[2][1] * Fn[0][2]) +
+                             Fn[2][0] * (Fn[0][1] * Fn[1][2] - Fn[1][1] * Fn[0][2]);
+                V = det * P.V0[i];
+                P.V[i] = V;
+            }
+            if (VAR != VAR_DP) {
+#pragma unroll
+                for (int k = 0; k < 6; k++) P.eps[k * n + i] += de[k];
+            }
             double sig[6];
 #pragma unroll
             for (int k = 0; k < 6; k++) sig[k] = P.sig[k * n + i];
             const double R = P.R[i];
-            bool queued = false;
-            if (MODE == MODE_SURROGATE && mat == sg.material) {
-                double dtr[6], J2, K;
-                trial_state(mt, mat, R, sig, de, dtr, J2, K);
-                if (!(J2 <= K * K)) {
-                    /* queue for the network: exact strain increment + normalised features */
-                    uint32_t slot = atomicAdd(sg.qcount, 1u);
-                    sg.qid[slot] = i;
-                    const size_t cap = (size_t)P.n;
-#pragma unroll
-                    for (int k = 0; k < 6; k++) sg.qde[k * cap + slot] = de[k];
-                    float feat[MLP_IN];
-                    for (int k = 0; k < 6; k++) feat[k] = (float)(sig[k] / sg.stress_scale);
-                    for (int k = 0; k < 6; k++) feat[6 + k] = (float)(de[k] / sg.strain_scale);
-                    feat[12] = (float)(R / sg.stress_scale);
-                    uint8_t inside = 1;
-                    for (int k = 0; k < MLP_IN; k++) {
-                        float z = (feat[k] - sg.norm.in_mean[k]) / sg.norm.in_std[k];
-                        inside &= (z >= sg.norm.in_lo[k] && z <= sg.norm.in_hi[k]) ? 1 : 0;
-                        sg.qx[(size_t)slot * MLP_IN + k] = z;
-                    }
-                    sg.qenv[slot] = inside;
-                    queued = true;
-                    f_queued = 1;
-                    f_plastic = 1;
-                    /* strain, stress and the rest are finished by the post kernel */
-                }
-            }
-            if (!queued) {
+            if (!valid) f_null = 1;
+            if (!valid && VAR != VAR_SP) {
+                /* DP:226-230, DPC:491-495: message and skip */
+                for (int k = 0; k < 6; k++) f_nonfinite |= !isfinite(sig[k]);
+            } else if (!valid) {
+                /* SP with no material: the per-particle defaults (elastic,
+                 * E = nu = 0, MaterialPoint_BC.h:48-57) */
                 CRResult cr;
-                constitutive(mt, mat, R, sig, de, cr);
-                f_plastic = cr.yielded;
-                if (MODE == MODE_CAPTURE && cr.yielded) {
-                    unsigned long long slot = atomicAdd(sg.capcount, 1ull);
-                    if (slot < (unsigned long long)sg.capcap) {
-                        double *row = sg.cap + slot * 14;
-                        for (int k = 0; k < 6; k++) row[k] = sig[k];
-                        for (int k = 0; k < 6; k++) row[6 + k] = de[k];
-                        row[12] = R;
-                        row[13] = (double)mat;
+                trial_increment(0.0, 0.0, de, cr.dsig);
+                for (int k = 0; k < 6; k++) cr.depsp[k] = 0.0;
+                cr.dR = 0.0;
+                cr.yielded = false;
+                f_nonfinite |= finish_update_new_stress(P, i, sig, cr, R, V, de);
+            } else {
+                bool queued = false;
+                if (MODE == MODE_SURROGATE && mat == sg.material) {
+                    double dtr[6], J2, K;
+                    trial_state(mt, mat, R, sig, de, dtr, J2, K);
+                    if (!(J2 <= K * K)) {
+                        /* queue for the network: exact strain increment + normalised features */
+                        uint32_t slot = atomicAdd(sg.qcount, 1u);
+                        sg.qid[slot] = i;
+                        const size_t cap = (size_t)P.n;
+#pragma unroll
+                        for (int k = 0; k < 6; k++) sg.qde[k * cap + slot] = de[k];
+                        float feat[MLP_IN];
+                        for (int k = 0; k < 6; k++) feat[k] = (float)(sig[k] / sg.stress_scale);
+                        for (int k = 0; k < 6; k++) feat[6 + k] = (float)(de[k] / sg.strain_scale);
+                        feat[12] = (float)(R / sg.stress_scale);
+                        uint8_t inside = 1;
+                        for (int k = 0; k < MLP_IN; k++) {
+                            float z = (feat[k] - sg.norm.in_mean[k]) / sg.norm.in_std[k];
+                            inside &= (z >= sg.norm.in_lo[k] && z <= sg.norm.in_hi[k]) ? 1 : 0;
+                            sg.qx[(size_t)slot * MLP_IN + k] = z;
+                        }
+                        sg.qenv[slot] = inside;
+                        queued = true;
+                        f_queued = 1;
+                        f_plastic = 1;
+                        /* stress and the rest are finished by the post kernel */
                     }
                 }
-                f_nonfinite |= finish_update(P, i, de, sig, cr, R, V, dt);
+                if (!queued) {
+                    CRResult cr;
+                    constitutive(mt, mat, R, sig, de, cr);
+                    f_plastic = cr.yielded;
+                    if (MODE == MODE_CAPTURE && cr.yielded) {
+                        unsigned long long slot = atomicAdd(sg.capcount, 1ull);
+                        if (slot < (unsigned long long)sg.capcap) {
+                            double *row = sg.cap + slot * 14;
+                            for (int k = 0; k < 6; k++) row[k] = sig[k];
+                            for (int k = 0; k < 6; k++) row[6 + k] = de[k];
+                            row[12] = R;
+                            row[13] = (double)mat;
+                        }
+                    }
+                    if (VAR == VAR_DP)
+                        f_nonfinite |= finish_update(P, i, de, sig, cr, R, V, dt);
+                    else
+                        f_nonfinite |= finish_update_new_stress(P, i, sig, cr, R, V, de);
+                }
             }
+            f_nonfinite |= !(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]));
         }
-        f_nonfinite |= !(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]));
-        /* displacement-control schedule, DP:766-789 */
-        if (I.tl_count != 0 && P.dc_rank[i] >= 0) {
-            uint8_t axm = P.dc_axes[i];
-            double mult = (double)P.dc_mult[i];
+        /* displacement-control schedule: DP:766-789 (flagged axes), or
+         * DPC:531-540 / SP:369-378 (all axes, then x += v dt) */
+        if (VAR == VAR_DP) {
+            if (I.tl_count != 0 && P.dc_rank[i] >= 0) {
+                uint8_t axm = P.dc_axes[i];
+                double mult = (double)P.dc_mult[i];
 #pragma unroll
-            for (int a = 0; a < 3; a++)
-                if ((axm >> a) & 1) P.v[a * n + i] = mult * I.vt[a];
+                for (int a = 0; a < 3; a++)
+                    if ((axm >> a) & 1) P.v[a * n + i] = mult * I.vt[a];
+            }
+        } else if (ctl) {
+            double v[3] = {P.v[i], P.v[n + i], P.v[2 * n + i]};
+            if (I.tl_count != 0) {
+                double mult = (double)P.dc_mult[i];
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    v[a] = mult * I.vt[a];
+                    P.v[a * n + i] = v[a];
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                x[a] = x0[a] + v[a] * dt;
+                P.x[a * n + i] = x[a];
+            }
         }
         /* next step's binning */
         uint32_t k = cell_key(g, x[0], x[1], x[2]);
@@ -909,8 +1196,8 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_strain(Batch bt, PView P, NView N
 }
 
 /* Applies the network's output to the queued points: the stress increment
- * is the exact elastic trial increment minus the predicted plastic
- * correction; it is accepted when the inputs were inside the training
+ * is the exact elastic trial increment minus the deviatoric part of the
+ * predicted plastic correction; it is accepted when the inputs were inside the training
  * envelope and the new stress lies on the yield surface within
  * tol * sigma_y; otherwise the exact radial return runs.  The plastic strain
  * and hardening increments follow from the accepted stress increment as in
@@ -936,10 +1223,17 @@ __global__ __launch_bounds__(BLOCK) void k_surrogate_post(Batch bt, PView P, Mat
             CRResult cr;
             bool accept = sg.qenv[slot] != 0;
             const float *y = sg.qy + (size_t)slot * MLP_OUT;
-            for (int k = 0; k < 6; k++) {
-                double corr = ((double)y[k] * (double)sg.norm.out_std[k] + (double)sg.norm.out_mean[k]) * sg.stress_scale;
-                cr.dsig[k] = dtr[k] - corr;
-            }
+            double corr[6];
+            for (int k = 0; k < 6; k++)
+                corr[k] = ((double)y[k] * (double)sg.norm.out_std[k] + (double)sg.norm.out_mean[k]) * sg.stress_scale;
+            /* the radial return keeps the trial pressure (plastic flow is
+             * deviatoric): drop the hydrostatic part of the prediction, which
+             * the yield-surface check below cannot see */
+            const double hyd = (corr[0] + corr[1] + corr[2]) / 3.0;
+            corr[0] -= hyd;
+            corr[1] -= hyd;
+            corr[2] -= hyd;
+            for (int k = 0; k < 6; k++) cr.dsig[k] = dtr[k] - corr[k];
             if (accept) {
                 double s2[6];
                 for (int k = 0; k < 6; k++) s2[k] = sig[k] + cr.dsig[k];
@@ -975,7 +1269,10 @@ __global__ __launch_bounds__(BLOCK) void k_surrogate_post(Batch bt, PView P, Mat
                 constitutive(mt, m, R, sig, de, cr);
                 f_fallback = 1;
             }
-            f_nonfinite = finish_update(P, i, de, sig, cr, R, P.V[i], bt.inst[inst_of_particle(bt, i)].dt);
+            if (bt.variant == VAR_DP)
+                f_nonfinite = finish_update(P, i, de, sig, cr, R, P.V[i], bt.inst[inst_of_particle(bt, i)].dt);
+            else
+                f_nonfinite = finish_update_new_stress(P, i, sig, cr, R, P.V[i], de);
         }
         block_count_add(&health[HEALTH_SURROGATE_FALLBACKS], f_fallback);
         block_count_add(&health[HEALTH_NONFINITE], f_nonfinite);
@@ -1155,6 +1452,17 @@ static unsigned partial_grid(int64_t n) {
     return (unsigned)std::min<int64_t>(b, 8192);
 }
 
+template <int VAR>
+static void launch_g2p_strain(int mode, dim3 grid, hipStream_t s, const Batch &bt, const PView &P, const NView &N,
+                              const BView &B, const MatTable &mt, unsigned long long *health, const SurrogateArgs &sg) {
+    if (mode == MODE_SURROGATE)
+        hipLaunchKernelGGL((k_g2p_strain<MODE_SURROGATE, VAR>), grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health, sg);
+    else if (mode == MODE_CAPTURE)
+        hipLaunchKernelGGL((k_g2p_strain<MODE_CAPTURE, VAR>), grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health, sg);
+    else
+        hipLaunchKernelGGL((k_g2p_strain<MODE_EXACT, VAR>), grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health, sg);
+}
+
 hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView &N, const BView &B,
                         const CellParts &C, int order, const MatTable &mt, unsigned long long *health,
                         const SurrogateArgs &sg, hipStream_t s) {
@@ -1185,12 +1493,12 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
     case PHASE_G2P_STRAIN:
         if (P.n > 0) {
             dim3 grid(blocks_for(P.n, BLOCK));
-            if (sg.mode == MODE_SURROGATE)
-                hipLaunchKernelGGL(k_g2p_strain<MODE_SURROGATE>, grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health, sg);
-            else if (sg.mode == MODE_CAPTURE)
-                hipLaunchKernelGGL(k_g2p_strain<MODE_CAPTURE>, grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health, sg);
+            if (bt.variant == VAR_DPC)
+                launch_g2p_strain<VAR_DPC>(sg.mode, grid, s, bt, P, N, B, mt, health, sg);
+            else if (bt.variant == VAR_SP)
+                launch_g2p_strain<VAR_SP>(sg.mode, grid, s, bt, P, N, B, mt, health, sg);
             else
-                hipLaunchKernelGGL(k_g2p_strain<MODE_EXACT>, grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health, sg);
+                launch_g2p_strain<VAR_DP>(sg.mode, grid, s, bt, P, N, B, mt, health, sg);
         }
         break;
     default:

@@ -25,13 +25,15 @@ from .capi import MON, NF, PF, check, ptr
 class Engine:
     """One simulation on one GPU (one HIP stream)."""
 
-    def __init__(self, device=0, order="cell"):
+    def __init__(self, device=0, order="cell", variant="doublepass"):
         self._lib = capi.load()
         h = ctypes.c_void_p()
         rc = self._lib.mpm_create(ctypes.byref(h), int(device))
         check(rc, None, "mpm_create")
         self._h = h
         self.set_order(order)
+        self.variant = None
+        self.set_variant(variant)
         self.n = 0
         self.nnodes = 0
         self.console_interval = None
@@ -44,6 +46,14 @@ class Engine:
         bit for bit) or "cell" (per-cell partial sums in a fixed order; fast)."""
         self._check(self._lib.mpm_set_summation_order(self._h, capi.ORDER[order]), "set_order")
         self.order = order
+
+    def set_variant(self, variant):
+        """Step loop: "doublepass" (runSimulation_Classic_DoublePass_MPLocks),
+        "doublepass_contact" (..._DoublePass_MPLocks_Contact) or "singlepass"
+        (..._SinglePass_MP)."""
+        if variant != self.variant:
+            self._check(self._lib.mpm_set_variant(self._h, capi.VARIANT[variant]), "set_variant")
+            self.variant = variant
 
     # -- lifetime ---------------------------------------------------------
     def close(self):
@@ -248,6 +258,23 @@ class Engine:
     def runSimulation_Classic_DoublePass_MPLocks(self, dt_total):
         """The reference's step loop with its console report
         (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:439-815)."""
+        self.set_variant("doublepass")
+        return self._run_loop(dt_total)
+
+    def runSimulation_Classic_DoublePass_MPLocks_Contact(self, dt_total):
+        """PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks_Contact.cpp:4-566."""
+        self.set_variant("doublepass_contact")
+        return self._run_loop(dt_total)
+
+    def runSimulation_Classic_SinglePass_MP(self, dt_total):
+        """PhysicsEngine_runSimulation_Classic_SinglePass_MP.cpp:4-404."""
+        self.set_variant("singlepass")
+        return self._run_loop(dt_total)
+
+    def _run_loop(self, dt_total):
+        """Loop control shared by the three (DP:439-458, 790-814; DPC:15-24,
+        545-565; SP:16-24, 383-403): dt clamped to the request, console
+        report when the interval has passed, 0 while d_Time < d_TimeEnd."""
         acc = 0.0
         dt_max = self.scene.dt if self.scene is not None else None
         time_end = self.scene.time_end if self.scene is not None else float("inf")

@@ -107,11 +107,6 @@ enum {
     MPM_MON_COUNT = 4
 };
 
-/* Step variant. */
-enum {
-    MPM_VARIANT_DOUBLEPASS = 0   /* runSimulation_Classic_DoublePass_MPLocks */
-};
-
 /* reportConsole columns (PhysicsEngine.cpp:40-183) without the principal
  * monitor.  A column whose list is empty reports has_* = 0. */
 typedef struct mpm_monitors {
@@ -228,6 +223,34 @@ int mpm_get_instance_monitors(mpm_engine *e, int32_t instance, mpm_monitors *out
  *   GPU (neighbouring lanes read the same particles). */
 enum { MPM_ORDER_PARTICLE = 0, MPM_ORDER_CELL = 1 };
 int mpm_set_summation_order(mpm_engine *e, int32_t order);
+
+/* --- timestep variant ------------------------------------------------------- */
+
+/* Which of the reference's three step loops mpm_step / mpm_run execute:
+ * MPM_VARIANT_DOUBLEPASS (default)
+ *   runSimulation_Classic_DoublePass_MPLocks
+ *   (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:439-815);
+ * MPM_VARIANT_DOUBLEPASS_CONTACT
+ *   runSimulation_Classic_DoublePass_MPLocks_Contact
+ *   (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks_Contact.cpp:4-566):
+ *   momentum integrated before the boundary conditions and no friction;
+ *   displacement control on all three axes after them; control particles
+ *   skip both grid-to-particle passes and the remap and move by x += v dt;
+ *   F += L F dt; energies from the new stress.  Its mass-gradient contact
+ *   flags (:152-237) feed nothing back into the state and are not computed;
+ * MPM_VARIANT_SINGLEPASS
+ *   runSimulation_Classic_SinglePass_MP
+ *   (PhysicsEngine_runSimulation_Classic_SinglePass_MP.cpp:4-404): USL;
+ *   control particles are left out of the particle-to-grid sums; grid
+ *   velocity is the mass-weighted particle velocity plus f/m dt, damping
+ *   along it; one grid-to-particle pass; F += L F dt.  Per-particle
+ *   material parameters (:328-345) are the particle's material's; a particle
+ *   with no valid material gets the per-particle defaults (elastic, E = 0).
+ * In every variant, "displacement-controlled particle" means a particle on
+ * the mpm_set_displacement_control list. */
+enum { MPM_VARIANT_DOUBLEPASS = 0, MPM_VARIANT_DOUBLEPASS_CONTACT = 1, MPM_VARIANT_SINGLEPASS = 2 };
+int mpm_set_variant(mpm_engine *e, int32_t variant);
+int mpm_get_variant(const mpm_engine *e, int32_t *variant);
 
 /* --- node tiles and the autotuner (no counterpart in the reference) -------- */
 

@@ -93,6 +93,7 @@ struct AGP {
 /* MaterialPoint_BC, MaterialPoint_BC.h:25-85 (fields on the timestep only). */
 struct Particle {
     int material = -1;
+    bool dc = false; /* on the displacement-control list */
     uint8_t dc_axes = 0;
     float dc_mult = 1.0f;
     double m = 0.0, V = 0.0, V0 = 0.0;
@@ -358,11 +359,13 @@ struct oracle_engine {
         }
     }
 
-    /* runSimulation_M2G, DP:48-89. */
-    void m2g(int nthreads) {
+    /* runSimulation_M2G, DP:48-89 (DPC:110-150).  skip_dc leaves out the
+     * displacement-controlled particles (SinglePass). */
+    void m2g(int nthreads, bool skip_dc = false) {
 #pragma omp for
         for (long i = 0; i < (long)particles.size(); i++) {
             Particle *p = particles[i];
+            if (skip_dc && p->dc) continue;
             for (size_t k = 0; k < p->agp.size(); k++) {
                 const AGP &a = p->agp[k];
                 Node *n = nodes[a.index];
@@ -426,7 +429,7 @@ struct oracle_engine {
         return nullptr;
     }
 
-    void m2g_cells() {
+    void m2g_cells(bool skip_dc = false) {
 #pragma omp for
         for (long ni = 0; ni < (long)nodes.size(); ni++) {
             Node *n = nodes[ni];
@@ -439,6 +442,7 @@ struct oracle_engine {
                 V3 sp, sf;
                 for (uint32_t q = bin_start[c]; q < bin_start[c + 1]; q++) {
                     const Particle *p = particles[bin_ids[q]];
+                    if (skip_dc && p->dc) continue;
                     const AGP *a = agp_for(p, (unsigned)ni);
                     double N = a->N;
                     V3 g = a->g;
@@ -461,7 +465,7 @@ struct oracle_engine {
         }
     }
 
-    void remap_cells() {
+    void remap_cells(bool skip_dc = false) {
 #pragma omp for
         for (long ni = 0; ni < (long)nodes.size(); ni++) {
             Node *n = nodes[ni];
@@ -473,6 +477,7 @@ struct oracle_engine {
                 V3 sv;
                 for (uint32_t q = bin_start[c]; q < bin_start[c + 1]; q++) {
                     const Particle *p = particles[bin_ids[q]];
+                    if (skip_dc && p->dc) continue;
                     double N = agp_for(p, (unsigned)ni)->N;
                     sv.x += N * (p->m * p->v.x) / n->m;
                     sv.y += N * (p->m * p->v.y) / n->m;
@@ -551,11 +556,13 @@ struct oracle_engine {
         }
     }
 
-    /* G2P pass 1, DP:490-510. */
-    void g2p_velocity(double dt) {
+    /* G2P pass 1, DP:490-510 (DPC:304-324 and the velocity line of
+     * SP:293-294 skip the displacement-controlled particles). */
+    void g2p_velocity(double dt, bool skip_dc = false) {
 #pragma omp for
         for (long i = 0; i < (long)particles.size(); i++) {
             Particle *p = particles[i];
+            if (skip_dc && p->dc) continue;
             for (size_t k = 0; k < p->agp.size(); k++) {
                 const AGP &a = p->agp[k];
                 Node *n = nodes[a.index];
@@ -568,11 +575,13 @@ struct oracle_engine {
         }
     }
 
-    /* Velocity remap P2G, DP:539-564. */
-    void remap(int nthreads) {
+    /* Velocity remap P2G, DP:539-564 (DPC:353-378 skips the
+     * displacement-controlled particles). */
+    void remap(int nthreads, bool skip_dc = false) {
 #pragma omp for
         for (long i = 0; i < (long)particles.size(); i++) {
             Particle *p = particles[i];
+            if (skip_dc && p->dc) continue;
             for (size_t k = 0; k < p->agp.size(); k++) {
                 const AGP &a = p->agp[k];
                 Node *n = nodes[a.index];
@@ -694,8 +703,247 @@ struct oracle_engine {
         }
     }
 
+
+    /* ---- the two other step loops ------------------------------------------ */
+    int variant = MPM_VARIANT_DOUBLEPASS;
+
+    /* Grid update of the contact variant, DPC:242-277: damping along p (as
+     * DP:100-101), momentum integrated first, then the fixed components
+     * zeroed with the y reaction taken before (no friction). */
+    void integrate_grid_dpc(double dt) {
+#pragma omp for
+        for (long i = 0; i < (long)nodes.size(); i++) {
+            Node *n = nodes[i];
+            if (!n->active) continue;
+            if (std::sqrt(glm_dot(n->p, n->p)) > 1.0e-24) {
+                double inv = 1.0 / std::sqrt(glm_dot(n->p, n->p));
+                V3 nrm = v3(n->p.x * inv, n->p.y * inv, n->p.z * inv);
+                double s = damping * std::fabs(glm_dot(n->f, nrm));
+                n->f.x -= s * nrm.x;
+                n->f.y -= s * nrm.y;
+                n->f.z -= s * nrm.z;
+            }
+            n->p.x += n->f.x * dt;
+            n->p.y += n->f.y * dt;
+            n->p.z += n->f.z * dt;
+            if (n->fixed & 1) { n->v.x = 0.0; n->p.x = 0.0; n->f.x = 0.0; }
+            if (n->fixed & 2) { n->v.y = 0.0; n->p.y = 0.0; n->ft.y += n->f.y; n->f.y = 0.0; }
+            if (n->fixed & 4) { n->v.z = 0.0; n->p.z = 0.0; n->f.z = 0.0; }
+        }
+    }
+
+    /* SinglePass particle-to-grid, SP:108-204, as one thread runs it: mass
+     * first (SP:111-130), then velocity (mass-weighted, only on nodes whose
+     * mass exceeds the minimum), momentum (likewise) and force (SP:152-185).
+     * Displacement-controlled particles are left out. */
+    void p2g_singlepass(int nthreads) {
+#pragma omp for
+        for (long i = 0; i < (long)particles.size(); i++) {
+            Particle *p = particles[i];
+            if (p->dc) continue;
+            for (size_t k = 0; k < p->agp.size(); k++) {
+                const AGP &a = p->agp[k];
+                Node *n = nodes[a.index];
+                if (nthreads > 1) omp_set_lock(locks[a.index]);
+                n->m += a.N * p->m;
+                if (nthreads > 1) omp_unset_lock(locks[a.index]);
+            }
+        }
+#pragma omp barrier
+#pragma omp for
+        for (long i = 0; i < (long)particles.size(); i++) {
+            Particle *p = particles[i];
+            if (p->dc) continue;
+            for (size_t k = 0; k < p->agp.size(); k++) {
+                const AGP &a = p->agp[k];
+                Node *n = nodes[a.index];
+                double N = a.N;
+                V3 g = a.g;
+                if (nthreads > 1) omp_set_lock(locks[a.index]);
+                if (n->m > mass_min) {
+                    n->v.x += N * (p->m * p->v.x) / n->m;
+                    n->v.y += N * (p->m * p->v.y) / n->m;
+                    n->v.z += N * (p->m * p->v.z) / n->m;
+                    n->p.x += N * (p->m * p->v.x);
+                    n->p.y += N * (p->m * p->v.y);
+                    n->p.z += N * (p->m * p->v.z);
+                }
+                double V = p->V;
+                n->f.x += -V * (g.x * p->sig[0] + g.y * p->sig[3] + g.z * p->sig[5]);
+                n->f.y += -V * (g.y * p->sig[1] + g.x * p->sig[3] + g.z * p->sig[4]);
+                n->f.z += -V * (g.z * p->sig[2] + g.x * p->sig[5] + g.y * p->sig[4]);
+                n->f.x += N * p->fext.x;
+                n->f.y += N * p->fext.y;
+                n->f.z += N * p->fext.z;
+                if (nthreads > 1) omp_unset_lock(locks[a.index]);
+            }
+        }
+    }
+
+    /* Cell order of the same sums: mass, momentum and force per-cell
+     * partials in one pass, momentum dropped on nodes at or below the mass
+     * minimum, then the mass-weighted velocity as in remap_cells. */
+    void p2g_singlepass_cells() {
+        m2g_cells(true);
+#pragma omp barrier
+#pragma omp for
+        for (long ni = 0; ni < (long)nodes.size(); ni++) {
+            Node *n = nodes[ni];
+            if (n->active && !(n->m > mass_min)) n->p = v3(0.0, 0.0, 0.0);
+        }
+#pragma omp barrier
+        remap_cells(true);
+    }
+
+    /* Grid update of the single pass, SP:210-245: damping along v with the
+     * force magnitude, v += f / m dt on nodes above the mass minimum,
+     * p += f dt, fixed components zeroed with the y reaction taken before. */
+    void integrate_grid_singlepass(double dt) {
+#pragma omp for
+        for (long i = 0; i < (long)nodes.size(); i++) {
+            Node *n = nodes[i];
+            if (!n->active) continue;
+            if (std::sqrt(glm_dot(n->v, n->v)) > 1.0e-9) {
+                double lf = std::sqrt(glm_dot(n->f, n->f));
+                double inv = 1.0 / std::sqrt(glm_dot(n->v, n->v));
+                V3 nrm = v3(n->v.x * inv, n->v.y * inv, n->v.z * inv);
+                double s = damping * lf;
+                n->f.x -= s * nrm.x;
+                n->f.y -= s * nrm.y;
+                n->f.z -= s * nrm.z;
+            }
+            if (n->m > mass_min) {
+                n->v.x += n->f.x / n->m * dt;
+                n->v.y += n->f.y / n->m * dt;
+                n->v.z += n->f.z / n->m * dt;
+            }
+            n->p.x += n->f.x * dt;
+            n->p.y += n->f.y * dt;
+            n->p.z += n->f.z * dt;
+            if (n->fixed & 1) { n->v.x = 0.0; n->p.x = 0.0; n->f.x = 0.0; }
+            if (n->fixed & 2) { n->v.y = 0.0; n->p.y = 0.0; n->ft.y += n->f.y; n->f.y = 0.0; }
+            if (n->fixed & 4) { n->v.z = 0.0; n->p.z = 0.0; n->f.z = 0.0; }
+        }
+    }
+
+    /* Displacement control on all three axes, DPC:283-297 and 416-432,
+     * SP:251-268: each node of a listed particle takes its velocity, the
+     * matching momentum and zero force; later list entries win. */
+    void displacement_control_all() {
+#pragma omp single
+        for (size_t i = 0; i < dc_list.size(); i++) {
+            Particle *p = particles[dc_list[i]];
+            for (size_t k = 0; k < p->agp.size(); k++) {
+                Node *n = nodes[p->agp[k].index];
+                n->v = p->v;
+                n->p = v3(n->m * p->v.x, n->m * p->v.y, n->m * p->v.z);
+                n->f = v3(0.0, 0.0, 0.0);
+            }
+        }
+    }
+
+    /* Grid-to-particle strain pass of the contact variant (DPC:438-525) and
+     * of the single pass (SP:274-363), displacement-controlled particles
+     * skipped: x from p/m, L from the grid velocity, F += (L F) dt,
+     * dstrain = dF - I in Voigt form straight from dF, strain updated before
+     * the material check (DPC) or the per-particle material defaults used
+     * (SP), energies from the new stress. */
+    void g2p_strain_variant(double dt, int64_t *null_count, int64_t *plastic_count, int64_t *outside_count,
+                            int64_t *nonfinite_count) {
+#pragma omp for
+        for (long i = 0; i < (long)particles.size(); i++) {
+            Particle *p = particles[i];
+            if (p->agp.empty()) {
+#pragma omp atomic
+                (*outside_count)++;
+            }
+            if (p->dc) continue;
+            M3 L = m3_zero();
+            for (size_t k = 0; k < p->agp.size(); k++) {
+                const AGP &a = p->agp[k];
+                Node *n = nodes[a.index];
+                if (n->m > mass_min) {
+                    p->x.x += a.N * (n->p.x / n->m) * dt;
+                    p->x.y += a.N * (n->p.y / n->m) * dt;
+                    p->x.z += a.N * (n->p.z / n->m) * dt;
+                }
+                double vv[3] = {n->v.x, n->v.y, n->v.z};
+                double gg[3] = {a.g.x, a.g.y, a.g.z};
+                for (int c = 0; c < 3; c++)
+                    for (int r = 0; r < 3; r++) L.m[c][r] += vv[r] * gg[c];
+            }
+            /* F += (L * F) * dt: glm mat3 product, then scaled, then added */
+            M3 LF = m3_mul(L, p->F);
+            for (int c = 0; c < 3; c++)
+                for (int r = 0; r < 3; r++) p->F.m[c][r] = p->F.m[c][r] + LF.m[c][r] * dt;
+            double det = m3_det(p->F);
+            p->V = det * p->V0;
+            M3 dF;
+            M3 I = m3_identity();
+            for (int c = 0; c < 3; c++)
+                for (int r = 0; r < 3; r++) dF.m[c][r] = I.m[c][r] + L.m[c][r] * dt;
+            double de[6];
+            de[0] = dF.m[0][0] - 1.0;
+            de[1] = dF.m[1][1] - 1.0;
+            de[2] = dF.m[2][2] - 1.0;
+            de[3] = dF.m[0][1] + dF.m[1][0];
+            de[4] = dF.m[1][2] + dF.m[2][1];
+            de[5] = dF.m[2][0] + dF.m[0][2];
+            for (int k = 0; k < 6; k++) p->eps[k] += de[k];
+            const bool valid = p->material >= 0 && p->material < (int)materials.size();
+            if (!valid) {
+#pragma omp atomic
+                (*null_count)++;
+                if (variant == MPM_VARIANT_DOUBLEPASS_CONTACT) { /* DPC:491-495: message and skip */
+                    if (!finite_state(p)) {
+#pragma omp atomic
+                        (*nonfinite_count)++;
+                    }
+                    continue;
+                }
+            }
+            Material defaults; /* MaterialPoint_BC.h:48-57: elastic, E = nu = 0 */
+            defaults.type = MPM_MAT_ELASTIC;
+            const Material &mt = valid ? materials[p->material] : defaults;
+            CR cr;
+            cr_dispatch(cr, mt, p->R, p->sig, de);
+            if (cr.yielded) {
+#pragma omp atomic
+                (*plastic_count)++;
+            }
+            for (int k = 0; k < 6; k++) p->sig[k] += cr.dsig[k];
+            for (int k = 0; k < 6; k++) p->epsp[k] += cr.depsp[k];
+            for (int k = 0; k < 6; k++) p->Ws += p->sig[k] * de[k] * p->V;
+            for (int k = 0; k < 6; k++) p->Wp += p->sig[k] * cr.depsp[k] * p->V;
+            p->R += cr.dR;
+            if (!finite_state(p)) {
+#pragma omp atomic
+                (*nonfinite_count)++;
+            }
+        }
+    }
+
+    /* Displacement-control schedule of DPC:531-540 and SP:369-378: all three
+     * axes from the timeline (when it has points), then x += v dt. */
+    void dc_schedule_move(double dt) {
+#pragma omp for
+        for (long i = 0; i < (long)dc_list.size(); i++) {
+            Particle *p = particles[dc_list[i]];
+            if (timeline.t.size() != 0) {
+                V3 vt = timeline.velocity(time);
+                double mult = (double)p->dc_mult;
+                p->v = v3(mult * vt.x, mult * vt.y, mult * vt.z);
+            }
+            p->x.x += p->v.x * dt;
+            p->x.y += p->v.y * dt;
+            p->x.z += p->v.z * dt;
+        }
+    }
+
     /* One pass of the while-loop body, DP:462-803. */
     void step(double dt, int nthreads) {
+        if (variant == MPM_VARIANT_DOUBLEPASS_CONTACT) return step_contact(dt, nthreads);
+        if (variant == MPM_VARIANT_SINGLEPASS) return step_singlepass(dt, nthreads);
         int64_t nulls = 0, plastic = 0, outside = 0, nonfinite = 0;
 #pragma omp parallel num_threads(nthreads)
         {
@@ -732,6 +980,82 @@ struct oracle_engine {
 #pragma omp barrier
             dc_schedule();
         }
+        finish_step(dt, nulls, plastic, outside, nonfinite);
+    }
+
+
+    /* One pass of the contact variant's loop body, DPC:27-543. */
+    void step_contact(double dt, int nthreads) {
+        int64_t nulls = 0, plastic = 0, outside = 0, nonfinite = 0;
+#pragma omp parallel num_threads(nthreads)
+        {
+            int nt = omp_get_num_threads();
+            reset_grid();
+#pragma omp barrier
+            find_agps();
+#pragma omp barrier
+            if (order == MPM_ORDER_CELL) {
+#pragma omp single
+                build_bins();
+                m2g_cells();
+            } else {
+                m2g(nt);
+            }
+#pragma omp barrier
+            integrate_grid_dpc(dt);
+#pragma omp barrier
+            displacement_control_all();
+#pragma omp barrier
+            g2p_velocity(dt, true);
+#pragma omp barrier
+            if (order == MPM_ORDER_CELL)
+                remap_cells(true);
+            else
+                remap(nt, true);
+#pragma omp barrier
+            reapply_bc();
+#pragma omp barrier
+            displacement_control_all();
+#pragma omp barrier
+            g2p_strain_variant(dt, &nulls, &plastic, &outside, &nonfinite);
+#pragma omp barrier
+            dc_schedule_move(dt);
+        }
+        finish_step(dt, nulls, plastic, outside, nonfinite);
+    }
+
+    /* One pass of the single-pass loop body, SP:27-381. */
+    void step_singlepass(double dt, int nthreads) {
+        int64_t nulls = 0, plastic = 0, outside = 0, nonfinite = 0;
+#pragma omp parallel num_threads(nthreads)
+        {
+            int nt = omp_get_num_threads();
+            reset_grid();
+#pragma omp barrier
+            find_agps();
+#pragma omp barrier
+            if (order == MPM_ORDER_CELL) {
+#pragma omp single
+                build_bins();
+                p2g_singlepass_cells();
+            } else {
+                p2g_singlepass(nt);
+            }
+#pragma omp barrier
+            integrate_grid_singlepass(dt);
+#pragma omp barrier
+            displacement_control_all();
+#pragma omp barrier
+            g2p_velocity(dt, true);
+#pragma omp barrier
+            g2p_strain_variant(dt, &nulls, &plastic, &outside, &nonfinite);
+#pragma omp barrier
+            dc_schedule_move(dt);
+        }
+        finish_step(dt, nulls, plastic, outside, nonfinite);
+    }
+
+    void finish_step(double dt, int64_t nulls, int64_t plastic, int64_t outside, int64_t nonfinite) {
         health.steps++;
         health.null_material += nulls;
         health.plastic_points += plastic;
@@ -878,7 +1202,9 @@ int oracle_set_displacement_control(oracle_engine *e, const int64_t *idx, const 
     for (int64_t i = 0; i < count; i++)
         if (idx[i] < 0 || idx[i] >= (int64_t)e->particles.size()) return MPM_ERR_ARG;
     e->dc_list.assign(idx, idx + count);
+    for (Particle *p : e->particles) p->dc = false;
     for (int64_t i = 0; i < count; i++) {
+        e->particles[idx[i]]->dc = true;
         e->particles[idx[i]]->dc_axes = axes[i];
         e->particles[idx[i]]->dc_mult = mult[i];
     }
@@ -919,6 +1245,12 @@ int oracle_set_damping(oracle_engine *e, double damping, double mass_min) {
 int oracle_set_order(oracle_engine *e, int32_t order) {
     if (!e || (order != MPM_ORDER_PARTICLE && order != MPM_ORDER_CELL)) return MPM_ERR_ARG;
     e->order = order;
+    return MPM_OK;
+}
+
+int oracle_set_variant(oracle_engine *e, int32_t variant) {
+    if (!e || variant < MPM_VARIANT_DOUBLEPASS || variant > MPM_VARIANT_SINGLEPASS) return MPM_ERR_ARG;
+    e->variant = variant;
     return MPM_OK;
 }
 

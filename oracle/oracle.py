@@ -50,6 +50,7 @@ def load():
         "oracle_set_damping": [P, DBL, DBL],
         "oracle_step": [P, DBL, I32, I32],
         "oracle_set_order": [P, I32],
+        "oracle_set_variant": [P, I32],
         "oracle_run": [P, DBL, I32, ctypes.POINTER(I32)],
         "oracle_get_time": [P, ctypes.POINTER(DBL), ctypes.POINTER(I64)],
         "oracle_get_node_count": [P, ctypes.POINTER(I64)],
@@ -69,6 +70,9 @@ def load():
     return lib
 
 
+VARIANT = {"doublepass": 0, "doublepass_contact": 1, "singlepass": 2}
+
+
 def _p(a):
     if a is None:
         return None
@@ -85,12 +89,13 @@ class OracleEngine:
     """The reference's DoublePass step on the CPU (single thread = the
     reference's summation order; more threads = its OpenMP lock scatter)."""
 
-    def __init__(self, threads=1, order="particle"):
+    def __init__(self, threads=1, order="particle", variant="doublepass"):
         self._lib = load()
         h = ctypes.c_void_p()
         _ok(self._lib.oracle_create(ctypes.byref(h)), "create")
         self._h = h
         _ok(self._lib.oracle_set_order(self._h, {"particle": 0, "cell": 1}[order]), "order")
+        self.set_variant(variant)
         self.threads = int(threads)
         self.n = 0
         self.nnodes = 0
@@ -160,6 +165,11 @@ class OracleEngine:
     def load_scene(self, scene):
         scene.apply(self)
         self.scene = scene
+
+    def set_variant(self, variant):
+        """doublepass | doublepass_contact | singlepass: the reference's three step loops."""
+        _ok(self._lib.oracle_set_variant(self._h, VARIANT[variant]), "variant")
+        self.variant = variant
 
     def step(self, dt, nsteps=1):
         _ok(self._lib.oracle_step(self._h, float(dt), int(nsteps), self.threads), "step")

@@ -17,16 +17,17 @@ def _scenes():
     return [a, b, c]
 
 
-def test_batch_instances_match_separate_runs_and_oracle():
+@pytest.mark.parametrize("variant", ["doublepass", "doublepass_contact", "singlepass"])
+def test_batch_instances_match_separate_runs_and_oracle(variant):
     from code_mpm_amd import Engine
     from oracle.oracle import OracleEngine
     sc = _scenes()
-    batch = Engine(order="particle")
+    batch = Engine(order="particle", variant=variant)
     offs = batch.load_batch(sc)
     steps = 12
     batch.step(0.0, steps)  # each instance by its own dt_max
     for k, s in enumerate(sc):
-        alone = OracleEngine(threads=1)
+        alone = OracleEngine(threads=1, variant=variant)
         alone.load_scene(s)
         alone.step(s.dt, steps)
         sl = slice(int(offs[k]), int(offs[k]) + s.n)
@@ -40,10 +41,11 @@ def test_batch_instances_match_separate_runs_and_oracle():
         assert batch.instance_time(k) == alone.getTime_Current()
 
 
-def test_graph_replay_matches_eager():
+@pytest.mark.parametrize("variant", ["doublepass", "singlepass"])
+def test_graph_replay_matches_eager(variant):
     from code_mpm_amd import Engine, scenes
     sc = scenes.honeycomb_crush(cells=(48, 48, 8), steps=100)
-    eager, graph = Engine(), Engine()
+    eager, graph = Engine(variant=variant), Engine(variant=variant)
     eager.load_scene(sc)
     graph.load_scene(sc)
     for rep in range(4):
@@ -70,11 +72,13 @@ def test_batch_graph_replay_matches_eager():
         graph.health()
 
 
-@pytest.mark.parametrize("order", ["particle", "cell"])
-def test_autotuned_step_equals_ordinary_step(order):
+@pytest.mark.parametrize("order,variant", [("particle", "doublepass"), ("cell", "doublepass"),
+                                           ("cell", "doublepass_contact"), ("cell", "singlepass"),
+                                           ("particle", "singlepass")])
+def test_autotuned_step_equals_ordinary_step(order, variant):
     from code_mpm_amd import Engine, scenes
     sc = scenes.honeycomb_crush(cells=(48, 48, 8), steps=100)
-    a, b = Engine(order=order), Engine(order=order)
+    a, b = Engine(order=order, variant=variant), Engine(order=order, variant=variant)
     a.load_scene(sc)
     b.load_scene(sc)
     a.step(sc.dt, 3)

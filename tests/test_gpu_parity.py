@@ -18,12 +18,12 @@ NODE_FIELDS = ["MASS", "MOMENTUM", "VELOCITY", "FORCE", "FORCE_TEMP", "ACTIVE"]
 ORDERS = ["particle", "cell"]
 
 
-def _pair(scene, order="particle"):
+def _pair(scene, order="particle", variant="doublepass"):
     from code_mpm_amd import Engine
     from oracle.oracle import OracleEngine
-    g = Engine(device=0, order=order)
+    g = Engine(device=0, order=order, variant=variant)
     g.load_scene(scene)
-    c = OracleEngine(threads=1, order=order)
+    c = OracleEngine(threads=1, order=order, variant=variant)
     c.load_scene(scene)
     return g, c
 
@@ -211,3 +211,65 @@ def test_cell_order_tracks_reference_order():
             for key in ("position_y", "momentum_y", "kinetic_energy", "strain_energy"):
                 scale = max(abs(mc[key]), p_scale) if key == "momentum_y" else abs(mc[key])
                 assert abs(mg[key] - mc[key]) <= 1e-9 * scale + 1e-300, (k, key)
+
+
+# ---- the contact and single-pass step loops ----------------------------------
+
+VARIANTS = ["doublepass_contact", "singlepass"]
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("order", ORDERS)
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_variant_random_block_per_step(seed, order, variant):
+    from code_mpm_amd import scenes
+    scene = scenes.random_block(seed=seed)
+    g, c = _pair(scene, order, variant)
+    for step in range(1, 13):
+        g.step(scene.dt, 1)
+        c.step(scene.dt, 1)
+        _assert_same(g, c, f"{variant} seed {seed} step {step}")
+    _assert_monitors(g, c, f"{variant} seed {seed}")
+    hg, hc = g.health(), c.health()
+    for k in ("steps", "outside_grid", "null_material", "plastic_points", "nonfinite_particles"):
+        assert hg[k] == hc[k], f"health {k}: {hg[k]} vs {hc[k]}"
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("order", ORDERS)
+def test_variant_dense_many_steps(order, variant):
+    from code_mpm_amd import scenes
+    scene = scenes.random_block(seed=11, cells=(4, 4, 3), n=2500)
+    g, c = _pair(scene, order, variant)
+    g.step(scene.dt, 60)
+    c.step(scene.dt, 60)
+    _assert_same(g, c, f"{variant} dense 60 steps")
+    _assert_monitors(g, c, f"{variant} dense")
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("order", ORDERS)
+def test_variant_honeycomb_crush(order, variant):
+    from code_mpm_amd import scenes
+    scene = scenes.honeycomb_crush(cells=(64, 64, 8), steps=200)
+    g, c = _pair(scene, order, variant)
+    for k in range(2):
+        g.step(scene.dt, 100)
+        c.step(scene.dt, 100)
+        _assert_monitors(g, c, f"{variant} honeycomb after {100 * (k + 1)} steps")
+    _assert_same(g, c, f"{variant} honeycomb 200 steps")
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_variant_run_loop(variant):
+    """runSimulation_Classic_* loop control and console report."""
+    from code_mpm_amd import scenes
+    scene = scenes.random_block(seed=5)
+    g, c = _pair(scene, "particle", variant)
+    total = 7.3 * scene.dt
+    method = {"doublepass_contact": "runSimulation_Classic_DoublePass_MPLocks_Contact",
+              "singlepass": "runSimulation_Classic_SinglePass_MP"}[variant]
+    assert getattr(g, method)(total) == c.run(total)
+    assert g.getTime_Current() == c.getTime_Current()
+    assert g.variant == variant
+    _assert_same(g, c, f"{variant} run")

@@ -135,7 +135,10 @@ def test_trained_surrogate_in_step():
     exact.load_scene(sc)
     exact.step(sc.dt, steps)
     a = exact.get_particle("STRESS")
-    for tol, bound in ((1e-4, 1e-3), (5e-2, 0.5)):
+    # accepted points sit on the yield surface within tol * sigma_y; what the
+    # check cannot see is the direction error of the predicted deviatoric
+    # correction, which accumulates over the accepted steps
+    for tol, bound in ((1e-4, 5e-3), (5e-2, 0.5)):
         sur = Engine()
         sur.load_scene(sc)
         sur.set_surrogate(s, material=0, residual_tolerance=tol)
