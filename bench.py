@@ -39,10 +39,10 @@ PER_NODE = {"g2p_strain": 7 * 8, "p2g_grid": 20 * 8, "g2p_velocity": 4 * 8, "rem
 KERNELS = {
     "cell": {"bin": ["k_scan_reduce", "k_scan_blocks", "k_scan_apply", "k_scatter", "k_cell_sort"],
              "p2g_grid": ["k_cell_partials<false>", "k_node_cells<false>"], "g2p_velocity": ["k_g2p_velocity"],
-             "remap_grid": ["k_cell_partials<true>", "k_node_cells<true>"], "g2p_strain": ["k_g2p_strain<0>"]},
+             "remap_grid": ["k_cell_partials<true>", "k_node_cells<true>"], "g2p_strain": ["k_g2p_strain<0, 0>"]},
     "particle": {"bin": ["k_scan_reduce", "k_scan_blocks", "k_scan_apply", "k_scatter", "k_cell_sort"],
                  "p2g_grid": ["k_p2g_grid"], "g2p_velocity": ["k_g2p_velocity"], "remap_grid": ["k_remap_grid"],
-                 "g2p_strain": ["k_g2p_strain<0>"]},
+                 "g2p_strain": ["k_g2p_strain<0, 0>"]},
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_honeycomb8m.txt")
 

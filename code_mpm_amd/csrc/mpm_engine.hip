@@ -172,7 +172,7 @@ struct mpm_engine {
         free_parts();
     }
     void free_parts() {
-        dfree(parts.m); dfree(parts.p); dfree(parts.f); dfree(parts.dc); dfree(bocc);
+        dfree(parts.w); dfree(bocc);
         parts = CellParts{};
         parts_cap = 0;
         occ_cap = 0;
@@ -193,16 +193,13 @@ struct mpm_engine {
     }
     int ensure_parts() {
         int64_t cap = occ_cap;
-        if (cap == parts_cap && parts.m) return MPM_OK;
-        dfree(parts.m); dfree(parts.p); dfree(parts.f); dfree(parts.dc);
+        if (cap == parts_cap && parts.w) return MPM_OK;
+        dfree(parts.w);
         parts = CellParts{};
         parts_cap = 0;
         const size_t c8 = 8 * (size_t)cap;
         int rc;
-        if ((rc = hip(hipMalloc(&parts.m, 8 * c8), "alloc partials")) ||
-            (rc = hip(hipMalloc(&parts.p, 24 * c8), "alloc partials")) ||
-            (rc = hip(hipMalloc(&parts.f, 24 * c8), "alloc partials")) ||
-            (rc = hip(hipMalloc(&parts.dc, 12 * c8), "alloc partials"))) {
+        if ((rc = hip(hipMalloc(&parts.w, 64 * c8), "alloc partials"))) {
             free_parts();
             return rc;
         }
@@ -278,7 +275,7 @@ struct mpm_engine {
     }
     NView nview() const {
         NView N;
-        N.m = nm; N.p = np; N.v = nv; N.f = nf; N.a = na; N.u = nu; N.fty = nfty;
+        N.m = nm; N.p = np; N.v = nv; N.f = nf; N.g1 = na; N.g2 = nu; N.fty = nfty;
         N.active = nact; N.fixed = nfix; N.friction = nfric;
         return N;
     }
@@ -302,6 +299,7 @@ struct mpm_engine {
         b.ty = tile[1];
         b.tz = tile[2];
         b.variant = variant;
+        b.fext = has_fext ? 1 : 0;
         return b;
     }
 
@@ -352,8 +350,8 @@ struct mpm_engine {
             (rc = hip(hipMalloc(&np, 24 * nn), "alloc node momentum")) ||
             (rc = hip(hipMalloc(&nv, 24 * nn), "alloc node velocity")) ||
             (rc = hip(hipMalloc(&nf, 24 * nn), "alloc node force")) ||
-            (rc = hip(hipMalloc(&na, 24 * nn), "alloc node f/m")) ||
-            (rc = hip(hipMalloc(&nu, 24 * nn), "alloc node p/m")) ||
+            (rc = hip(hipMalloc(&na, 32 * nn), "alloc node G2P records")) ||
+            (rc = hip(hipMalloc(&nu, 64 * nn), "alloc node G2P records")) ||
             (rc = hip(hipMalloc(&nfty, 8 * nn), "alloc node reaction")) ||
             (rc = hip(hipMalloc(&nfric, 8 * nn), "alloc node friction")) ||
             (rc = hip(hipMalloc(&nact, nn), "alloc node active")) ||

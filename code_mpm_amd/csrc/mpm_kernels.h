@@ -57,8 +57,8 @@ struct PView {
 /* Node state; 3-vectors component-major over nnodes. */
 struct NView {
     double *m, *p, *v, *f;
-    double *a;     /* f / m after the grid update (G2P pass 1) */
-    double *u;     /* p / m after the remap (G2P pass 2)       */
+    double *g1;    /* per node {m, f/m}: what G2P pass 1 reads, 32 B  */
+    double *g2;    /* per node {m, p/m, v, 0}: G2P pass 2, 64 B       */
     double *fty;   /* support reaction, Force_Temp.y           */
     uint8_t *active, *fixed;
     double *friction;
@@ -86,11 +86,17 @@ enum { VAR_DP = 0, VAR_DPC = 1, VAR_SP = 2 };
 /* Per (occupied cell, corner) partial sums of the cell order, component
  * stride cap8 = 8 x occupied-cell capacity: P2G m, p3, f3; the remap reuses
  * f for velocity; dc = id of the latest controlled particle per axis. */
+/* Cell-order partial sums: one record per (occupied cell, corner), item
+ * t = 8 * occupied rank + corner; 8 doubles (m, p, f, pad) in the P2G pass,
+ * 4 (v, pad) in the remap pass. */
 struct CellParts {
-    double *m, *p, *f;
-    uint32_t *dc;
+    double *w;
     int64_t cap8;
 };
+
+/* slot[] of a cell: its rank among occupied cells, MPM_NONE when empty,
+ * with SLOT_DC set when it holds a displacement-controlled particle. */
+#define SLOT_DC 0x80000000u
 
 /* One instance of a batch (a single scene is a batch of one): geometry,
  * where its nodes / cells / tiles / particles start in the shared arrays,
@@ -117,6 +123,7 @@ struct Batch {
     int64_t nnodes;           /* all instances: component stride of node arrays */
     int tx, ty, tz;           /* node tile shape, tx * ty * tz == NODE_BLOCK */
     int variant;              /* VAR_* */
+    int fext;                 /* any particle carries an external force */
 };
 
 struct MonLists {

@@ -186,11 +186,9 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(const uint32_t *count, int
     for (int j = 0; j < SCAN_ITEMS; j++) {
         if (base + j < n) {
             start[base + j] = (uint32_t)ex;
-            if (v[j]) {
-                uint32_t r = (uint32_t)(ex >> 32);
-                slot[base + j] = r;
-                occ[r] = (uint32_t)(base + j);
-            }
+            uint32_t r = (uint32_t)(ex >> 32);
+            slot[base + j] = v[j] ? r : MPM_NONE;
+            if (v[j]) occ[r] = (uint32_t)(base + j);
         }
         ex += pack_count(v[j]);
     }
@@ -207,6 +205,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PView P, BView B) {
     if (k == MPM_NONE) return;
     uint32_t slot = B.start[k] + atomicSub(&B.count[k], 1u) - 1u;
     B.sorted[slot] = i;
+    if (P.dc_rank[i] >= 0) atomicOr(&B.slot[k], SLOT_DC); /* the cell holds a controlled particle */
 }
 
 /* Ascending ids inside each cell: the order the node gathers merge in. */
@@ -297,6 +296,22 @@ __device__ __forceinline__ void dc_consider(DCWin &w, const PView &P, uint32_t i
             }
 }
 
+/* Packed node records read by the grid-to-particle passes (one 32 / 64
+ * byte access per adjacent node instead of seven scattered ones). */
+__device__ __forceinline__ void store_g1(const NView &Nd, int64_t node, double m, const double f[3]) {
+    double2 *o = reinterpret_cast<double2 *>(Nd.g1 + (size_t)node * 4u);
+    o[0] = make_double2(m, f[0] / m); /* hoisted f/m of DP:508 */
+    o[1] = make_double2(f[1] / m, f[2] / m);
+}
+__device__ __forceinline__ void store_g2(const NView &Nd, int64_t node, double m, const double p[3],
+                                         const double v[3]) {
+    double2 *o = reinterpret_cast<double2 *>(Nd.g2 + (size_t)node * 8u);
+    o[0] = make_double2(m, p[0] / m); /* hoisted p/m of DP:198 */
+    o[1] = make_double2(p[1] / m, p[2] / m);
+    o[2] = make_double2(v[0], v[1]);
+    o[3] = make_double2(v[2], 0.0);
+}
+
 /* DC override then runSimulation_IntegrateGrid (DP:90-142) on one node's
  * sums, in registers; stores m, p, f, f/m and the reaction. */
 __device__ __forceinline__ void node_integrate(const Inst &I, const NView &Nd, int64_t node, size_t nn, double m,
@@ -356,8 +371,8 @@ __device__ __forceinline__ void node_integrate(const Inst &I, const NView &Nd, i
     for (int a = 0; a < 3; a++) {
         Nd.p[a * nn + node] = p[a];
         Nd.f[a * nn + node] = f[a];
-        Nd.a[a * nn + node] = f[a] / m; /* hoisted f/m of DP:508 */
     }
+    store_g1(Nd, node, m, f);
     Nd.fty[node] = fty;
 }
 
@@ -408,8 +423,8 @@ __device__ __forceinline__ void node_integrate_contact(const Inst &I, const NVie
     for (int a = 0; a < 3; a++) {
         Nd.p[a * nn + node] = p[a];
         Nd.f[a * nn + node] = f[a];
-        Nd.a[a * nn + node] = f[a] / m;
     }
+    store_g1(Nd, node, m, f);
     Nd.fty[node] = fty;
 }
 
@@ -485,9 +500,9 @@ __device__ __forceinline__ void node_update_singlepass(const Inst &I, const NVie
         Nd.v[a * nn + node] = v[a];
         Nd.p[a * nn + node] = p[a];
         Nd.f[a * nn + node] = f[a];
-        Nd.a[a * nn + node] = f[a] / m;
-        Nd.u[a * nn + node] = p[a] / m;
     }
+    store_g1(Nd, node, m, f);
+    store_g2(Nd, node, m, p, v);
     Nd.fty[node] = fty;
 }
 
@@ -595,11 +610,14 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_velocity(Batch bt, PView P, NView
         if (!(ok[0][ox] && ok[1][oy] && ok[2][oz])) continue;
         size_t node = (size_t)I.node_off + (size_t)(b[0] + ox) +
                       (size_t)g.nn[0] * ((size_t)(b[1] + oy) + (size_t)g.nn[1] * (b[2] + oz));
-        if (Nd.m[node] > I.mass_min) {
+        const double2 *r = reinterpret_cast<const double2 *>(Nd.g1 + node * 4u);
+        const double2 r0 = r[0];
+        if (r0.x > I.mass_min) {
+            const double2 r1 = r[1];
             double N = s[0][ox] * s[1][oy] * s[2][oz];
-            v[0] += N * Nd.a[node] * dt;
-            v[1] += N * Nd.a[nn + node] * dt;
-            v[2] += N * Nd.a[2 * nn + node] * dt;
+            v[0] += N * r0.y * dt;
+            v[1] += N * r1.x * dt;
+            v[2] += N * r1.y * dt;
         }
     }
     P.v[i] = v[0];
@@ -626,8 +644,8 @@ __device__ __forceinline__ void node_remap_finish(const NView &Nd, int64_t node,
         }
         Nd.v[a * nn + node] = v[a];
         Nd.p[a * nn + node] = p[a];
-        Nd.u[a * nn + node] = p[a] / m;
     }
+    store_g2(Nd, node, m, p, v);
 }
 
 /* Contact loop: BC re-applied (DPC:380-411), then displacement control on
@@ -649,8 +667,8 @@ __device__ __forceinline__ void node_remap_finish_contact(const NView &Nd, int64
         }
         Nd.v[a * nn + node] = v[a];
         Nd.p[a * nn + node] = p[a];
-        Nd.u[a * nn + node] = p[a] / m;
     }
+    store_g2(Nd, node, m, p, v);
 }
 
 /* The node side of the remap phase for each loop (the single pass runs it
@@ -719,103 +737,156 @@ __device__ __forceinline__ void node_remap_dispatch(const Batch &bt, const Inst 
  * (k >> 2, k >> 1, k) & 1, node = c + corner; the node sees that cell as its
  * (7 - k)-th neighbour. */
 
+/* Each block takes groups of PART_CELLS consecutive occupied cells (8
+ * lanes per cell, one per corner).  The group's particles are contiguous in
+ * the sorted list; they are staged through LDS in chunks of PART_STAGE with
+ * one coalesced pass, and every lane then sums its cell's particles for its
+ * corner from LDS in ascending id, the same order and arithmetic as reading
+ * them from memory. */
+#define PART_CELLS (BLOCK / 8)
+#define PART_STAGE 256
+
 template <bool REMAP>
 __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NView Nd, BView B, CellParts C) {
+    /* staged particle inputs: x, m, v, then (P2G) V, sigma, f_ext */
+    __shared__ double sx[3][PART_STAGE], sm[PART_STAGE], sv[3][PART_STAGE];
+    __shared__ double sV[REMAP ? 1 : PART_STAGE], ss[REMAP ? 1 : 6][REMAP ? 1 : PART_STAGE];
+    __shared__ double sf[REMAP ? 1 : 3][REMAP ? 1 : PART_STAGE];
+    __shared__ int sdc[PART_STAGE];
     const uint32_t nocc = *B.nocc;
-    const size_t n = P.n, cap = (size_t)C.cap8;
-    for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < (uint64_t)nocc * 8u;
-         t += (uint64_t)gridDim.x * BLOCK) {
-        const uint32_t slot = (uint32_t)(t >> 3);
-        const int corner = (int)(t & 7u);
-        const uint32_t c = B.occ[slot];
-        const uint32_t s0 = B.start[c], s1 = B.start[c + 1];
-        const Inst &I = bt.inst[inst_of_particle(bt, B.sorted[s0])];
-        const GridGeom &g = I.g;
-        const uint32_t e = c - (uint32_t)I.cell_off;
-        const int ex = (int)(e % (uint32_t)g.ec[0]);
-        const int ey = (int)((e / (uint32_t)g.ec[0]) % (uint32_t)g.ec[1]);
-        const int ez = (int)(e / ((uint32_t)g.ec[0] * (uint32_t)g.ec[1]));
-        const int ix = ex - 1 + (corner >> 2), iy = ey - 1 + ((corner >> 1) & 1), iz = ez - 1 + (corner & 1);
-        if (ix < 0 || iy < 0 || iz < 0 || ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) continue;
-        const double xn[3] = {node_coord(ix, g.L[0], g.cells[0]), node_coord(iy, g.L[1], g.cells[1]),
-                              node_coord(iz, g.L[2], g.cells[2])};
-        int brank[3] = {-1, -1, -1};
-        uint32_t bid[3] = {MPM_NONE, MPM_NONE, MPM_NONE};
-        const size_t j = (size_t)t;
-        const bool all_axes = bt.variant != VAR_DP;
-        if (!REMAP) {
-            double m = 0.0, p[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0};
-            const bool skip_dc = bt.variant == VAR_SP;
-            for (uint32_t q = s0; q < s1; q++) {
-                const uint32_t id = B.sorted[q];
-                const int r = P.dc_rank[id];
-                if (r >= 0) {
-                    uint8_t axm = all_axes ? (uint8_t)7 : P.dc_axes[id];
-                    for (int a = 0; a < 3; a++)
-                        if (((axm >> a) & 1) && r > brank[a]) {
-                            brank[a] = r;
-                            bid[a] = id;
-                        }
-                    if (skip_dc) continue; /* SP:116-117, 157-158 */
-                }
-                AxisShape ax = axis_shape(P.x[id], xn[0], g.h[0]);
-                AxisShape ay = axis_shape(P.x[n + id], xn[1], g.h[1]);
-                AxisShape az = axis_shape(P.x[2 * n + id], xn[2], g.h[2]);
-                double N, gx, gy, gz;
-                shape_full(ax, ay, az, g.h, N, gx, gy, gz);
-                double mp = P.m[id];
-                m += N * mp;
-                p[0] += N * (mp * P.v[id]);
-                p[1] += N * (mp * P.v[n + id]);
-                p[2] += N * (mp * P.v[2 * n + id]);
-                double V = P.V[id];
-                const double *sg = P.sig;
-                double s0_ = sg[id], s1_ = sg[n + id], s2_ = sg[2 * n + id], s3_ = sg[3 * n + id],
-                       s4_ = sg[4 * n + id], s5_ = sg[5 * n + id];
-                f[0] += -V * (gx * s0_ + gy * s3_ + gz * s5_);
-                f[1] += -V * (gy * s1_ + gx * s3_ + gz * s4_);
-                f[2] += -V * (gz * s2_ + gx * s5_ + gy * s4_);
-                if (I.has_fext) {
-                    f[0] += N * P.fext[id];
-                    f[1] += N * P.fext[n + id];
-                    f[2] += N * P.fext[2 * n + id];
-                }
-            }
-            C.m[j] = m;
-            for (int a = 0; a < 3; a++) {
-                C.p[a * cap + j] = p[a];
-                C.f[a * cap + j] = f[a];
-            }
-        } else {
-            const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
-            const double mn = Nd.m[node];
-            const bool massive = mn > I.mass_min;
-            double v[3] = {0.0, 0.0, 0.0};
-            const bool skip_dc = bt.variant != VAR_DP;
-            for (uint32_t q = s0; q < s1; q++) {
-                const uint32_t id = B.sorted[q];
-                const int r = P.dc_rank[id];
-                if (massive && !(skip_dc && r >= 0)) { /* DPC:358-359, SP:157-158 */
-                    double N = axis_shape(P.x[id], xn[0], g.h[0]).s * axis_shape(P.x[n + id], xn[1], g.h[1]).s *
-                               axis_shape(P.x[2 * n + id], xn[2], g.h[2]).s;
-                    double mp = P.m[id];
-                    v[0] += N * (mp * P.v[id]) / mn;
-                    v[1] += N * (mp * P.v[n + id]) / mn;
-                    v[2] += N * (mp * P.v[2 * n + id]) / mn;
-                }
-                if (r >= 0) {
-                    uint8_t axm = all_axes ? (uint8_t)7 : P.dc_axes[id];
-                    for (int a = 0; a < 3; a++)
-                        if (((axm >> a) & 1) && r > brank[a]) {
-                            brank[a] = r;
-                            bid[a] = id;
-                        }
-                }
-            }
-            for (int a = 0; a < 3; a++) C.f[a * cap + j] = v[a];
+    const size_t n = P.n;
+    const int lc = threadIdx.x >> 3, corner = threadIdx.x & 7;
+    const bool skip_dc = REMAP ? bt.variant != VAR_DP : bt.variant == VAR_SP;
+    const bool fext = !REMAP && bt.fext;
+    for (uint32_t g0 = blockIdx.x * PART_CELLS; g0 < nocc; g0 += gridDim.x * PART_CELLS) {
+        const uint32_t g1 = min(g0 + (uint32_t)PART_CELLS, nocc);
+        const uint32_t r0 = B.start[B.occ[g0]], r1 = B.start[B.occ[g1 - 1] + 1];
+        /* this lane's (cell, corner) */
+        const uint32_t slot = g0 + (uint32_t)lc;
+        bool have = slot < g1;
+        uint32_t s0 = 0, s1 = 0;
+        int ix = 0, iy = 0, iz = 0;
+        const Inst *Ip = nullptr;
+        if (have) {
+            const uint32_t c = B.occ[slot];
+            s0 = B.start[c];
+            s1 = B.start[c + 1];
+            Ip = &bt.inst[inst_of_particle(bt, B.sorted[s0])];
+            const GridGeom &g = Ip->g;
+            const uint32_t e = c - (uint32_t)Ip->cell_off;
+            const int ex = (int)(e % (uint32_t)g.ec[0]);
+            const int ey = (int)((e / (uint32_t)g.ec[0]) % (uint32_t)g.ec[1]);
+            const int ez = (int)(e / ((uint32_t)g.ec[0] * (uint32_t)g.ec[1]));
+            ix = ex - 1 + (corner >> 2);
+            iy = ey - 1 + ((corner >> 1) & 1);
+            iz = ez - 1 + (corner & 1);
+            if (ix < 0 || iy < 0 || iz < 0 || ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) have = false;
         }
-        for (int a = 0; a < 3; a++) C.dc[a * cap + j] = bid[a];
+        double xn[3] = {0.0, 0.0, 0.0}, hh[3] = {1.0, 1.0, 1.0};
+        double mn = 0.0;
+        bool massive = false;
+        if (have) {
+            const GridGeom &g = Ip->g;
+            xn[0] = node_coord(ix, g.L[0], g.cells[0]);
+            xn[1] = node_coord(iy, g.L[1], g.cells[1]);
+            xn[2] = node_coord(iz, g.L[2], g.cells[2]);
+            hh[0] = g.h[0];
+            hh[1] = g.h[1];
+            hh[2] = g.h[2];
+            if (REMAP) {
+                const int64_t node =
+                    Ip->node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
+                mn = Nd.m[node];
+                massive = mn > Ip->mass_min;
+            }
+        }
+        double m = 0.0, p[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0};
+        for (uint32_t base = r0; base < r1; base += PART_STAGE) {
+            const uint32_t top = min(base + (uint32_t)PART_STAGE, r1);
+            __syncthreads();
+            for (uint32_t q = base + threadIdx.x; q < top; q += BLOCK) {
+                const uint32_t id = B.sorted[q], j = q - base;
+                sx[0][j] = P.x[id];
+                sx[1][j] = P.x[n + id];
+                sx[2][j] = P.x[2 * n + id];
+                sm[j] = P.m[id];
+                sv[0][j] = P.v[id];
+                sv[1][j] = P.v[n + id];
+                sv[2][j] = P.v[2 * n + id];
+                sdc[j] = skip_dc ? P.dc_rank[id] : -1;
+                if (!REMAP) {
+                    sV[j] = P.V[id];
+#pragma unroll
+                    for (int k = 0; k < 6; k++) ss[k][j] = P.sig[k * n + id];
+                    if (fext) {
+                        sf[0][j] = P.fext[id];
+                        sf[1][j] = P.fext[n + id];
+                        sf[2][j] = P.fext[2 * n + id];
+                    }
+                }
+            }
+            __syncthreads();
+            if (!have) continue;
+            const uint32_t a = max(s0, base), b = min(s1, top);
+            if (!REMAP) {
+                for (uint32_t q = a; q < b; q++) {
+                    const uint32_t j = q - base;
+                    if (sdc[j] >= 0) continue; /* SP:116-117, 157-158 */
+                    AxisShape ax = axis_shape(sx[0][j], xn[0], hh[0]);
+                    AxisShape ay = axis_shape(sx[1][j], xn[1], hh[1]);
+                    AxisShape az = axis_shape(sx[2][j], xn[2], hh[2]);
+                    double N, gx, gy, gz;
+                    shape_full(ax, ay, az, hh, N, gx, gy, gz);
+                    const double mp = sm[j];
+                    m += N * mp;
+                    p[0] += N * (mp * sv[0][j]);
+                    p[1] += N * (mp * sv[1][j]);
+                    p[2] += N * (mp * sv[2][j]);
+                    const double V = sV[j];
+                    f[0] += -V * (gx * ss[0][j] + gy * ss[3][j] + gz * ss[5][j]);
+                    f[1] += -V * (gy * ss[1][j] + gx * ss[3][j] + gz * ss[4][j]);
+                    f[2] += -V * (gz * ss[2][j] + gx * ss[5][j] + gy * ss[4][j]);
+                    if (fext) {
+                        f[0] += N * sf[0][j];
+                        f[1] += N * sf[1][j];
+                        f[2] += N * sf[2][j];
+                    }
+                }
+            } else if (massive) {
+                for (uint32_t q = a; q < b; q++) {
+                    const uint32_t j = q - base;
+                    if (sdc[j] >= 0) continue; /* DPC:358-359, SP:157-158 */
+                    double N = axis_shape(sx[0][j], xn[0], hh[0]).s * axis_shape(sx[1][j], xn[1], hh[1]).s *
+                               axis_shape(sx[2][j], xn[2], hh[2]).s;
+                    const double mp = sm[j];
+                    f[0] += N * (mp * sv[0][j]) / mn;
+                    f[1] += N * (mp * sv[1][j]) / mn;
+                    f[2] += N * (mp * sv[2][j]) / mn;
+                }
+            }
+        }
+        if (!have) continue;
+        const size_t t = (size_t)slot * 8u + (size_t)corner;
+        if (!REMAP) {
+            /* one 64-byte record per (cell, corner): m, p, f */
+            double2 *o = reinterpret_cast<double2 *>(C.w + t * 8u);
+            o[0] = make_double2(m, p[0]);
+            o[1] = make_double2(p[1], p[2]);
+            o[2] = make_double2(f[0], f[1]);
+            o[3] = make_double2(f[2], 0.0);
+        } else {
+            /* one 32-byte record per (cell, corner): v */
+            double2 *o = reinterpret_cast<double2 *>(C.w + t * 4u);
+            o[0] = make_double2(f[0], f[1]);
+            o[1] = make_double2(f[2], 0.0);
+        }
     }
+}
+
+/* Displacement-control winner among the controlled particles of one cell
+ * (cells holding one are flagged in their slot by the binning). */
+__device__ __forceinline__ void dc_scan_cell(DCWin &w, const PView &P, const BView &B, uint32_t c, bool all_axes) {
+    for (uint32_t q = B.start[c]; q < B.start[c + 1]; q++) dc_consider(w, P, B.sorted[q], all_axes);
 }
 
 /* Node side of the cell order: adds the 8 partials in neighbour order
@@ -830,34 +901,46 @@ __global__ __launch_bounds__(BLOCK) void k_node_cells(Batch bt, PView P, NView N
     node_of_tile(bt, g, tile - I.tile_off, ix, iy, iz);
     if (ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) return;
     const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
-    const size_t nn = (size_t)bt.nnodes, cap = (size_t)C.cap8, n = P.n;
     if (REMAP && !Nd.active[node]) return;
     double m = 0.0, p[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0};
     DCWin w;
     w.rank[0] = w.rank[1] = w.rank[2] = -1;
     w.v[0] = w.v[1] = w.v[2] = 0.0;
+    const bool all_axes = bt.variant != VAR_DP;
     bool any = false;
+    uint32_t sl[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const uint32_t c = (uint32_t)I.cell_off + (uint32_t)(ix + (k >> 2)) +
                            (uint32_t)g.ec[0] * ((uint32_t)(iy + ((k >> 1) & 1)) + (uint32_t)g.ec[1] * (uint32_t)(iz + (k & 1)));
-        if (B.start[c] == B.start[c + 1]) continue;
+        sl[k] = B.slot[c];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        if (sl[k] == MPM_NONE) continue;
         any = true;
-        const size_t j = (size_t)B.slot[c] * 8u + (size_t)(7 - k);
+        const size_t j = (size_t)(sl[k] & ~SLOT_DC) * 8u + (size_t)(7 - k);
         if (!REMAP) {
-            m += C.m[j];
-            for (int a = 0; a < 3; a++) p[a] += C.p[a * cap + j];
+            const double2 *q = reinterpret_cast<const double2 *>(C.w + j * 8u);
+            const double2 a = q[0], b = q[1], c2 = q[2], d = q[3];
+            m += a.x;
+            p[0] += a.y;
+            p[1] += b.x;
+            p[2] += b.y;
+            f[0] += c2.x;
+            f[1] += c2.y;
+            f[2] += d.x;
+        } else {
+            const double2 *q = reinterpret_cast<const double2 *>(C.w + j * 4u);
+            const double2 a = q[0], b = q[1];
+            f[0] += a.x;
+            f[1] += a.y;
+            f[2] += b.x;
         }
-        for (int a = 0; a < 3; a++) f[a] += C.f[a * cap + j];
-        for (int a = 0; a < 3; a++) {
-            uint32_t id = C.dc[a * cap + j];
-            if (id != MPM_NONE) {
-                int r = P.dc_rank[id];
-                if (r > w.rank[a]) {
-                    w.rank[a] = r;
-                    w.v[a] = P.v[(size_t)a * n + id];
-                }
-            }
+        if (sl[k] & SLOT_DC) {
+            const uint32_t c = (uint32_t)I.cell_off + (uint32_t)(ix + (k >> 2)) +
+                               (uint32_t)g.ec[0] * ((uint32_t)(iy + ((k >> 1) & 1)) + (uint32_t)g.ec[1] * (uint32_t)(iz + (k & 1)));
+            dc_scan_cell(w, P, B, c, all_axes);
         }
     }
     if (!REMAP) {
@@ -1008,12 +1091,14 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_strain(Batch bt, PView P, NView N
                               (size_t)g.nn[0] * ((size_t)(b[1] + oy) + (size_t)g.nn[1] * (b[2] + oz));
                 double N, gr[3];
                 shape_full(as[0][ox], as[1][oy], as[2][oz], g.h, N, gr[0], gr[1], gr[2]);
-                if (Nd.m[node] > I.mass_min) {
-                    x[0] += N * Nd.u[node] * dt;
-                    x[1] += N * Nd.u[nn + node] * dt;
-                    x[2] += N * Nd.u[2 * nn + node] * dt;
+                const double2 *rec = reinterpret_cast<const double2 *>(Nd.g2 + node * 8u);
+                const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+                if (r0.x > I.mass_min) {
+                    x[0] += N * r0.y * dt;
+                    x[1] += N * r1.x * dt;
+                    x[2] += N * r1.y * dt;
                 }
-                double vv[3] = {Nd.v[node], Nd.v[nn + node], Nd.v[2 * nn + node]};
+                double vv[3] = {r2.x, r2.y, r3.x};
 #pragma unroll
                 for (int c = 0; c < 3; c++)
 #pragma unroll
@@ -1447,8 +1532,10 @@ hipError_t launch_bin(const PView &P, const BView &B, uint64_t *block_sum, hipSt
     return hipGetLastError();
 }
 
+/* Blocks for the cell partials: one group of PART_CELLS occupied cells per
+ * block and pass, at most as many groups as particles allow. */
 static unsigned partial_grid(int64_t n) {
-    int64_t b = blocks_for(8 * std::max<int64_t>(n, 1), BLOCK);
+    int64_t b = (std::max<int64_t>(n, 1) + PART_CELLS - 1) / PART_CELLS;
     return (unsigned)std::min<int64_t>(b, 8192);
 }
 
