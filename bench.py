@@ -23,26 +23,32 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s at 8M J2 particles; surrogate vs exact speedup at stated err"
 PEAK_HBM_GBS = 8000.0
 
-# Algorithmic bytes (FP64 words counted once per pass that needs them):
-#   g2p_strain: reads x3 F9 sigma6 eps6 epsp6 R W4 V0 (36 words) and writes
-#     x3 F9 V sigma6 eps6 epsp6 R W4 (36 words), plus material id (4 B) and
-#     control flags (1 B); per active node it reads u3 v3 m (7 words).
-#   p2g_grid: reads x3 v3 m V sigma6 (14 words, +3 with external force);
-#     per active node writes m p3 f3 (7 words, the P2G flush) and reads
-#     fixed/friction and writes p3 f3 f/m3 reaction (13 words, the grid update).
-#   g2p_velocity: reads x3 v3, writes v3; per active node reads f/m3 m.
-#   remap_grid: reads x3 v3 m (7 words); per active node reads m p3, writes v3 p/m3.
-PER_PARTICLE = {"g2p_strain": 72 * 8 + 5, "p2g_grid": 14 * 8, "g2p_velocity": 9 * 8, "remap_grid": 7 * 8,
-                "bin": 0}
-PER_NODE = {"g2p_strain": 7 * 8, "p2g_grid": 20 * 8, "g2p_velocity": 4 * 8, "remap_grid": 10 * 8, "bin": 0}
-# kernels of each phase, per summation order (names as rocprofv3 reports them)
+# Algorithmic bytes per kernel (FP64 words counted once per pass that needs
+# them; intermediate records and the binning are implementation traffic):
+#   p2g_partials: reads x3 v3 m V sigma6 (14 words, +3 with external force)
+#   p2g_nodes: per active node writes m p3 f3 (7 words, the P2G flush) and
+#     reads fixed/friction and writes p3 f3 f/m3 reaction (13, grid update)
+#   g2p_velocity: reads x3 v3, writes v3; per active node reads f/m3 m
+#   remap_partials: reads x3 v3 m (7 words)
+#   remap_nodes: per active node reads m p3, writes v3 p/m3 (10 words)
+#   g2p_kinematics: reads x3 F9 V0, writes x3 F9 V (26 words) + material id
+#     and control flags (5 B); per active node reads p/m3 v3 m (7 words)
+#   g2p_constitutive: reads and writes sigma6 eps6 epsp6 R W4 (46 words)
+PER_PARTICLE = {"p2g_partials": 14 * 8, "p2g_nodes": 0, "g2p_velocity": 9 * 8, "remap_partials": 7 * 8,
+                "remap_nodes": 0, "g2p_kinematics": 26 * 8 + 5, "g2p_constitutive": 46 * 8, "bin": 0}
+PER_NODE = {"p2g_partials": 0, "p2g_nodes": 20 * 8, "g2p_velocity": 4 * 8, "remap_partials": 0,
+            "remap_nodes": 10 * 8, "g2p_kinematics": 7 * 8, "g2p_constitutive": 0, "bin": 0}
+# kernel of each slot, per summation order (names as rocprofv3 reports them)
 KERNELS = {
-    "cell": {"bin": ["k_scan_reduce", "k_scan_blocks", "k_scan_apply", "k_scatter", "k_cell_sort"],
-             "p2g_grid": ["k_cell_partials<false>", "k_node_cells<false>"], "g2p_velocity": ["k_g2p_velocity"],
-             "remap_grid": ["k_cell_partials<true>", "k_node_cells<true>"], "g2p_strain": ["k_g2p_strain<0, 0>"]},
-    "particle": {"bin": ["k_scan_reduce", "k_scan_blocks", "k_scan_apply", "k_scatter", "k_cell_sort"],
-                 "p2g_grid": ["k_p2g_grid"], "g2p_velocity": ["k_g2p_velocity"], "remap_grid": ["k_remap_grid"],
-                 "g2p_strain": ["k_g2p_strain<0, 0>"]},
+    "cell": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
+             "p2g_partials": "k_cell_partials<false>", "p2g_nodes": "k_node_cells<false>",
+             "g2p_velocity": "k_g2p_velocity", "remap_partials": "k_cell_partials<true>",
+             "remap_nodes": "k_node_cells<true>", "g2p_kinematics": "k_g2p_kinematics<0>",
+             "g2p_constitutive": "k_g2p_constitutive<0, 0>"},
+    "particle": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
+                 "p2g_partials": "k_p2g_grid", "p2g_nodes": None, "g2p_velocity": "k_g2p_velocity",
+                 "remap_partials": "k_remap_grid", "remap_nodes": None, "g2p_kinematics": "k_g2p_kinematics<0>",
+                 "g2p_constitutive": "k_g2p_constitutive<0, 0>"},
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_honeycomb8m.txt")
 
@@ -76,6 +82,82 @@ def cpu_baseline(scene, threads, steps=20, warmup=3):
     return sample.n * steps / el, sample.n, steps, el
 
 
+def batch_bench(n_inst, steps):
+    """Graded-foam batch: n_inst independent instances (seeds 1000 + i) in one
+    engine, steps replayed from one captured HIP graph."""
+    from code_mpm_amd import Engine, scenes
+    t = time.perf_counter()
+    scs = [scenes.graded_foam(1000 + i) for i in range(n_inst)]
+    eng = Engine(device=0, order="cell")
+    eng.load_batch(scs)
+    ntot = sum(sc.n for sc in scs)
+    print(f"# batch: {n_inst} instances, {ntot} particles, setup {time.perf_counter() - t:.1f} s",
+          file=sys.stderr, flush=True)
+    eng.step_graph(steps)          # capture and first replay
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.step_graph(steps)
+    eng.synchronize()
+    el = time.perf_counter() - t0
+    eng.close()
+    return {"workload": f"graded J2 foam batch, {n_inst} instances (seeds 1000..{999 + n_inst}), "
+                        "perfectly plastic, porosity U[0.3, 0.7], 1e4-2e5 particles each",
+            "instances": n_inst, "particles": ntot, "steps": steps, "graph": True,
+            "value": round(ntot * steps / el, 1), "unit": "particle-steps/s",
+            "ms_per_step": round(1e3 * el / steps, 4)}
+
+
+def surrogate_bench(scene, steps, train_steps=1500):
+    """The learned return map against the exact one on the honeycomb with
+    every particle's stress put on or near the yield surface (so that every
+    particle takes the plastic branch): step time both ways, the share of
+    points the network answers within the 1e-4 sigma_y residual, and the
+    stress error against the exact FP64 run after `steps` steps."""
+    import numpy as np
+    from code_mpm_amd import Engine
+    from code_mpm_amd import surrogate as su
+    mat = scene.materials[0]
+    rows = su.synthetic_states(mat, scene.n, seed=7)
+    scene.fields["STRESS"] = np.ascontiguousarray(rows[:, :6].T)
+    scene.fields["BACKSTRESS"] = np.ascontiguousarray(rows[:, 12])
+    s = su.Surrogate(seed=11)
+    data = su.make_training_set(mat, su.synthetic_states(mat, 200000, seed=3))
+    s.fit_normalization(data)
+    xs, ts = s.standardise(data)
+    t = time.perf_counter()
+    hist = s.train(xs, ts, steps=train_steps, batch=4096, lr=2e-3, seed=12)
+    t_train = time.perf_counter() - t
+    out = {}
+    stress = {}
+    for mode in ("exact", "surrogate"):
+        eng = Engine(device=0, order="cell")
+        eng.load_scene(scene)
+        if mode == "surrogate":
+            eng.set_surrogate(s, material=0, residual_tolerance=1e-4)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        eng.step(scene.dt, steps)
+        eng.synchronize()
+        el = time.perf_counter() - t0
+        out[mode] = {"ms_per_step": round(1e3 * el / steps, 4), "health": eng.health()}
+        stress[mode] = eng.get_particle("STRESS")
+        eng.close()
+    a, b = stress["exact"], stress["surrogate"]
+    sy = mat["yield_stress"]
+    rel = np.linalg.norm(a - b, axis=0) / np.maximum(np.linalg.norm(a, axis=0), sy)
+    h = out["surrogate"]["health"]
+    calls, falls = h["surrogate_calls"], h["surrogate_fallbacks"]
+    return {"workload": "honeycomb 8.0M with every stress on or near the yield surface (von Mises hardening)",
+            "steps": steps, "residual_tolerance_sigma_y": 1e-4,
+            "exact_ms_per_step": out["exact"]["ms_per_step"], "surrogate_ms_per_step": out["surrogate"]["ms_per_step"],
+            "speedup": round(out["exact"]["ms_per_step"] / out["surrogate"]["ms_per_step"], 3),
+            "network_calls": calls, "accepted_fraction": round((calls - falls) / max(calls, 1), 4),
+            "stress_rel_error_max": float(rel.max()), "stress_rel_error_mean": float(rel.mean()),
+            "stress_rel_error_l2": float(np.linalg.norm(a - b) / np.linalg.norm(a)),
+            "train_steps": train_steps, "train_seconds": round(t_train, 2),
+            "train_loss_first_last": [float(hist[:10].mean()), float(hist[-10:].mean())]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +170,11 @@ def main():
                     help="node summation order (cell: fast, particle: the reference's one-thread order)")
     ap.add_argument("--variant-steps", type=int, default=10,
                     help="timed steps of the contact and single-pass loops on the same state (0: skip)")
+    ap.add_argument("--batch-instances", type=int, default=256,
+                    help="graded-foam batch size for the batched measurement (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=50)
+    ap.add_argument("--surrogate-steps", type=int, default=10,
+                    help="steps of the learned-return-map comparison (0: skip)")
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="first run one device copy of this many GiB (the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -129,8 +216,8 @@ def main():
     value = scene.n * args.steps / elapsed
     print(f"# timed {args.steps} steps in {elapsed:.4f} s", file=sys.stderr, flush=True)
 
-    phases = eng.step_timed(scene.dt, args.profile_steps)
-    avg_ms = {k: v / args.profile_steps for k, v in phases.items()}
+    kms = eng.step_timed_kernels(scene.dt, args.profile_steps)
+    avg_ms = {k: v / args.profile_steps for k, v in kms.items()}
     active = int(eng.get_node("ACTIVE").sum())
     health = eng.health()
     steps_total = args.warmup + args.steps + args.profile_steps
@@ -139,14 +226,16 @@ def main():
     measured = read_traffic() if args.order == "cell" else {}
     per_kernel = {}
     for k, ms in avg_ms.items():
+        if kernels[k] is None:
+            continue
         b = PER_PARTICLE[k] * scene.n + PER_NODE[k] * active
-        t = [measured.get(kn) for kn in kernels[k]]
-        per_kernel[k] = dict(ms=round(ms, 4), kernels=kernels[k], algorithmic_bytes=b,
-                             gbs=round(b / (ms * 1e-3) / 1e9, 1) if b else None,
-                             traffic=round(sum(t)) if t and all(x is not None for x in t) else None)
-    # the dominant single kernel (a phase made of one kernel), so the figure
-    # matches one line of the rocprofv3 kernel summary
-    single = [k for k in avg_ms if len(kernels[k]) == 1 and PER_PARTICLE[k]]
+        t = [measured.get(kn) for kn in kernels[k].split("+")]
+        per_kernel[k] = dict(ms=round(ms, 4), kernel=kernels[k], algorithmic_bytes=b,
+                             gbs=round(b / (ms * 1e-3) / 1e9, 1) if (b and ms > 0) else None,
+                             traffic=round(sum(t)) if all(x is not None for x in t) else None)
+    # the dominant kernel (one kernel, so the figure matches one line of the
+    # rocprofv3 kernel summary)
+    single = [k for k in per_kernel if "+" not in per_kernel[k]["kernel"]]
     dominant = max(single, key=avg_ms.get)
     dom_bytes = per_kernel[dominant]["algorithmic_bytes"]
     achieved = dom_bytes / (avg_ms[dominant] * 1e-3) / 1e9
@@ -168,6 +257,10 @@ def main():
                          "ms_per_step": round(1e3 * el / args.variant_steps, 4), "steps": args.variant_steps}
         print(f"# {var}: {variants[var]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     eng.set_variant("doublepass")
+    eng.close()
+
+    batch = batch_bench(args.batch_instances, args.batch_steps) if args.batch_instances > 0 else None
+    sur = surrogate_bench(scenes.honeycomb_crush(), args.surrogate_steps) if args.surrogate_steps > 0 else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -186,13 +279,15 @@ def main():
                                "(von Mises hardening), platen 10 m/s, seed 1",
                    "particles": scene.n, "active_nodes": active, "plastic_fraction": round(plastic_frac, 5),
                    "return_map": "exact", "summation_order": args.order},
-        "roofline": {"bound": "hbm", "kernel": kernels[dominant][0], "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": kernels[dominant], "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes": dom_bytes,
                      "avg_launch_ms": round(avg_ms[dominant], 4)},
         "step_effective_gbs": round(step_bytes / (elapsed / args.steps) / 1e9, 1),
-        "phases": per_kernel,
+        "kernels": per_kernel,
         "variants": variants,
+        "batch": batch,
+        "surrogate": sur,
         "cpu_baseline": cpu,
         "gpu_over_cpu": round(value / cpu["value"], 1) if cpu else None,
     }

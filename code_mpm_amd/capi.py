@@ -27,6 +27,8 @@ NF = dict(MASS=(0, 1), MOMENTUM=(1, 3), VELOCITY=(2, 3), FORCE=(3, 3), FORCE_TEM
 MON = dict(DISPLACEMENT=0, MOMENTUM=1, ENERGY=2, STRESS=3)
 VARIANT = dict(doublepass=0, doublepass_contact=1, singlepass=2)
 ORDER = dict(particle=0, cell=1)
+KSLOTS = ("bin", "p2g_partials", "p2g_nodes", "g2p_velocity", "remap_partials", "remap_nodes", "g2p_kinematics",
+          "g2p_constitutive")
 PHASES = ("bin", "p2g_grid", "g2p_velocity", "remap_grid", "g2p_strain")
 
 
@@ -122,6 +124,7 @@ SIGNATURES = {
     "mpm_get_instance_monitors": (I32, [P, I32, ctypes.POINTER(Monitors)]),
     "mpm_run": (I32, [P, DBL, PI32]),
     "mpm_step_timed": (I32, [P, DBL, I32, PDBL]),
+    "mpm_step_timed_kernels": (I32, [P, DBL, I32, PDBL]),
     "mpm_synchronize": (I32, [P]),
     "mpm_get_time": (I32, [P, PDBL, PI64]),
     "mpm_get_node_count": (I32, [P, PI64]),

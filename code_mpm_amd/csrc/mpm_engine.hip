@@ -101,6 +101,8 @@ struct mpm_engine {
     int32_t *pmat = nullptr, *pdc_rank = nullptr;
     uint8_t *pdc_axes = nullptr;
     float *pdc_mult = nullptr;
+    double *pde = nullptr;
+    uint8_t *pflag = nullptr;
     bool has_fext = false;
 
     /* bins */
@@ -134,7 +136,6 @@ struct mpm_engine {
     int sur_material = -1;
     double sur_tol = 1.0e-4;
     uint32_t *qcount = nullptr, *qid = nullptr;
-    double *qde = nullptr;
     float *qx = nullptr, *qy = nullptr;
     uint8_t *qenv = nullptr;
     int64_t qcap = 0;
@@ -210,12 +211,12 @@ struct mpm_engine {
     }
     void free_particles() {
         for (int f = 0; f < MPM_PF_COUNT; f++) dfree(pf[f]);
-        dfree(pmat); dfree(pdc_rank); dfree(pdc_axes); dfree(pdc_mult);
+        dfree(pmat); dfree(pdc_rank); dfree(pdc_axes); dfree(pdc_mult); dfree(pde); dfree(pflag);
         dfree(bkey); dfree(bsorted); dfree(dpinst);
         for (int k = 0; k < MPM_MON_COUNT; k++) mon[k].clear();
     }
     void free_queue() {
-        dfree(qid); dfree(qde); dfree(qx); dfree(qy); dfree(qenv);
+        dfree(qid); dfree(qx); dfree(qy); dfree(qenv);
         qcap = 0;
     }
     int ensure_queue() {
@@ -224,7 +225,7 @@ struct mpm_engine {
         if (n == 0) return MPM_OK;
         const size_t sn = (size_t)n;
         int rc;
-        if ((rc = hip(hipMalloc(&qid, 4 * sn), "alloc queue")) || (rc = hip(hipMalloc(&qde, 48 * sn), "alloc queue")) ||
+        if ((rc = hip(hipMalloc(&qid, 4 * sn), "alloc queue")) ||
             (rc = hip(hipMalloc(&qx, 4 * MLP_IN * sn), "alloc queue")) ||
             (rc = hip(hipMalloc(&qy, 4 * MLP_OUT * sn), "alloc queue")) ||
             (rc = hip(hipMalloc(&qenv, sn), "alloc queue"))) {
@@ -244,7 +245,7 @@ struct mpm_engine {
             a.strain_scale = sur->strain_scale;
             a.norm = sur->norm;
         }
-        a.qcount = qcount; a.qid = qid; a.qde = qde; a.qx = qx; a.qy = qy; a.qenv = qenv;
+        a.qcount = qcount; a.qid = qid; a.qx = qx; a.qy = qy; a.qenv = qenv;
         a.cap = cap; a.capcount = capcount; a.capcap = capcap;
         return a;
     }
@@ -270,6 +271,8 @@ struct mpm_engine {
         P.dc_rank = pdc_rank;
         P.dc_axes = pdc_axes;
         P.dc_mult = pdc_mult;
+        P.de = pde;
+        P.flag = pflag;
         P.n = n;
         return P;
     }
@@ -300,6 +303,7 @@ struct mpm_engine {
         b.tz = tile[2];
         b.variant = variant;
         b.fext = has_fext ? 1 : 0;
+        b.nmat = mt.n;
         return b;
     }
 
@@ -429,7 +433,7 @@ struct mpm_engine {
     /* The kernels of one step (no host work).  With ev != nullptr an event is
      * recorded before the binning (ev[0]), before each later phase in launch
      * order (ev[1..4], see phase_order) and after the last (ev[PHASE_COUNT]). */
-    int enqueue_step(hipEvent_t *ev = nullptr) {
+    int enqueue_step(hipEvent_t *ev = nullptr, hipEvent_t *mid = nullptr) {
         const Batch b = batch();
         int rc;
         if ((rc = hip(launch_prepare(b, stream), "step prepare"))) return rc;
@@ -443,13 +447,14 @@ struct mpm_engine {
             if (ev && (rc = hip(hipEventRecord(ev[1 + q], stream), "event"))) return rc;
             if (ph == PHASE_G2P_STRAIN && sur && (rc = hip(launch_zero_u32(qcount, 1, stream), "queue reset")))
                 return rc;
-            if ((rc = hip(launch_phase(ph, b, pview(), nview(), bview(), parts, order, mt, d_health, sa, stream),
+            if ((rc = hip(launch_phase(ph, b, pview(), nview(), bview(), parts, order, mt, d_health, sa, stream,
+                                       mid ? mid[1 + q] : nullptr),
                           "step kernels")))
                 return rc;
         }
         if (sur) {
             /* the network on the queued points, then acceptance / fallback */
-            rc = hip(mlp_launch_forward(sur->params, sur->wbf16, qx, qy, n, stream, qcount), "surrogate forward");
+            rc = hip(mlp_launch_forward(sur->params, sur->wbf16, qx, 1, n, qy, 1, n, n, stream, qcount), "surrogate forward");
             if (!rc) rc = hip(launch_surrogate_post(b, pview(), mt, d_health, sa, stream), "surrogate post");
             if (rc) return rc;
         }
@@ -475,11 +480,11 @@ struct mpm_engine {
     }
 
     /* One eager step; dt0 (when given) replaces instance 0's dt_max. */
-    int step_once(const double *dt0, hipEvent_t *ev = nullptr) {
+    int step_once(const double *dt0, hipEvent_t *ev = nullptr, hipEvent_t *mid = nullptr) {
         int rc = begin_steps();
         if (rc) return rc;
         if ((rc = hip(launch_set_dt(batch(), 0, dt0 ? *dt0 : inst[0].dt_max, stream), "set dt"))) return rc;
-        if ((rc = enqueue_step(ev))) return rc;
+        if ((rc = enqueue_step(ev, mid))) return rc;
         keys_valid = true; /* g2p_strain wrote the next keys and counts */
         advance_mirror(dt0);
         return MPM_OK;
@@ -519,10 +524,11 @@ struct mpm_engine {
         /* the table upload cleared vt: recompute it from the device clock */
         return hip(launch_prepare(batch(), stream), "step prepare");
     }
-    /* One step in which the node phases (P2G + grid update, remap + BC),
-     * which only read particle and bin state and overwrite node state, are
-     * timed under every candidate tile shape; the fastest shape is kept and
-     * the step completes exactly as an ordinary one. */
+    /* One step in which the particle-to-grid phase, which only reads
+     * particle and bin state and overwrites node state (its result does not
+     * depend on the tile shape), is timed under every candidate node tile
+     * shape; the fastest shape is kept and the step then completes exactly
+     * as an ordinary one. */
     int autotune_step(int reps, double *best_ms) {
         int rc = begin_steps();
         if (rc) return rc;
@@ -530,63 +536,38 @@ struct mpm_engine {
         hipEvent_t e0, e1;
         if ((rc = hip(hipEventCreate(&e0), "event")) || (rc = hip(hipEventCreate(&e1), "event"))) return rc;
         const SurrogateArgs sa = sargs();
-        std::vector<double> t_p2g(kNumTileShapes, 0.0), t_remap(kNumTileShapes, 0.0);
+        std::vector<double> t_p2g(kNumTileShapes, 0.0);
         rc = hip(launch_prepare(batch(), stream), "step prepare");
         if (!rc) rc = hip(launch_bin(pview(), bview(), bblock, stream), "cell binning");
-        double once = 0.0;
-        int best = 0;
-        if (variant == VAR_SP) {
-            /* the single pass's grid update reads the P2G sums and updates
-             * them in place: time it after a fresh P2G each time */
-            for (int k = 0; k < kNumTileShapes && !rc; k++) {
-                double pair = 0.0;
-                rc = set_tile(kTileShapes[k]);
-                if (!rc) rc = time_phase(PHASE_P2G_GRID, reps, sa, e0, e1, t_p2g[k]);
-                if (!rc) rc = time_phase(PHASE_P2G_GRID, reps, sa, e0, e1, pair, PHASE_REMAP_GRID);
-                t_remap[k] = pair - t_p2g[k];
-            }
-            for (int k = 1; k < kNumTileShapes; k++)
-                if (t_p2g[k] + t_remap[k] < t_p2g[best] + t_remap[best]) best = k;
-            if (!rc) rc = set_tile(kTileShapes[best]);
-            if (!rc) rc = time_phase(PHASE_P2G_GRID, 1, sa, e0, e1, once, PHASE_REMAP_GRID);
-            if (!rc) rc = hip(launch_phase(PHASE_G2P_VELOCITY, batch(), pview(), nview(), bview(), parts, order, mt,
-                                           d_health, sa, stream), "g2p velocity");
-        } else {
-            /* both node phases only read particle and bin state and
-             * overwrite node state */
-            for (int k = 0; k < kNumTileShapes && !rc; k++) {
-                rc = set_tile(kTileShapes[k]);
-                if (!rc) rc = time_phase(PHASE_P2G_GRID, reps, sa, e0, e1, t_p2g[k]);
-            }
-            /* G2P pass 1 must see the node state of one P2G pass: redo it with the current shape */
-            if (!rc) rc = time_phase(PHASE_P2G_GRID, 1, sa, e0, e1, once);
-            if (!rc) rc = hip(launch_phase(PHASE_G2P_VELOCITY, batch(), pview(), nview(), bview(), parts, order, mt,
-                                           d_health, sa, stream), "g2p velocity");
-            for (int k = 0; k < kNumTileShapes && !rc; k++) {
-                rc = set_tile(kTileShapes[k]);
-                if (!rc) rc = time_phase(PHASE_REMAP_GRID, reps, sa, e0, e1, t_remap[k]);
-            }
-            for (int k = 1; k < kNumTileShapes; k++)
-                if (t_p2g[k] + t_remap[k] < t_p2g[best] + t_remap[best]) best = k;
-            if (!rc) rc = set_tile(kTileShapes[best]);
-            if (!rc) rc = time_phase(PHASE_REMAP_GRID, 1, sa, e0, e1, once);
+        for (int k = 0; k < kNumTileShapes && !rc; k++) {
+            rc = set_tile(kTileShapes[k]);
+            if (!rc) rc = time_phase(PHASE_P2G_GRID, reps, sa, e0, e1, t_p2g[k]);
         }
+        int best = 0;
+        for (int k = 1; k < kNumTileShapes; k++)
+            if (t_p2g[k] < t_p2g[best]) best = k;
+        if (!rc) rc = set_tile(kTileShapes[best]);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (rc) return rc;
-        if (sur && (rc = hip(launch_zero_u32(qcount, 1, stream), "queue reset"))) return rc;
-        if ((rc = hip(launch_phase(PHASE_G2P_STRAIN, batch(), pview(), nview(), bview(), parts, order, mt, d_health, sa,
-                                   stream), "g2p strain")))
-            return rc;
+        int order_of[4];
+        phase_order(order_of);
+        for (int q = 1; q < 4; q++) {
+            if (order_of[q] == PHASE_G2P_STRAIN && sur && (rc = hip(launch_zero_u32(qcount, 1, stream), "queue reset")))
+                return rc;
+            if ((rc = hip(launch_phase(order_of[q], batch(), pview(), nview(), bview(), parts, order, mt, d_health, sa,
+                                       stream), "step kernels")))
+                return rc;
+        }
         if (sur) {
-            rc = hip(mlp_launch_forward(sur->params, sur->wbf16, qx, qy, n, stream, qcount), "surrogate forward");
+            rc = hip(mlp_launch_forward(sur->params, sur->wbf16, qx, 1, n, qy, 1, n, n, stream, qcount), "surrogate forward");
             if (!rc) rc = hip(launch_surrogate_post(batch(), pview(), mt, d_health, sa, stream), "surrogate post");
             if (rc) return rc;
         }
         if ((rc = hip(launch_advance(batch(), stream), "step advance"))) return rc;
         keys_valid = true;
         advance_mirror(nullptr);
-        if (best_ms) *best_ms = t_p2g[best] + t_remap[best];
+        if (best_ms) *best_ms = t_p2g[best];
         return MPM_OK;
     }
 
@@ -700,6 +681,8 @@ struct mpm_engine {
             (rc = hip(hipMalloc(&pdc_rank, 4 * sn), "alloc control ranks")) ||
             (rc = hip(hipMalloc(&pdc_axes, sn), "alloc control axes")) ||
             (rc = hip(hipMalloc(&pdc_mult, 4 * sn), "alloc control multipliers")) ||
+            (rc = hip(hipMalloc(&pde, 48 * sn), "alloc strain increments")) ||
+            (rc = hip(hipMalloc(&pflag, sn), "alloc particle flags")) ||
             (rc = hip(hipMalloc(&bkey, 4 * sn), "alloc keys")) ||
             (rc = hip(hipMalloc(&bsorted, 4 * sn), "alloc sorted ids"))) {
             free_particles();
@@ -1137,6 +1120,46 @@ int mpm_step_timed(mpm_engine *e, double dt, int32_t nsteps, double phase_ms[MPM
     }
     for (int k = 0; k <= PHASE_COUNT; k++)
         if (ev[k]) (void)hipEventDestroy(ev[k]);
+    return rc;
+}
+
+int mpm_step_timed_kernels(mpm_engine *e, double dt, int32_t nsteps, double kernel_ms[MPM_KSLOT_COUNT]) {
+    CHECK_E(e);
+    if (nsteps < 0 || !kernel_ms) return e->fail(MPM_ERR_ARG, "bad timed-step arguments");
+    if (e->inst.size() > 1 && dt > 0.0) return e->fail(MPM_ERR_ARG, "multi-instance engines take dt <= 0");
+    for (int k = 0; k < MPM_KSLOT_COUNT; k++) kernel_ms[k] = 0.0;
+    hipEvent_t ev[PHASE_COUNT + 1] = {}, mid[PHASE_COUNT + 1] = {};
+    int rc = MPM_OK;
+    for (int k = 0; k <= PHASE_COUNT && !rc; k++) rc = e->hip(hipEventCreate(&ev[k]), "event create");
+    for (int k = 1; k < PHASE_COUNT && !rc; k++) rc = e->hip(hipEventCreate(&mid[k]), "event create");
+    /* first kernel slot of each phase; the phase's second kernel takes the next slot */
+    const int first[PHASE_COUNT] = {MPM_KSLOT_BIN, MPM_KSLOT_P2G_PARTIALS, MPM_KSLOT_G2P_VELOCITY,
+                                    MPM_KSLOT_REMAP_PARTIALS, MPM_KSLOT_G2P_KINEMATICS};
+    for (int s = 0; s < nsteps && !rc; s++) {
+        rc = e->step_once(dt > 0.0 ? &dt : nullptr, ev, mid);
+        if (!rc) rc = e->hip(hipEventSynchronize(ev[PHASE_COUNT]), "event sync");
+        int order_of[4];
+        e->phase_order(order_of);
+        float ms = 0.f;
+        if (!rc) rc = e->hip(hipEventElapsedTime(&ms, ev[0], ev[1]), "event time");
+        kernel_ms[MPM_KSLOT_BIN] += ms;
+        for (int q = 0; q < 4 && !rc; q++) {
+            const int ph = order_of[q];
+            float a = 0.f, b = 0.f;
+            rc = e->hip(hipEventElapsedTime(&a, ev[1 + q], mid[1 + q]), "event time");
+            if (!rc) rc = e->hip(hipEventElapsedTime(&b, mid[1 + q], ev[2 + q]), "event time");
+            if (ph == PHASE_G2P_VELOCITY) {
+                kernel_ms[first[ph]] += a + b;
+            } else {
+                kernel_ms[first[ph]] += a;
+                kernel_ms[first[ph] + 1] += b;
+            }
+        }
+    }
+    for (int k = 0; k <= PHASE_COUNT; k++) {
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
+        if (mid[k]) (void)hipEventDestroy(mid[k]);
+    }
     return rc;
 }
 

@@ -30,9 +30,8 @@ struct SurrogateArgs {
     MLPNorm norm;
     uint32_t *qcount;           /* queue length (device) */
     uint32_t *qid;              /* particle of each queue slot */
-    double *qde;                /* 6 x capacity strain increments */
-    float *qx;                  /* capacity x 13 normalised inputs */
-    float *qy;                  /* capacity x 7 normalised outputs */
+    float *qx;                  /* 13 x capacity normalised inputs, feature-major */
+    float *qy;                  /* 7 x capacity normalised outputs, feature-major */
     uint8_t *qenv;              /* inputs inside the training envelope */
     double *cap;                /* capture rows: sigma6 de6 R material */
     unsigned long long *capcount;
@@ -51,6 +50,8 @@ struct PView {
     int32_t *dc_rank;              /* latest position in the control list, -1 = not controlled */
     uint8_t *dc_axes;              /* bit0 x, bit1 y, bit2 z */
     float *dc_mult;                /* f_DisplacementControl_Multiplier */
+    double *de;                    /* 6n: this step's strain increment (kinematics -> return map) */
+    uint8_t *flag;                 /* bit0: non-finite position after the kinematics */
     int64_t n;
 };
 
@@ -124,6 +125,7 @@ struct Batch {
     int tx, ty, tz;           /* node tile shape, tx * ty * tz == NODE_BLOCK */
     int variant;              /* VAR_* */
     int fext;                 /* any particle carries an external force */
+    int nmat;                 /* materials in the table */
 };
 
 struct MonLists {
@@ -141,7 +143,8 @@ enum { PHASE_BIN = 0, PHASE_P2G_GRID = 1, PHASE_G2P_VELOCITY = 2, PHASE_REMAP_GR
        PHASE_COUNT = 5 };
 hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView &N, const BView &B,
                         const CellParts &C, int order, const MatTable &mt, unsigned long long *health,
-                        const SurrogateArgs &sg, hipStream_t s);
+                        const SurrogateArgs &sg, hipStream_t s,
+                        hipEvent_t mid = nullptr);
 hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t s);
 hipError_t launch_surrogate_post(const Batch &bt, const PView &P, const MatTable &mt, unsigned long long *health,
                                  const SurrogateArgs &sg, hipStream_t s);

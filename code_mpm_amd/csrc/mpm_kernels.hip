@@ -104,14 +104,31 @@ __device__ __forceinline__ uint32_t cell_key(const GridGeom &g, double x, double
     return (uint32_t)c[0] + (uint32_t)g.ec[0] * ((uint32_t)c[1] + (uint32_t)g.ec[1] * (uint32_t)c[2]);
 }
 
+/* count[key] += 1 for every lane with want set, one atomic per distinct
+ * key in the wave (neighbouring particles mostly share a cell).  Every lane
+ * of the wave must call it. */
+__device__ __forceinline__ void wave_key_add(uint32_t *count, uint32_t key, bool want) {
+    uint64_t pending = __ballot(want);
+    const int lane = (int)(threadIdx.x & 63);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lk = __shfl(key, leader, 64);
+        const uint64_t same = __ballot(want && key == lk);
+        if (lane == leader) atomicAdd(&count[lk], (uint32_t)__popcll(same));
+        pending &= ~same;
+    }
+}
+
 __global__ __launch_bounds__(BLOCK) void k_keys(Batch bt, PView P, BView B) {
     uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-    if (i >= P.n) return;
-    const Inst &I = bt.inst[inst_of_particle(bt, i)];
-    uint32_t k = cell_key(I.g, P.x[i], P.x[P.n + i], P.x[2 * P.n + i]);
-    if (k != MPM_NONE) k += (uint32_t)I.cell_off;
-    B.key[i] = k;
-    if (k != MPM_NONE) atomicAdd(&B.count[k], 1u);
+    uint32_t k = MPM_NONE;
+    if (i < P.n) {
+        const Inst &I = bt.inst[inst_of_particle(bt, i)];
+        k = cell_key(I.g, P.x[i], P.x[P.n + i], P.x[2 * P.n + i]);
+        if (k != MPM_NONE) k += (uint32_t)I.cell_off;
+        B.key[i] = k;
+    }
+    wave_key_add(B.count, k, k != MPM_NONE);
 }
 
 /* ---- exclusive scan of cell counts (reduce, scan of block sums, apply) ---- */
@@ -198,20 +215,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(const uint32_t *count, int
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_scatter(PView P, BView B) {
-    uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-    if (i >= P.n) return;
-    uint32_t k = B.key[i];
-    if (k == MPM_NONE) return;
-    uint32_t slot = B.start[k] + atomicSub(&B.count[k], 1u) - 1u;
-    B.sorted[slot] = i;
-    if (P.dc_rank[i] >= 0) atomicOr(&B.slot[k], SLOT_DC); /* the cell holds a controlled particle */
-}
-
-/* Ascending ids inside each cell: the order the node gathers merge in. */
-__global__ __launch_bounds__(BLOCK) void k_cell_sort(BView B) {
-    int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (c >= B.necells) return;
+__device__ __forceinline__ void sort_cell(const BView &B, uint32_t c) {
     uint32_t s = B.start[c], e = B.start[c + 1];
     if (e - s < 2) return;
     for (uint32_t i = s + 1; i < e; i++) {
@@ -225,6 +229,42 @@ __global__ __launch_bounds__(BLOCK) void k_cell_sort(BView B) {
     }
 }
 
+
+/* Particle ids into their cell's range of the sorted list.  The lanes of a
+ * wave that share a cell take consecutive slots in id order with one atomic
+ * decrement of the cell's count per wave (which also leaves the counts at
+ * zero for the next step); k_cell_sort orders what different waves put in
+ * one cell. */
+__global__ __launch_bounds__(BLOCK) void k_scatter(PView P, BView B) {
+    uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    const bool live = i < P.n;
+    const uint32_t k = live ? B.key[i] : MPM_NONE;
+    const bool want = k != MPM_NONE;
+    const int lane = (int)(threadIdx.x & 63);
+    uint64_t pending = __ballot(want);
+    uint32_t slot = 0;
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lk = __shfl(k, leader, 64);
+        const uint64_t same = __ballot(want && k == lk);
+        const uint32_t cnt = (uint32_t)__popcll(same);
+        uint32_t top = 0;
+        if (lane == leader) top = atomicSub(&B.count[lk], cnt);
+        top = __shfl(top, leader, 64);
+        if ((same >> lane) & 1ull) slot = B.start[lk] + top - cnt + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+    }
+    if (!want) return;
+    B.sorted[slot] = i;
+    if (P.dc_rank[i] >= 0) atomicOr(&B.slot[k], SLOT_DC); /* the cell holds a controlled particle */
+}
+
+/* Ascending ids inside each occupied cell: the order every node sum uses. */
+__global__ __launch_bounds__(BLOCK) void k_cell_sort(BView B) {
+    const uint32_t nocc = *B.nocc;
+    for (uint32_t t = blockIdx.x * BLOCK + threadIdx.x; t < nocc; t += gridDim.x * BLOCK)
+        sort_cell(B, B.occ[t]);
+}
 /* ---- 8-way merge of a node's adjacent cell lists in ascending id order ---- */
 
 struct Merge8 {
@@ -578,48 +618,66 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd,
     node_p2g_finish(bt, I, Nd, node, m, p, f, w);
 }
 
-/* G2P pass 1: v_p += N * (f_i / m_i) * dt over the adjacent nodes in
- * GridPoint_Mediator order (x, then y, then z innermost), DP:490-510. */
-__global__ __launch_bounds__(BLOCK) void k_g2p_velocity(Batch bt, PView P, NView Nd) {
-    uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-    if (i >= P.n) return;
-    if (bt.variant != VAR_DP && P.dc_rank[i] >= 0) return; /* DPC:309-310, SP:279-280 */
-    const Inst &I = bt.inst[inst_of_particle(bt, i)];
+/* G2P pass 1 for one particle: v_p += N * (f_i / m_i) * dt over the
+ * adjacent nodes in GridPoint_Mediator order (x, then y, then z innermost),
+ * DP:490-510. */
+__device__ __forceinline__ void g2p_velocity_update(const Inst &I, const NView &Nd, const double x[3], double v[3]) {
     const GridGeom &g = I.g;
-    const size_t n = P.n, nn = (size_t)bt.nnodes;
-    double x[3] = {P.x[i], P.x[n + i], P.x[2 * n + i]};
+    /* grid fields read once, outside any branch */
+    const double gL[3] = {g.L[0], g.L[1], g.L[2]}, gh[3] = {g.h[0], g.h[1], g.h[2]};
+    const int gc[3] = {g.cells[0], g.cells[1], g.cells[2]}, gn[3] = {g.nn[0], g.nn[1], g.nn[2]};
     int b[3], lo[3], hi[3];
     double s[3][2];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        base_cell(x[a], g.h[a], g.nn[a], b[a], lo[a], hi[a]);
+        base_cell(x[a], gh[a], gn[a], b[a], lo[a], hi[a]);
 #pragma unroll
-        for (int o = 0; o < 2; o++) s[a][o] = axis_shape(x[a], node_coord(b[a] + o, g.L[a], g.cells[a]), g.h[a]).s;
+        for (int o = 0; o < 2; o++) s[a][o] = axis_shape(x[a], node_coord(b[a] + o, gL[a], gc[a]), gh[a]).s;
     }
     bool ok[3][2];
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int o = 0; o < 2; o++) ok[a][o] = o >= lo[a] && o < hi[a] && b[a] + o >= 0 && b[a] + o <= g.nn[a] - 1;
-    double v[3] = {P.v[i], P.v[n + i], P.v[2 * n + i]};
-    const double dt = I.dt;
-    /* corners in GridPoint_Mediator order: x outer, z inner */
+        for (int o = 0; o < 2; o++)
+            ok[a][o] = (o >= lo[a]) & (o < hi[a]) & (b[a] + o >= 0) & (b[a] + o <= gn[a] - 1);
+    /* instance fields read once: inside the per-corner branches they would
+     * be reloaded (and waited for) corner by corner */
+    const double dt = I.dt, mass_min = I.mass_min;
+    const size_t node_off = (size_t)I.node_off, nx = (size_t)gn[0], ny = (size_t)gn[1];
+    /* branch-free over the 8 corners (off-grid corners read the instance's
+     * first node and add nothing), so that all gathers can be in flight */
+    double2 ra[8], rb[8];
+    bool in[8];
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const int ox = c >> 2, oy = (c >> 1) & 1, oz = c & 1;
-        if (!(ok[0][ox] && ok[1][oy] && ok[2][oz])) continue;
-        size_t node = (size_t)I.node_off + (size_t)(b[0] + ox) +
-                      (size_t)g.nn[0] * ((size_t)(b[1] + oy) + (size_t)g.nn[1] * (b[2] + oz));
+        in[c] = ok[0][ox] & ok[1][oy] & ok[2][oz];
+        const size_t node =
+            in[c] ? node_off + (size_t)(b[0] + ox) + nx * ((size_t)(b[1] + oy) + ny * (size_t)(b[2] + oz)) : node_off;
         const double2 *r = reinterpret_cast<const double2 *>(Nd.g1 + node * 4u);
-        const double2 r0 = r[0];
-        if (r0.x > I.mass_min) {
-            const double2 r1 = r[1];
-            double N = s[0][ox] * s[1][oy] * s[2][oz];
-            v[0] += N * r0.y * dt;
-            v[1] += N * r1.x * dt;
-            v[2] += N * r1.y * dt;
-        }
+        ra[c] = r[0];
+        rb[c] = r[1];
     }
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int ox = c >> 2, oy = (c >> 1) & 1, oz = c & 1;
+        const bool take = in[c] & (ra[c].x > mass_min);
+        const double N = s[0][ox] * s[1][oy] * s[2][oz];
+        const double t0 = v[0] + N * ra[c].y * dt, t1 = v[1] + N * rb[c].x * dt, t2 = v[2] + N * rb[c].y * dt;
+        v[0] = take ? t0 : v[0];
+        v[1] = take ? t1 : v[1];
+        v[2] = take ? t2 : v[2];
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_g2p_velocity(Batch bt, PView P, NView Nd) {
+    uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (i >= P.n) return;
+    if (bt.variant != VAR_DP && P.dc_rank[i] >= 0) return; /* DPC:309-310, SP:279-280 */
+    const size_t n = P.n;
+    const double x[3] = {P.x[i], P.x[n + i], P.x[2 * n + i]};
+    double v[3] = {P.v[i], P.v[n + i], P.v[2 * n + i]};
+    g2p_velocity_update(bt.inst[inst_of_particle(bt, i)], Nd, x, v);
     P.v[i] = v[0];
     P.v[n + i] = v[1];
     P.v[2 * n + i] = v[2];
@@ -746,7 +804,10 @@ __device__ __forceinline__ void node_remap_dispatch(const Batch &bt, const Inst 
 #define PART_CELLS (BLOCK / 8)
 #define PART_STAGE 256
 
-template <bool REMAP>
+/* G1 (remap pass of DoublePass and its contact variant): G2P pass 1 runs
+ * here too, on each particle as it is staged (each particle is staged by
+ * exactly one block), and the updated velocity is written back. */
+template <bool REMAP, bool G1>
 __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NView Nd, BView B, CellParts C) {
     /* staged particle inputs: x, m, v, then (P2G) V, sigma, f_ext */
     __shared__ double sx[3][PART_STAGE], sm[PART_STAGE], sv[3][PART_STAGE];
@@ -806,14 +867,23 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
             __syncthreads();
             for (uint32_t q = base + threadIdx.x; q < top; q += BLOCK) {
                 const uint32_t id = B.sorted[q], j = q - base;
-                sx[0][j] = P.x[id];
-                sx[1][j] = P.x[n + id];
-                sx[2][j] = P.x[2 * n + id];
+                const double x[3] = {P.x[id], P.x[n + id], P.x[2 * n + id]};
+                double v[3] = {P.v[id], P.v[n + id], P.v[2 * n + id]};
+                const int dcr = (skip_dc || (G1 && bt.variant != VAR_DP)) ? P.dc_rank[id] : -1;
+                if (G1 && !(bt.variant != VAR_DP && dcr >= 0)) { /* DPC:309-310 */
+                    g2p_velocity_update(bt.inst[inst_of_particle(bt, id)], Nd, x, v);
+                    P.v[id] = v[0];
+                    P.v[n + id] = v[1];
+                    P.v[2 * n + id] = v[2];
+                }
+                sx[0][j] = x[0];
+                sx[1][j] = x[1];
+                sx[2][j] = x[2];
                 sm[j] = P.m[id];
-                sv[0][j] = P.v[id];
-                sv[1][j] = P.v[n + id];
-                sv[2][j] = P.v[2 * n + id];
-                sdc[j] = skip_dc ? P.dc_rank[id] : -1;
+                sv[0][j] = v[0];
+                sv[1][j] = v[1];
+                sv[2][j] = v[2];
+                sdc[j] = skip_dc ? dcr : -1;
                 if (!REMAP) {
                     sV[j] = P.V[id];
 #pragma unroll
@@ -907,47 +977,73 @@ __global__ __launch_bounds__(BLOCK) void k_node_cells(Batch bt, PView P, NView N
     w.rank[0] = w.rank[1] = w.rank[2] = -1;
     w.v[0] = w.v[1] = w.v[2] = 0.0;
     const bool all_axes = bt.variant != VAR_DP;
-    bool any = false;
+    const uint32_t cell_off = (uint32_t)I.cell_off, ecx = (uint32_t)g.ec[0], ecy = (uint32_t)g.ec[1];
     uint32_t sl[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        const uint32_t c = (uint32_t)I.cell_off + (uint32_t)(ix + (k >> 2)) +
-                           (uint32_t)g.ec[0] * ((uint32_t)(iy + ((k >> 1) & 1)) + (uint32_t)g.ec[1] * (uint32_t)(iz + (k & 1)));
+        const uint32_t c = cell_off + (uint32_t)(ix + (k >> 2)) +
+                           ecx * ((uint32_t)(iy + ((k >> 1) & 1)) + ecy * (uint32_t)(iz + (k & 1)));
         sl[k] = B.slot[c];
     }
+    bool any = false;
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        if (sl[k] == MPM_NONE) continue;
-        any = true;
-        const size_t j = (size_t)(sl[k] & ~SLOT_DC) * 8u + (size_t)(7 - k);
-        if (!REMAP) {
+    for (int k = 0; k < 8; k++) any |= sl[k] != MPM_NONE;
+    if (!REMAP && !any) {
+        Nd.active[node] = 0;
+        return;
+    }
+    /* all partial records first (an empty cell reads record 0 and adds
+     * nothing), then the sums in neighbour order */
+    if (!REMAP) {
+        double2 ra[8], rb[8], rc[8], rd[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const size_t j = sl[k] == MPM_NONE ? 0 : (size_t)(sl[k] & ~SLOT_DC) * 8u + (size_t)(7 - k);
             const double2 *q = reinterpret_cast<const double2 *>(C.w + j * 8u);
-            const double2 a = q[0], b = q[1], c2 = q[2], d = q[3];
-            m += a.x;
-            p[0] += a.y;
-            p[1] += b.x;
-            p[2] += b.y;
-            f[0] += c2.x;
-            f[1] += c2.y;
-            f[2] += d.x;
-        } else {
-            const double2 *q = reinterpret_cast<const double2 *>(C.w + j * 4u);
-            const double2 a = q[0], b = q[1];
-            f[0] += a.x;
-            f[1] += a.y;
-            f[2] += b.x;
+            ra[k] = q[0];
+            rb[k] = q[1];
+            rc[k] = q[2];
+            rd[k] = q[3];
         }
-        if (sl[k] & SLOT_DC) {
-            const uint32_t c = (uint32_t)I.cell_off + (uint32_t)(ix + (k >> 2)) +
-                               (uint32_t)g.ec[0] * ((uint32_t)(iy + ((k >> 1) & 1)) + (uint32_t)g.ec[1] * (uint32_t)(iz + (k & 1)));
-            dc_scan_cell(w, P, B, c, all_axes);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const bool o = sl[k] != MPM_NONE;
+            const double t0 = m + ra[k].x, t1 = p[0] + ra[k].y, t2 = p[1] + rb[k].x, t3 = p[2] + rb[k].y;
+            const double t4 = f[0] + rc[k].x, t5 = f[1] + rc[k].y, t6 = f[2] + rd[k].x;
+            m = o ? t0 : m;
+            p[0] = o ? t1 : p[0];
+            p[1] = o ? t2 : p[1];
+            p[2] = o ? t3 : p[2];
+            f[0] = o ? t4 : f[0];
+            f[1] = o ? t5 : f[1];
+            f[2] = o ? t6 : f[2];
+        }
+    } else {
+        double2 ra[8], rb[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const size_t j = sl[k] == MPM_NONE ? 0 : (size_t)(sl[k] & ~SLOT_DC) * 8u + (size_t)(7 - k);
+            const double2 *q = reinterpret_cast<const double2 *>(C.w + j * 4u);
+            ra[k] = q[0];
+            rb[k] = q[1];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const bool o = sl[k] != MPM_NONE;
+            const double t0 = f[0] + ra[k].x, t1 = f[1] + ra[k].y, t2 = f[2] + rb[k].x;
+            f[0] = o ? t0 : f[0];
+            f[1] = o ? t1 : f[1];
+            f[2] = o ? t2 : f[2];
         }
     }
-    if (!REMAP) {
-        if (!any) {
-            Nd.active[node] = 0;
-            return;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if (sl[k] != MPM_NONE && (sl[k] & SLOT_DC)) {
+            const uint32_t c = cell_off + (uint32_t)(ix + (k >> 2)) +
+                               ecx * ((uint32_t)(iy + ((k >> 1) & 1)) + ecy * (uint32_t)(iz + (k & 1)));
+            dc_scan_cell(w, P, B, c, all_axes);
         }
+    if (!REMAP) {
         node_p2g_finish(bt, I, Nd, node, m, p, f, w);
     } else {
         node_remap_dispatch(bt, I, Nd, node, Nd.m[node], f, w);
@@ -988,6 +1084,27 @@ __device__ __forceinline__ uint32_t finish_update(const PView &P, size_t i, cons
     uint32_t bad = 0;
     for (int k = 0; k < 6; k++) bad |= !isfinite(sig[k]);
     return bad;
+}
+
+/* A slot in an append-only queue, one atomic per wave: the active lanes
+ * that call it take consecutive slots in lane order after the wave's base. */
+__device__ __forceinline__ uint32_t wave_append_u32(uint32_t *counter) {
+    const uint64_t mask = __ballot(1);
+    const int lane = (int)(threadIdx.x & 63);
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+__device__ __forceinline__ unsigned long long wave_append_u64(unsigned long long *counter) {
+    const uint64_t mask = __ballot(1);
+    const int lane = (int)(threadIdx.x & 63);
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    return base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
 /* Stress, plastic strain, energies and backstress of the contact and
@@ -1037,33 +1154,35 @@ __device__ __forceinline__ void trial_state(const MatTable &mt, int m, double R,
     if (mt.type[m] == 4) K = K + R / sqrt(2.0);
 }
 
-/* runSimulation_G2P_P2_SmallStrain (DP:175-280) + the displacement-control
- * schedule (DP:766-789) + next step's binning key.
- *   MODE_EXACT      the exact FP64 return map for every particle;
- *   MODE_CAPTURE    as exact, and every yielding point's (sigma, de, R,
- *                   material) is appended to the sample buffer;
- *   MODE_SURROGATE  points of the surrogate's material whose trial state
- *                   yields are queued for the MLP; everything else, and
- *                   all kinematics, is finished here. */
-template <int MODE, int VAR>
-__global__ __launch_bounds__(BLOCK) void k_g2p_strain(Batch bt, PView P, NView Nd, BView B, MatTable mt,
-                                                     unsigned long long *health, SurrogateArgs sg) {
+/* runSimulation_G2P_P2_SmallStrain (DP:175-280), in two kernels so that
+ * each keeps enough loads in flight: the kinematics (node gather, x, L, the
+ * strain increment, F and V, the displacement-control schedule of
+ * DP:766-789 and the next step's binning key) and then the return map and
+ * state update, a pure per-particle stream.  The strain increment passes
+ * between them in P.de. */
+template <int VAR>
+__global__ __launch_bounds__(BLOCK) void k_g2p_kinematics(Batch bt, PView P, NView Nd, BView B,
+                                                         unsigned long long *health) {
     uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     const bool live = i < P.n;
-    uint32_t f_outside = 0, f_null = 0, f_plastic = 0, f_nonfinite = 0, f_queued = 0;
+    uint32_t f_outside = 0;
+    uint32_t next_key = MPM_NONE;
     if (live) {
         const Inst &I = bt.inst[inst_of_particle(bt, i)];
         const GridGeom &g = I.g;
-        const size_t n = P.n, nn = (size_t)bt.nnodes;
+        const size_t n = P.n;
+        /* grid fields read once, outside any branch */
+        const double gL[3] = {g.L[0], g.L[1], g.L[2]}, gh[3] = {g.h[0], g.h[1], g.h[2]};
+        const int gc[3] = {g.cells[0], g.cells[1], g.cells[2]}, gn[3] = {g.nn[0], g.nn[1], g.nn[2]};
         const double x0[3] = {P.x[i], P.x[n + i], P.x[2 * n + i]};
         double x[3] = {x0[0], x0[1], x0[2]};
         int b[3], lo[3], hi[3];
         AxisShape as[3][2];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            base_cell(x0[a], g.h[a], g.nn[a], b[a], lo[a], hi[a]);
+            base_cell(x0[a], gh[a], gn[a], b[a], lo[a], hi[a]);
 #pragma unroll
-            for (int o = 0; o < 2; o++) as[a][o] = axis_shape(x0[a], node_coord(b[a] + o, g.L[a], g.cells[a]), g.h[a]);
+            for (int o = 0; o < 2; o++) as[a][o] = axis_shape(x0[a], node_coord(b[a] + o, gL[a], gc[a]), gh[a]);
         }
         double L[3][3]; /* glm column-major L[col][row] */
 #pragma unroll
@@ -1076,39 +1195,52 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_strain(Batch bt, PView P, NView N
         for (int a = 0; a < 3; a++)
 #pragma unroll
             for (int o = 0; o < 2; o++)
-                ok[a][o] = o >= lo[a] && o < hi[a] && b[a] + o >= 0 && b[a] + o <= g.nn[a] - 1;
+                ok[a][o] = (o >= lo[a]) & (o < hi[a]) & (b[a] + o >= 0) & (b[a] + o <= gn[a] - 1);
         f_outside = !((ok[0][0] || ok[0][1]) && (ok[1][0] || ok[1][1]) && (ok[2][0] || ok[2][1]));
         /* the contact and single-pass loops leave controlled particles out
          * of the grid-to-particle pass (DPC:443-444, SP:279-280) */
         const bool ctl = VAR != VAR_DP && P.dc_rank[i] >= 0;
+        uint8_t flag = 0;
         if (!ctl) {
+            /* instance fields read once: inside the per-corner branches they
+             * would be reloaded (and waited for) corner by corner */
+            const double mass_min = I.mass_min;
+            const double hx[3] = {gh[0], gh[1], gh[2]};
+            const size_t node_off = (size_t)I.node_off, nx = (size_t)gn[0], ny = (size_t)gn[1];
             /* corners in GridPoint_Mediator order: x outer, z inner */
+            /* branch-free over the 8 corners (off-grid corners read the
+             * instance's first node and add nothing), so that all gathers can
+             * be in flight */
 #pragma unroll
             for (int cc = 0; cc < 8; cc++) {
                 const int ox = cc >> 2, oy = (cc >> 1) & 1, oz = cc & 1;
-                if (!(ok[0][ox] && ok[1][oy] && ok[2][oz])) continue;
-                size_t node = (size_t)I.node_off + (size_t)(b[0] + ox) +
-                              (size_t)g.nn[0] * ((size_t)(b[1] + oy) + (size_t)g.nn[1] * (b[2] + oz));
+                const bool in = ok[0][ox] && ok[1][oy] && ok[2][oz];
+                const size_t node =
+                    in ? node_off + (size_t)(b[0] + ox) + nx * ((size_t)(b[1] + oy) + ny * (size_t)(b[2] + oz))
+                       : node_off;
                 double N, gr[3];
-                shape_full(as[0][ox], as[1][oy], as[2][oz], g.h, N, gr[0], gr[1], gr[2]);
+                shape_full(as[0][ox], as[1][oy], as[2][oz], hx, N, gr[0], gr[1], gr[2]);
                 const double2 *rec = reinterpret_cast<const double2 *>(Nd.g2 + node * 8u);
                 const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
-                if (r0.x > I.mass_min) {
-                    x[0] += N * r0.y * dt;
-                    x[1] += N * r1.x * dt;
-                    x[2] += N * r1.y * dt;
-                }
-                double vv[3] = {r2.x, r2.y, r3.x};
+                const bool take = in && r0.x > mass_min;
+                const double t0 = x[0] + N * r0.y * dt, t1 = x[1] + N * r1.x * dt, t2 = x[2] + N * r1.y * dt;
+                x[0] = take ? t0 : x[0];
+                x[1] = take ? t1 : x[1];
+                x[2] = take ? t2 : x[2];
+                const double vv[3] = {r2.x, r2.y, r3.x};
 #pragma unroll
                 for (int c = 0; c < 3; c++)
 #pragma unroll
-                    for (int r = 0; r < 3; r++) L[c][r] += vv[r] * gr[c];
+                    for (int r = 0; r < 3; r++) {
+                        const double t = L[c][r] + vv[r] * gr[c];
+                        L[c][r] = in ? t : L[c][r];
+                    }
             }
             P.x[i] = x[0];
             P.x[n + i] = x[1];
             P.x[2 * n + i] = x[2];
             const int mat = P.mat[i];
-            const bool valid = mat >= 0 && mat < mt.n;
+            const bool valid = mat >= 0 && mat < bt.nmat;
             double dF[3][3];
 #pragma unroll
             for (int c = 0; c < 3; c++)
@@ -1136,10 +1268,10 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_strain(Batch bt, PView P, NView N
                 de[4] = dF[1][2] + dF[2][1];
                 de[5] = dF[2][0] + dF[0][2];
             }
-            /* kinematics: F and V.  DP: F = F dF after the material check
-             * (DP:242-245); DPC / SP: F += (L F) dt before it (DPC:464-467,
-             * SP:305-308) */
-            double V = P.V[i];
+#pragma unroll
+            for (int k = 0; k < 6; k++) P.de[k * n + i] = de[k];
+            /* DP: F = F dF after the material check (DP:242-245); DPC / SP:
+             * F += (L F) dt before it (DPC:464-467, SP:305-308) */
             if (VAR != VAR_DP || valid) {
                 double F[3][3], Fn[3][3];
 #pragma unroll
@@ -1166,81 +1298,11 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_strain(Batch bt, PView P, NView N
                 double det = +Fn[0][0] * (Fn[1][1] * Fn[2][2] - Fn[2][1] * Fn[1][2]) -
                              Fn[1][0] * (Fn[0][1] * Fn[2][2] - Fn[2][1] * Fn[0][2]) +
                              Fn[2][0] * (Fn[0][1] * Fn[1][2] - Fn[1][1] * Fn[0][2]);
-                V = det * P.V0[i];
-                P.V[i] = V;
+                P.V[i] = det * P.V0[i];
             }
-            if (VAR != VAR_DP) {
-#pragma unroll
-                for (int k = 0; k < 6; k++) P.eps[k * n + i] += de[k];
-            }
-            double sig[6];
-#pragma unroll
-            for (int k = 0; k < 6; k++) sig[k] = P.sig[k * n + i];
-            const double R = P.R[i];
-            if (!valid) f_null = 1;
-            if (!valid && VAR != VAR_SP) {
-                /* DP:226-230, DPC:491-495: message and skip */
-                for (int k = 0; k < 6; k++) f_nonfinite |= !isfinite(sig[k]);
-            } else if (!valid) {
-                /* SP with no material: the per-particle defaults (elastic,
-                 * E = nu = 0, MaterialPoint_BC.h:48-57) */
-                CRResult cr;
-                trial_increment(0.0, 0.0, de, cr.dsig);
-                for (int k = 0; k < 6; k++) cr.depsp[k] = 0.0;
-                cr.dR = 0.0;
-                cr.yielded = false;
-                f_nonfinite |= finish_update_new_stress(P, i, sig, cr, R, V, de);
-            } else {
-                bool queued = false;
-                if (MODE == MODE_SURROGATE && mat == sg.material) {
-                    double dtr[6], J2, K;
-                    trial_state(mt, mat, R, sig, de, dtr, J2, K);
-                    if (!(J2 <= K * K)) {
-                        /* queue for the network: exact strain increment + normalised features */
-                        uint32_t slot = atomicAdd(sg.qcount, 1u);
-                        sg.qid[slot] = i;
-                        const size_t cap = (size_t)P.n;
-#pragma unroll
-                        for (int k = 0; k < 6; k++) sg.qde[k * cap + slot] = de[k];
-                        float feat[MLP_IN];
-                        for (int k = 0; k < 6; k++) feat[k] = (float)(sig[k] / sg.stress_scale);
-                        for (int k = 0; k < 6; k++) feat[6 + k] = (float)(de[k] / sg.strain_scale);
-                        feat[12] = (float)(R / sg.stress_scale);
-                        uint8_t inside = 1;
-                        for (int k = 0; k < MLP_IN; k++) {
-                            float z = (feat[k] - sg.norm.in_mean[k]) / sg.norm.in_std[k];
-                            inside &= (z >= sg.norm.in_lo[k] && z <= sg.norm.in_hi[k]) ? 1 : 0;
-                            sg.qx[(size_t)slot * MLP_IN + k] = z;
-                        }
-                        sg.qenv[slot] = inside;
-                        queued = true;
-                        f_queued = 1;
-                        f_plastic = 1;
-                        /* stress and the rest are finished by the post kernel */
-                    }
-                }
-                if (!queued) {
-                    CRResult cr;
-                    constitutive(mt, mat, R, sig, de, cr);
-                    f_plastic = cr.yielded;
-                    if (MODE == MODE_CAPTURE && cr.yielded) {
-                        unsigned long long slot = atomicAdd(sg.capcount, 1ull);
-                        if (slot < (unsigned long long)sg.capcap) {
-                            double *row = sg.cap + slot * 14;
-                            for (int k = 0; k < 6; k++) row[k] = sig[k];
-                            for (int k = 0; k < 6; k++) row[6 + k] = de[k];
-                            row[12] = R;
-                            row[13] = (double)mat;
-                        }
-                    }
-                    if (VAR == VAR_DP)
-                        f_nonfinite |= finish_update(P, i, de, sig, cr, R, V, dt);
-                    else
-                        f_nonfinite |= finish_update_new_stress(P, i, sig, cr, R, V, de);
-                }
-            }
-            f_nonfinite |= !(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]));
+            flag = !(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2])) ? 1 : 0;
         }
+        P.flag[i] = flag;
         /* displacement-control schedule: DP:766-789 (flagged axes), or
          * DPC:531-540 / SP:369-378 (all axes, then x += v dt) */
         if (VAR == VAR_DP) {
@@ -1268,12 +1330,106 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_strain(Batch bt, PView P, NView N
             }
         }
         /* next step's binning */
-        uint32_t k = cell_key(g, x[0], x[1], x[2]);
-        if (k != MPM_NONE) k += (uint32_t)I.cell_off;
-        B.key[i] = k;
-        if (k != MPM_NONE) atomicAdd(&B.count[k], 1u);
+        next_key = cell_key(g, x[0], x[1], x[2]);
+        if (next_key != MPM_NONE) next_key += (uint32_t)I.cell_off;
+        B.key[i] = next_key;
     }
+    wave_key_add(B.count, next_key, next_key != MPM_NONE);
     block_count_add(&health[HEALTH_OUTSIDE], f_outside);
+}
+
+/* The return map and state update of the same pass (DP:204-278, DPC:479-524,
+ * SP:320-362) on the strain increment left by k_g2p_kinematics.
+ *   MODE_EXACT      the exact FP64 return map for every particle;
+ *   MODE_CAPTURE    as exact, and every yielding point's (sigma, de, R,
+ *                   material) is appended to the sample buffer;
+ *   MODE_SURROGATE  points of the surrogate's material whose trial state
+ *                   yields are queued for the MLP; everything else is
+ *                   finished here. */
+template <int MODE, int VAR>
+__global__ __launch_bounds__(BLOCK) void k_g2p_constitutive(Batch bt, PView P, MatTable mt,
+                                                           unsigned long long *health, SurrogateArgs sg) {
+    uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    uint32_t f_null = 0, f_plastic = 0, f_nonfinite = 0, f_queued = 0;
+    const size_t n = P.n;
+    if (i < P.n && !(VAR != VAR_DP && P.dc_rank[i] >= 0)) {
+        const double dt = bt.inst[inst_of_particle(bt, i)].dt;
+        const int mat = P.mat[i];
+        const bool valid = mat >= 0 && mat < mt.n;
+        double de[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) de[k] = P.de[k * n + i];
+        if (VAR != VAR_DP) {
+#pragma unroll
+            for (int k = 0; k < 6; k++) P.eps[k * n + i] += de[k];
+        }
+        const double V = P.V[i];
+        double sig[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) sig[k] = P.sig[k * n + i];
+        const double R = P.R[i];
+        f_nonfinite = P.flag[i] & 1u;
+        if (!valid) f_null = 1;
+        if (!valid && VAR != VAR_SP) {
+            /* DP:226-230, DPC:491-495: message and skip */
+            for (int k = 0; k < 6; k++) f_nonfinite |= !isfinite(sig[k]);
+        } else if (!valid) {
+            /* SP with no material: the per-particle defaults (elastic,
+             * E = nu = 0, MaterialPoint_BC.h:48-57) */
+            CRResult cr;
+            trial_increment(0.0, 0.0, de, cr.dsig);
+            for (int k = 0; k < 6; k++) cr.depsp[k] = 0.0;
+            cr.dR = 0.0;
+            cr.yielded = false;
+            f_nonfinite |= finish_update_new_stress(P, i, sig, cr, R, V, de);
+        } else {
+            bool queued = false;
+            if (MODE == MODE_SURROGATE && mat == sg.material) {
+                double dtr[6], J2, K;
+                trial_state(mt, mat, R, sig, de, dtr, J2, K);
+                if (!(J2 <= K * K)) {
+                    /* queue for the network: exact strain increment + normalised features */
+                    uint32_t slot = wave_append_u32(sg.qcount);
+                    sg.qid[slot] = i;
+                    const size_t cap = (size_t)P.n;
+                    float feat[MLP_IN];
+                    for (int k = 0; k < 6; k++) feat[k] = (float)(sig[k] / sg.stress_scale);
+                    for (int k = 0; k < 6; k++) feat[6 + k] = (float)(de[k] / sg.strain_scale);
+                    feat[12] = (float)(R / sg.stress_scale);
+                    uint8_t inside = 1;
+                    for (int k = 0; k < MLP_IN; k++) {
+                        float z = (feat[k] - sg.norm.in_mean[k]) / sg.norm.in_std[k];
+                        inside &= (z >= sg.norm.in_lo[k] && z <= sg.norm.in_hi[k]) ? 1 : 0;
+                        sg.qx[(size_t)k * cap + slot] = z; /* feature-major: coalesced */
+                    }
+                    sg.qenv[slot] = inside;
+                    queued = true;
+                    f_queued = 1;
+                    f_plastic = 1;
+                    /* stress and the rest are finished by the post kernel */
+                }
+            }
+            if (!queued) {
+                CRResult cr;
+                constitutive(mt, mat, R, sig, de, cr);
+                f_plastic = cr.yielded;
+                if (MODE == MODE_CAPTURE && cr.yielded) {
+                    unsigned long long slot = wave_append_u64(sg.capcount);
+                    if (slot < (unsigned long long)sg.capcap) {
+                        double *row = sg.cap + slot * 14;
+                        for (int k = 0; k < 6; k++) row[k] = sig[k];
+                        for (int k = 0; k < 6; k++) row[6 + k] = de[k];
+                        row[12] = R;
+                        row[13] = (double)mat;
+                    }
+                }
+                if (VAR == VAR_DP)
+                    f_nonfinite |= finish_update(P, i, de, sig, cr, R, V, dt);
+                else
+                    f_nonfinite |= finish_update_new_stress(P, i, sig, cr, R, V, de);
+            }
+        }
+    }
     block_count_add(&health[HEALTH_NULL], f_null);
     block_count_add(&health[HEALTH_PLASTIC], f_plastic);
     block_count_add(&health[HEALTH_NONFINITE], f_nonfinite);
@@ -1300,17 +1456,17 @@ __global__ __launch_bounds__(BLOCK) void k_surrogate_post(Batch bt, PView P, Mat
             const uint32_t i = sg.qid[slot];
             const int m = P.mat[i];
             double de[6], sig[6];
-            for (int k = 0; k < 6; k++) de[k] = sg.qde[k * cap + slot];
+            for (int k = 0; k < 6; k++) de[k] = P.de[k * n + i];
             for (int k = 0; k < 6; k++) sig[k] = P.sig[k * n + i];
             const double R = P.R[i];
             double dtr[6], J2, K;
             trial_state(mt, m, R, sig, de, dtr, J2, K);
             CRResult cr;
             bool accept = sg.qenv[slot] != 0;
-            const float *y = sg.qy + (size_t)slot * MLP_OUT;
             double corr[6];
             for (int k = 0; k < 6; k++)
-                corr[k] = ((double)y[k] * (double)sg.norm.out_std[k] + (double)sg.norm.out_mean[k]) * sg.stress_scale;
+                corr[k] = ((double)sg.qy[(size_t)k * cap + slot] * (double)sg.norm.out_std[k] +
+                           (double)sg.norm.out_mean[k]) * sg.stress_scale;
             /* the radial return keeps the trial pressure (plastic flow is
              * deviatoric): drop the hydrostatic part of the prediction, which
              * the yield-surface check below cannot see */
@@ -1528,7 +1684,8 @@ hipError_t launch_bin(const PView &P, const BView &B, uint64_t *block_sum, hipSt
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(BLOCK), 0, s, B.count, B.necells, block_sum, B.start,
                        B.slot, B.occ, B.nocc);
     if (P.n > 0) hipLaunchKernelGGL(k_scatter, dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, P, B);
-    hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(B.necells, BLOCK)), dim3(BLOCK), 0, s, B);
+    hipLaunchKernelGGL(k_cell_sort, dim3((unsigned)std::min<int64_t>(blocks_for(std::max<int64_t>(P.n, 1), BLOCK), 4096)),
+                       dim3(BLOCK), 0, s, B);
     return hipGetLastError();
 }
 
@@ -1541,51 +1698,76 @@ static unsigned partial_grid(int64_t n) {
 
 template <int VAR>
 static void launch_g2p_strain(int mode, dim3 grid, hipStream_t s, const Batch &bt, const PView &P, const NView &N,
-                              const BView &B, const MatTable &mt, unsigned long long *health, const SurrogateArgs &sg) {
+                              const BView &B, const MatTable &mt, unsigned long long *health, const SurrogateArgs &sg,
+                              hipEvent_t mid) {
+    hipLaunchKernelGGL((k_g2p_kinematics<VAR>), grid, dim3(BLOCK), 0, s, bt, P, N, B, health);
+    if (mid) (void)hipEventRecord(mid, s);
     if (mode == MODE_SURROGATE)
-        hipLaunchKernelGGL((k_g2p_strain<MODE_SURROGATE, VAR>), grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health, sg);
+        hipLaunchKernelGGL((k_g2p_constitutive<MODE_SURROGATE, VAR>), grid, dim3(BLOCK), 0, s, bt, P, mt, health, sg);
     else if (mode == MODE_CAPTURE)
-        hipLaunchKernelGGL((k_g2p_strain<MODE_CAPTURE, VAR>), grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health, sg);
+        hipLaunchKernelGGL((k_g2p_constitutive<MODE_CAPTURE, VAR>), grid, dim3(BLOCK), 0, s, bt, P, mt, health, sg);
     else
-        hipLaunchKernelGGL((k_g2p_strain<MODE_EXACT, VAR>), grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health, sg);
+        hipLaunchKernelGGL((k_g2p_constitutive<MODE_EXACT, VAR>), grid, dim3(BLOCK), 0, s, bt, P, mt, health, sg);
 }
 
 hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView &N, const BView &B,
                         const CellParts &C, int order, const MatTable &mt, unsigned long long *health,
-                        const SurrogateArgs &sg, hipStream_t s) {
+                        const SurrogateArgs &sg, hipStream_t s, hipEvent_t mid) {
     switch (phase) {
     case PHASE_P2G_GRID:
-        if (!bt.ntiles) break;
+        if (!bt.ntiles) {
+            if (mid) (void)hipEventRecord(mid, s);
+            break;
+        }
         if (order == ORDER_CELL) {
             if (P.n > 0)
-                hipLaunchKernelGGL(k_cell_partials<false>, dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt, P, N, B, C);
+                hipLaunchKernelGGL((k_cell_partials<false, false>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt, P,
+                                   N, B, C);
+            if (mid) (void)hipEventRecord(mid, s);
             hipLaunchKernelGGL(k_node_cells<false>, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
         } else {
             hipLaunchKernelGGL(k_p2g_grid, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B);
+            if (mid) (void)hipEventRecord(mid, s);
         }
         break;
     case PHASE_G2P_VELOCITY:
-        if (P.n > 0) hipLaunchKernelGGL(k_g2p_velocity, dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, bt, P, N);
+        /* in the cell order of DoublePass and its contact variant the remap's
+         * partials kernel runs this pass */
+        if (P.n > 0 && !(order == ORDER_CELL && bt.variant != VAR_SP))
+            hipLaunchKernelGGL(k_g2p_velocity, dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, bt, P, N);
+        if (mid) (void)hipEventRecord(mid, s);
         break;
     case PHASE_REMAP_GRID:
-        if (!bt.ntiles) break;
+        if (!bt.ntiles) {
+            if (mid) (void)hipEventRecord(mid, s);
+            break;
+        }
         if (order == ORDER_CELL) {
-            if (P.n > 0)
-                hipLaunchKernelGGL(k_cell_partials<true>, dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt, P, N, B, C);
+            if (P.n > 0) {
+                if (bt.variant != VAR_SP)
+                    hipLaunchKernelGGL((k_cell_partials<true, true>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt,
+                                       P, N, B, C);
+                else
+                    hipLaunchKernelGGL((k_cell_partials<true, false>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt,
+                                       P, N, B, C);
+            }
+            if (mid) (void)hipEventRecord(mid, s);
             hipLaunchKernelGGL(k_node_cells<true>, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
         } else {
             hipLaunchKernelGGL(k_remap_grid, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B);
+            if (mid) (void)hipEventRecord(mid, s);
         }
         break;
     case PHASE_G2P_STRAIN:
+        if (P.n == 0 && mid) (void)hipEventRecord(mid, s);
         if (P.n > 0) {
             dim3 grid(blocks_for(P.n, BLOCK));
             if (bt.variant == VAR_DPC)
-                launch_g2p_strain<VAR_DPC>(sg.mode, grid, s, bt, P, N, B, mt, health, sg);
+                launch_g2p_strain<VAR_DPC>(sg.mode, grid, s, bt, P, N, B, mt, health, sg, mid);
             else if (bt.variant == VAR_SP)
-                launch_g2p_strain<VAR_SP>(sg.mode, grid, s, bt, P, N, B, mt, health, sg);
+                launch_g2p_strain<VAR_SP>(sg.mode, grid, s, bt, P, N, B, mt, health, sg, mid);
             else
-                launch_g2p_strain<VAR_DP>(sg.mode, grid, s, bt, P, N, B, mt, health, sg);
+                launch_g2p_strain<VAR_DP>(sg.mode, grid, s, bt, P, N, B, mt, health, sg, mid);
         }
         break;
     default:
@@ -1602,7 +1784,9 @@ hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t s) {
 hipError_t launch_surrogate_post(const Batch &bt, const PView &P, const MatTable &mt, unsigned long long *health,
                                  const SurrogateArgs &sg, hipStream_t s) {
     if (P.n == 0) return hipSuccess;
-    unsigned grid = (unsigned)std::min<int64_t>(blocks_for(P.n, BLOCK), 2048);
+    /* the queue length is only known on the device: enough blocks for a
+     * full queue, the surplus exits at once */
+    unsigned grid = (unsigned)std::min<int64_t>(blocks_for(P.n, BLOCK), 32768);
     hipLaunchKernelGGL(k_surrogate_post, dim3(grid), dim3(BLOCK), 0, s, bt, P, mt, health, sg);
     return hipGetLastError();
 }

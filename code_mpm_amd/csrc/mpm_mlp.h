@@ -40,10 +40,12 @@ struct MLPNorm {
     float out_mean[MLP_OUT], out_std[MLP_OUT];
 };
 
-/* Normalised FP32 training/inference rows: x [n][MLP_IN], y [n][MLP_OUT]. */
-/* dcount (device, optional) caps n at launch time. */
-hipError_t mlp_launch_forward(const float *params, const uint16_t *wbf16, const float *x, float *y, int64_t n,
-                              hipStream_t s, const uint32_t *dcount = nullptr);
+/* Inference on n normalised FP32 samples: feature k of sample i is
+ * x[i * xs + k * xk] (13 features) and output k goes to y[i * ys + k * yk]
+ * (7 outputs).  dcount (device, optional) caps n at launch time. */
+hipError_t mlp_launch_forward(const float *params, const uint16_t *wbf16, const float *x, int64_t xs, int64_t xk,
+                              float *y, int64_t ys, int64_t yk, int64_t n, hipStream_t s,
+                              const uint32_t *dcount = nullptr);
 hipError_t mlp_launch_pack(const float *params, uint16_t *wbf16, hipStream_t s);
 /* Gradient of mean((y - t)^2 over the 7 outputs and n rows) w.r.t. params;
  * partial: scratch of ceil(n / 64) * PARAM_COUNT floats; loss: 1 float. */

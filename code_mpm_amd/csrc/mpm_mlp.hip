@@ -7,8 +7,9 @@
  *   A (16 x 16, row-major M x K): lane holds A[r][4g + j], j = 0..3
  *   B (16 x 16, K x N):           lane holds B[4g + j][r]
  *   D (16 x 16 FP32):             lane holds D[4g + j][r]
- * Weights live in LDS as row-major BF16 with padded row strides; each wave
- * owns 64 sample rows and keeps its activations in LDS.
+ * Training: weights live in LDS as row-major BF16 with padded row strides;
+ * each wave owns 64 sample rows and keeps its activations in LDS.
+ * Inference: see k_mlp_infer.
  */
 #include "mpm_mlp.h"
 
@@ -160,29 +161,101 @@ __device__ __forceinline__ void wave_forward(const uint16_t *W, const float *bia
     __syncthreads();
 }
 
-/* Inference: one 64-thread block per 64-row chunk (grid-stride). */
-#define FWD_WAVES 1
-__global__ __launch_bounds__(64) void k_mlp_forward(const float *params, const uint16_t *wbf16, const float *x,
-                                                    float *y, int64_t n, const uint32_t *dcount) {
+/* Inference in the transposed form H^T = W^T X^T.  The weights are the
+ * MFMA A operand and stay in registers for the whole kernel; 16 samples form
+ * the N dimension.  A layer's FP32 output tile (D layout: lane holds
+ * H^T[4g + j][r]) is, after bias, SiLU and BF16 rounding, exactly the B
+ * operand of the next layer (lane holds B[4g + j][r]), so activations never
+ * leave registers.  Sample s, feature k of x is x[s * xs + k * xk]; of y,
+ * y[s * ys + k * yk]. */
+#define INFER_WAVES 4
+__global__ __launch_bounds__(64 * INFER_WAVES) void k_mlp_infer(const float *params, const uint16_t *wbf16,
+                                                                const float *x, int64_t xs, int64_t xk, float *y,
+                                                                int64_t ys, int64_t yk, int64_t n,
+                                                                const uint32_t *dcount) {
     if (dcount) n = min(n, (int64_t)*dcount);
-    __shared__ uint16_t W[LDS_W_TOTAL];
     __shared__ float bias[208];
-    __shared__ uint16_t lx[64 * XS];
-    __shared__ uint16_t ha[64 * AS], hb[64 * AS];
-    __shared__ float out[64 * MLP_OUT_PAD];
-    load_weights_lds(wbf16, params, W, bias);
+    for (int i = threadIdx.x; i < MLP_HID; i += blockDim.x) {
+        bias[i] = params[OFF_B1 + i];
+        bias[64 + i] = params[OFF_B2 + i];
+        bias[128 + i] = params[OFF_B3 + i];
+    }
+    for (int i = threadIdx.x; i < MLP_OUT_PAD; i += blockDim.x) bias[192 + i] = params[OFF_B4 + i];
     __syncthreads();
-    const int lane = threadIdx.x;
-    for (int64_t base = (int64_t)blockIdx.x * 64; base < n; base += (int64_t)gridDim.x * 64) {
-        int64_t row = base + lane;
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const int wave = threadIdx.x >> 6;
+    /* A fragments of W^T: tile (mt, kt) lane holds W[kt*16 + 4g + j][mt*16 + r] */
+    bf16x4 w1[4], w2[4][4], w3[4][4], w4[4];
 #pragma unroll
-        for (int k = 0; k < MLP_IN_PAD; k++)
-            lx[lane * XS + k] = (row < n && k < MLP_IN) ? f2bf(x[row * MLP_IN + k]) : (uint16_t)0;
-        __syncthreads();
-        wave_forward(W, bias, lx, nullptr, ha, nullptr, hb, nullptr, ha, out);
-        if (row < n)
-            for (int k = 0; k < MLP_OUT; k++) y[row * MLP_OUT + k] = out[lane * MLP_OUT_PAD + k];
-        __syncthreads();
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) w1[mt][j] = (short)wbf16[G_W1 + (4 * g + j) * MLP_HID + mt * 16 + r];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                w2[mt][kt][j] = (short)wbf16[G_W2 + (kt * 16 + 4 * g + j) * MLP_HID + mt * 16 + r];
+                w3[mt][kt][j] = (short)wbf16[G_W3 + (kt * 16 + 4 * g + j) * MLP_HID + mt * 16 + r];
+            }
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) w4[kt][j] = (short)wbf16[G_W4 + (kt * 16 + 4 * g + j) * MLP_OUT_PAD + r];
+    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    for (int64_t s0 = ((int64_t)blockIdx.x * INFER_WAVES + wave) * 16; s0 < n;
+         s0 += (int64_t)gridDim.x * INFER_WAVES * 16) {
+        const int64_t smp = s0 + r;
+        const bool live = smp < n;
+        /* layer 1 B operand: X^T[4g + j][r] = x(sample s0 + r, feature 4g + j) */
+        bf16x4 xb;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = 4 * g + j;
+            xb[j] = (short)f2bf((live && k < MLP_IN) ? x[smp * xs + k * xk] : 0.f);
+        }
+        bf16x4 a[4], b[4];
+        f32x4 acc[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[mt], xb, zero, 0, 0, 0);
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) a[mt][j] = (short)f2bf(silu(acc[mt][j] + bias[mt * 16 + 4 * g + j]));
+        /* layer 2 */
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) acc[mt] = zero;
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++)
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++)
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2[mt][kt], a[kt], acc[mt], 0, 0, 0);
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[mt][j] = (short)f2bf(silu(acc[mt][j] + bias[64 + mt * 16 + 4 * g + j]));
+        /* layer 3 */
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) acc[mt] = zero;
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++)
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++)
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3[mt][kt], b[kt], acc[mt], 0, 0, 0);
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) a[mt][j] = (short)f2bf(silu(acc[mt][j] + bias[128 + mt * 16 + 4 * g + j]));
+        /* layer 4, linear: lane holds Y^T[4g + j][r] */
+        f32x4 o = zero;
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++) o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w4[kt], a[kt], o, 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int f = 4 * g + j;
+            if (live && f < MLP_OUT) y[smp * ys + f * yk] = o[j] + bias[192 + f];
+        }
     }
 }
 
@@ -338,12 +411,14 @@ __global__ void k_mlp_adam(float *params, const float *grad, float *m, float *v,
     params[p] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
 }
 
-hipError_t mlp_launch_forward(const float *params, const uint16_t *wbf16, const float *x, float *y, int64_t n,
-                              hipStream_t s, const uint32_t *dcount) {
+hipError_t mlp_launch_forward(const float *params, const uint16_t *wbf16, const float *x, int64_t xs, int64_t xk,
+                              float *y, int64_t ys, int64_t yk, int64_t n, hipStream_t s, const uint32_t *dcount) {
     if (n <= 0) return hipSuccess;
-    int64_t chunks = (n + 63) / 64;
-    unsigned grid = (unsigned)(chunks < 4096 ? chunks : 4096);
-    hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(64), 0, s, params, wbf16, x, y, n, dcount);
+    int64_t tiles = (n + 15) / 16;
+    int64_t blocks = (tiles + INFER_WAVES - 1) / INFER_WAVES;
+    unsigned grid = (unsigned)(blocks < 2048 ? blocks : 2048);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(64 * INFER_WAVES), 0, s, params, wbf16, x, xs, xk, y, ys, yk, n,
+                       dcount);
     return hipGetLastError();
 }
 

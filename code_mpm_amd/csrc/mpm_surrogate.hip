@@ -152,7 +152,7 @@ int mpm_surrogate_forward(mpm_surrogate *s, const float *x, float *y, int64_t n)
     int rc = s->ensure_io(n);
     if (rc) return rc;
     if (hipMemcpy(s->io_x, x, sizeof(float) * (size_t)n * MLP_IN, hipMemcpyHostToDevice) != hipSuccess ||
-        mlp_launch_forward(s->params, s->wbf16, s->io_x, s->io_y, n, s->stream) != hipSuccess ||
+        mlp_launch_forward(s->params, s->wbf16, s->io_x, MLP_IN, 1, s->io_y, MLP_OUT, 1, n, s->stream) != hipSuccess ||
         hipStreamSynchronize(s->stream) != hipSuccess ||
         hipMemcpy(y, s->io_y, sizeof(float) * (size_t)n * MLP_OUT, hipMemcpyDeviceToHost) != hipSuccess)
         return MPM_ERR_HIP;

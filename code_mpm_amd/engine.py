@@ -247,6 +247,13 @@ class Engine:
                     "step_timed")
         return dict(zip(capi.PHASES, out.tolist()))
 
+    def step_timed_kernels(self, dt, nsteps=1):
+        """Steps with HIP-event timing per kernel; returns {slot: total ms}."""
+        out = np.zeros(len(capi.KSLOTS))
+        self._check(self._lib.mpm_step_timed_kernels(self._h, float(dt), int(nsteps),
+                                                     out.ctypes.data_as(capi.PDBL)), "step_timed_kernels")
+        return dict(zip(capi.KSLOTS, out.tolist()))
+
     def run(self, dt_total):
         fin = ctypes.c_int32()
         self._check(self._lib.mpm_run(self._h, float(dt_total), ctypes.byref(fin)), "run")

@@ -420,18 +420,29 @@ def graded_foam(seed, h=1.0e-4, n_target=None, E=10.0e9, nu=0.3, rho=1000.0, yie
     lo = np.array([margin, 1, margin]) * h
     hi = lo + body * h
     keep = np.ones(x.shape[1], bool)
-    # voids: add until the porosity is reached; centre density linear in y (1 at the bottom, 3 at the top)
+    # voids: add until the porosity is reached; centre density linear in y (1 at the bottom, 3 at the top).
+    # Particles come 8 per cell, cells x fastest: a void only tests the cells of its bounding box.
     target_removed = porosity * x.shape[1]
+    removed = 0
+    sub8 = np.arange(8)
     for _ in range(100000):
-        if (~keep).sum() >= target_removed:
+        if removed >= target_removed:
             break
         r = np.exp(rng.normal(np.log(3.0 * h), 0.3))
         while True:
             c = lo + rng.uniform(0, 1, 3) * (hi - lo)
             if rng.uniform() < (1.0 + 2.0 * (c[1] - lo[1]) / (hi[1] - lo[1])) / 3.0:
                 break
-        d2 = np.sum((x - c[:, None]) ** 2, 0)
-        keep &= d2 > r * r
+        a0 = np.clip(np.floor((c - r - lo) / h).astype(int) - 1, 0, body - 1)
+        a1 = np.clip(np.floor((c + r - lo) / h).astype(int) + 1, 0, body - 1)
+        bz, by, bx = np.meshgrid(np.arange(a0[2], a1[2] + 1), np.arange(a0[1], a1[1] + 1),
+                                 np.arange(a0[0], a1[0] + 1), indexing="ij")
+        cj = (bx + body[0] * (by + body[1] * bz)).ravel()
+        cand = (8 * cj[:, None] + sub8[None, :]).ravel()
+        d2 = np.sum((x[:, cand] - c[:, None]) ** 2, 0)
+        hit = cand[(d2 <= r * r) & keep[cand]]
+        keep[hit] = False
+        removed += len(hit)
     x = np.ascontiguousarray(x[:, keep])
     n_body = x.shape[1]
     # platen: one cell layer right above the body

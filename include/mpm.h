@@ -258,12 +258,11 @@ int mpm_get_variant(const mpm_engine *e, int32_t *variant);
  * per workgroup (default 8 x 8 x 4). */
 int mpm_set_tile_shape(mpm_engine *e, const int32_t shape[3]);
 int mpm_get_tile_shape(const mpm_engine *e, int32_t shape[3]);
-/* Performs one step during which the two node phases are launched `reps`
- * times under each candidate tile shape (they read particle and bin state
- * and overwrite node state, so repeating them changes nothing) and keeps
- * the shape with the least total time; the step's results equal an
- * ordinary mpm_step of dt_max.  Returns the chosen shape and its node-phase
- * milliseconds. */
+/* Performs one step during which the particle-to-grid phase is launched
+ * `reps` times under each candidate tile shape (it reads particle and bin
+ * state and overwrites node state, so repeating it changes nothing) and
+ * keeps the fastest shape; the step's results equal an ordinary mpm_step of
+ * dt_max.  Returns the chosen shape and its particle-to-grid milliseconds. */
 int mpm_autotune(mpm_engine *e, int32_t reps, int32_t shape[3], double *node_ms);
 
 /* --- the hot path ---------------------------------------------------------- */
@@ -300,6 +299,22 @@ enum {
  * PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:487-790).
  * Synchronises after every step. */
 int mpm_step_timed(mpm_engine *e, double dt, int32_t nsteps, double phase_ms[MPM_PHASE_COUNT]);
+/* The same with one time per kernel (HIP events on the engine's stream),
+ * summed over nsteps, in these slots.  A phase that runs one kernel in the
+ * particle order (P2G, remap) books it in its first slot; with a surrogate
+ * set, its network and post kernels count in the constitutive slot. */
+enum {
+    MPM_KSLOT_BIN = 0,                /* scan, scatter, per-cell sort */
+    MPM_KSLOT_P2G_PARTIALS = 1,       /* cell partials (cell order) or the node gather (particle order) */
+    MPM_KSLOT_P2G_NODES = 2,          /* node sums + grid update (cell order) */
+    MPM_KSLOT_G2P_VELOCITY = 3,
+    MPM_KSLOT_REMAP_PARTIALS = 4,
+    MPM_KSLOT_REMAP_NODES = 5,
+    MPM_KSLOT_G2P_KINEMATICS = 6,     /* x, L, strain increment, F, V, control schedule, next cell */
+    MPM_KSLOT_G2P_CONSTITUTIVE = 7,   /* return map, stress, energies */
+    MPM_KSLOT_COUNT = 8
+};
+int mpm_step_timed_kernels(mpm_engine *e, double dt, int32_t nsteps, double kernel_ms[MPM_KSLOT_COUNT]);
 /* Waits for the engine's stream. */
 int mpm_synchronize(mpm_engine *e);
 
