@@ -29,7 +29,10 @@ PEAK_HBM_GBS = 8000.0
 #   p2g_nodes: per active node writes m p3 f3 (7 words, the P2G flush) and
 #     reads fixed/friction and writes p3 f3 f/m3 reaction (13, grid update)
 #   g2p_velocity: reads x3 v3, writes v3; per active node reads f/m3 m
-#   remap_partials: reads x3 v3 m (7 words)
+#   remap_partials: reads x3 v3 m (7 words); in the cell order G2P pass 1
+#     runs inside it on each staged particle, which adds the v3 write (10
+#     words) and per active node the f/m3 m reads (4 words), and the
+#     g2p_velocity slot launches nothing
 #   remap_nodes: per active node reads m p3, writes v3 p/m3 (10 words)
 #   g2p_kinematics: reads x3 F9 V0, writes x3 F9 V (26 words) + material id
 #     and control flags (5 B); per active node reads p/m3 v3 m (7 words)
@@ -38,17 +41,18 @@ PER_PARTICLE = {"p2g_partials": 14 * 8, "p2g_nodes": 0, "g2p_velocity": 9 * 8, "
                 "remap_nodes": 0, "g2p_kinematics": 26 * 8 + 5, "g2p_constitutive": 46 * 8, "bin": 0}
 PER_NODE = {"p2g_partials": 0, "p2g_nodes": 20 * 8, "g2p_velocity": 4 * 8, "remap_partials": 0,
             "remap_nodes": 10 * 8, "g2p_kinematics": 7 * 8, "g2p_constitutive": 0, "bin": 0}
+FUSED_CELL = {"remap_partials": (10 * 8, 4 * 8)}
 # kernel of each slot, per summation order (names as rocprofv3 reports them)
 KERNELS = {
     "cell": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
-             "p2g_partials": "k_cell_partials<false>", "p2g_nodes": "k_node_cells<false>",
-             "g2p_velocity": "k_g2p_velocity", "remap_partials": "k_cell_partials<true>",
-             "remap_nodes": "k_node_cells<true>", "g2p_kinematics": "k_g2p_kinematics<0>",
+             "p2g_partials": "k_cell_partials<false, false, false>", "p2g_nodes": "k_node_cells<false, false>",
+             "g2p_velocity": None, "remap_partials": "k_cell_partials<true, true, false>",
+             "remap_nodes": "k_node_cells<true, false>", "g2p_kinematics": "k_g2p_kinematics<0, false>",
              "g2p_constitutive": "k_g2p_constitutive<0, 0>"},
     "particle": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
-                 "p2g_partials": "k_p2g_grid", "p2g_nodes": None, "g2p_velocity": "k_g2p_velocity",
-                 "remap_partials": "k_remap_grid", "remap_nodes": None, "g2p_kinematics": "k_g2p_kinematics<0>",
-                 "g2p_constitutive": "k_g2p_constitutive<0, 0>"},
+                 "p2g_partials": "k_p2g_grid", "p2g_nodes": None, "g2p_velocity": "k_g2p_velocity<false>",
+                 "remap_partials": "k_remap_grid", "remap_nodes": None,
+                 "g2p_kinematics": "k_g2p_kinematics<0, false>", "g2p_constitutive": "k_g2p_constitutive<0, 0>"},
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_honeycomb8m.txt")
 
@@ -228,7 +232,10 @@ def main():
     for k, ms in avg_ms.items():
         if kernels[k] is None:
             continue
-        b = PER_PARTICLE[k] * scene.n + PER_NODE[k] * active
+        per_p, per_n = PER_PARTICLE[k], PER_NODE[k]
+        if args.order == "cell" and k in FUSED_CELL:
+            per_p, per_n = FUSED_CELL[k]
+        b = per_p * scene.n + per_n * active
         t = [measured.get(kn) for kn in kernels[k].split("+")]
         per_kernel[k] = dict(ms=round(ms, 4), kernel=kernels[k], algorithmic_bytes=b,
                              gbs=round(b / (ms * 1e-3) / 1e9, 1) if (b and ms > 0) else None,
@@ -240,7 +247,8 @@ def main():
     dom_bytes = per_kernel[dominant]["algorithmic_bytes"]
     achieved = dom_bytes / (avg_ms[dominant] * 1e-3) / 1e9
     traffic = per_kernel[dominant]["traffic"]
-    step_bytes = 768 * scene.n + 272 * active
+    # the whole step's algorithmic bytes: the sum over its kernels
+    step_bytes = sum(v["algorithmic_bytes"] for v in per_kernel.values())
     for k, v in per_kernel.items():
         print(f"# {k:13s} {v['ms']:8.3f} ms  {v['gbs']} GB/s  traffic {v['traffic']}", file=sys.stderr)
 

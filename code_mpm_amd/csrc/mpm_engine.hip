@@ -117,6 +117,8 @@ struct mpm_engine {
     /* node summation order and the cell-order partial sums */
     int order = ORDER_CELL;
     int variant = VAR_DP;
+    uint8_t *dtflag = nullptr; /* node tile flags, see Batch::tflag */
+    int64_t tflag_cap = 0;
     CellParts parts{};
     int64_t parts_cap = 0;
 
@@ -165,6 +167,8 @@ struct mpm_engine {
         graphs.clear();
     }
     void free_nodes() {
+        dfree(dtflag);
+        tflag_cap = 0;
         dfree(nm); dfree(np); dfree(nv); dfree(nf); dfree(na); dfree(nu); dfree(nfty); dfree(nfric);
         dfree(nact); dfree(nfix);
         fixed_y.clear();
@@ -304,6 +308,7 @@ struct mpm_engine {
         b.variant = variant;
         b.fext = has_fext ? 1 : 0;
         b.nmat = mt.n;
+        b.tflag = order == ORDER_CELL ? dtflag : nullptr;
         return b;
     }
 
@@ -327,6 +332,14 @@ struct mpm_engine {
             if ((int64_t)ntiles + I.tiles >= (int64_t)0xFFFFFFFFll) return fail(MPM_ERR_RANGE, "too many node tiles");
             ntiles += I.tiles;
         }
+        if ((int64_t)ntiles > tflag_cap) {
+            dfree(dtflag);
+            tflag_cap = 0;
+            int rc = hip(hipMalloc(&dtflag, std::max<size_t>(ntiles, 1)), "alloc tile flags");
+            if (!rc) rc = hip(hipMemset(dtflag, 0, std::max<size_t>(ntiles, 1)), "clear tile flags");
+            if (rc) return rc;
+            tflag_cap = ntiles;
+        }
         std::vector<uint32_t> to(inst.size() + 1);
         for (size_t k = 0; k < inst.size(); k++) to[k] = inst[k].tile_off;
         to[inst.size()] = ntiles;
@@ -347,7 +360,7 @@ struct mpm_engine {
             nnodes += I.g.nnodes;
             ncells += I.g.necells;
         }
-        if (ncells >= (int64_t)0xFFFFFFFFll) return fail(MPM_ERR_RANGE, "grid too large: %lld cells", (long long)ncells);
+        if (scan_padded_cells(ncells) >= (int64_t)0xFFFFFFFFll) return fail(MPM_ERR_RANGE, "grid too large: %lld cells", (long long)ncells);
         const size_t nn = (size_t)nnodes;
         int rc = 0;
         if ((rc = hip(hipMalloc(&nm, 8 * nn), "alloc node mass")) ||
@@ -360,10 +373,10 @@ struct mpm_engine {
             (rc = hip(hipMalloc(&nfric, 8 * nn), "alloc node friction")) ||
             (rc = hip(hipMalloc(&nact, nn), "alloc node active")) ||
             (rc = hip(hipMalloc(&nfix, nn), "alloc node fixed")) ||
-            (rc = hip(hipMalloc(&bcount, 4 * (size_t)ncells), "alloc cell counts")) ||
-            (rc = hip(hipMalloc(&bstart, 4 * ((size_t)ncells + 1)), "alloc cell starts")) ||
-            (rc = hip(hipMalloc(&bblock, 8 * (size_t)scan_block_count(ncells)), "alloc scan blocks")) ||
-            (rc = hip(hipMalloc(&bslot, 4 * (size_t)ncells), "alloc cell slots")) ||
+            (rc = hip(hipMalloc(&bcount, 4 * (size_t)scan_padded_cells(ncells)), "alloc cell counts")) ||
+            (rc = hip(hipMalloc(&bstart, 4 * ((size_t)scan_padded_cells(ncells) + 1)), "alloc cell starts")) ||
+            (rc = hip(hipMalloc(&bblock, 8 * (size_t)scan_block_count(scan_padded_cells(ncells))), "alloc scan blocks")) ||
+            (rc = hip(hipMalloc(&bslot, 4 * (size_t)scan_padded_cells(ncells)), "alloc cell slots")) ||
             (rc = hip(hipMalloc(&bnocc, 4), "alloc occupied count")) ||
             (rc = hip(hipMalloc(&dinst, sizeof(Inst) * inst.size()), "alloc instance table")) ||
             (rc = hip(hipMalloc(&dtile_off, 4 * (inst.size() + 1)), "alloc tile offsets"))) {
@@ -422,7 +435,7 @@ struct mpm_engine {
     /* Cell keys and counts for the current positions (after an upload). */
     int ensure_keys() {
         if (keys_valid) return MPM_OK;
-        int rc = hip(hipMemsetAsync(bcount, 0, sizeof(uint32_t) * ncells, stream), "clear cell counts");
+        int rc = hip(hipMemsetAsync(bcount, 0, sizeof(uint32_t) * scan_padded_cells(ncells), stream), "clear cell counts");
         if (rc) return rc;
         rc = hip(launch_keys(batch(), pview(), bview(), stream), "cell keys");
         if (rc) return rc;
@@ -547,6 +560,9 @@ struct mpm_engine {
         for (int k = 1; k < kNumTileShapes; k++)
             if (t_p2g[k] < t_p2g[best]) best = k;
         if (!rc) rc = set_tile(kTileShapes[best]);
+        /* once more with the chosen shape: the tile flags follow its numbering */
+        double once = 0.0;
+        if (!rc) rc = time_phase(PHASE_P2G_GRID, 1, sa, e0, e1, once);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (rc) return rc;

@@ -126,6 +126,9 @@ struct Batch {
     int variant;              /* VAR_* */
     int fext;                 /* any particle carries an external force */
     int nmat;                 /* materials in the table */
+    uint8_t *tflag;           /* per node tile: 1 when a corner of an occupied cell lies in it
+                                 (set by the P2G partials, cleared by the remap node pass);
+                                 null in the particle order */
 };
 
 struct MonLists {
@@ -137,6 +140,8 @@ struct MonLists {
 
 hipError_t launch_keys(const Batch &bt, const PView &P, const BView &B, hipStream_t s);
 int64_t scan_block_count(int64_t necells);
+/* Cells the count, slot (and, plus one, start) arrays hold: whole scan tiles. */
+int64_t scan_padded_cells(int64_t necells);
 hipError_t launch_bin(const PView &P, const BView &B, uint64_t *block_sum, hipStream_t s);
 /* Step phases after binning, in launch order (PHASE_BIN is launch_bin). */
 enum { PHASE_BIN = 0, PHASE_P2G_GRID = 1, PHASE_G2P_VELOCITY = 2, PHASE_REMAP_GRID = 3, PHASE_G2P_STRAIN = 4,

@@ -57,6 +57,19 @@ __device__ __forceinline__ int inst_of_tile(const Batch &bt, uint32_t t) {
     return lo;
 }
 
+/* Instance lookups for the heavy kernels, specialised at launch on whether
+ * the batch has more than one instance.  With one, the instance is the
+ * constant 0: its table row is read through scalar loads into SGPRs instead
+ * of occupying vector registers of every lane. */
+template <bool MULTI>
+__device__ __forceinline__ const Inst &inst_p(const Batch &bt, uint32_t i) {
+    return MULTI ? bt.inst[bt.pinst[i]] : bt.inst[0];
+}
+template <bool MULTI>
+__device__ __forceinline__ const Inst &inst_t(const Batch &bt, uint32_t t) {
+    return MULTI ? bt.inst[inst_of_tile(bt, t)] : bt.inst[0];
+}
+
 /* TimeLine::getVelocity (TimeLine.h:33-46): the last interval [t_i, t_i+1)
  * containing the time wins; zero outside every interval. */
 __device__ __forceinline__ void timeline_velocity(const double *tl, int off, int count, double time, double out[3]) {
@@ -104,19 +117,35 @@ __device__ __forceinline__ uint32_t cell_key(const GridGeom &g, double x, double
     return (uint32_t)c[0] + (uint32_t)g.ec[0] * ((uint32_t)c[1] + (uint32_t)g.ec[1] * (uint32_t)c[2]);
 }
 
-/* count[key] += 1 for every lane with want set, one atomic per distinct
- * key in the wave (neighbouring particles mostly share a cell).  Every lane
- * of the wave must call it. */
-__device__ __forceinline__ void wave_key_add(uint32_t *count, uint32_t key, bool want) {
-    uint64_t pending = __ballot(want);
+/* Runs of equal keys over adjacent lanes of a wave: each lane's run head
+ * (first lane) and run length, from one ballot.  Neighbouring particle ids
+ * mostly share a cell, so a wave holds few runs and their heads can issue
+ * one atomic each in a single wave-wide instruction.  A key that shows up in
+ * two separate runs gets two atomics, which is still exact.  Every lane of
+ * the wave must call it. */
+struct KeyRun {
+    int head;
+    uint32_t len;
+};
+
+__device__ __forceinline__ KeyRun wave_key_run(uint32_t key) {
     const int lane = (int)(threadIdx.x & 63);
-    while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const uint32_t lk = __shfl(key, leader, 64);
-        const uint64_t same = __ballot(want && key == lk);
-        if (lane == leader) atomicAdd(&count[lk], (uint32_t)__popcll(same));
-        pending &= ~same;
-    }
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const uint64_t starts = __ballot(lane == 0 || prev != key);
+    const uint64_t upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    const uint64_t above = starts & ~upto;
+    KeyRun r;
+    r.head = 63 - __clzll((long long)(starts & upto)); /* bit 0 is always set */
+    r.len = (uint32_t)((above ? __ffsll((unsigned long long)above) - 1 : 64) - r.head);
+    return r;
+}
+
+/* count[key] += 1 for every lane with want set, one atomic per run of equal
+ * keys.  Every lane of the wave must call it. */
+__device__ __forceinline__ void wave_key_add(uint32_t *count, uint32_t key, bool want) {
+    const uint32_t k = want ? key : MPM_NONE;
+    const KeyRun r = wave_key_run(k);
+    if (want && (int)(threadIdx.x & 63) == r.head) atomicAdd(&count[k], r.len);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_keys(Batch bt, PView P, BView B) {
@@ -160,13 +189,26 @@ __device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t *l
     return off + x - v;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_scan_reduce(const uint32_t *count, int64_t n, uint64_t *block_sum) {
+/* The count, start and slot arrays are padded to whole tiles (zero counts
+ * past the last cell), so every load and store below is an unconditional
+ * 16-byte access.  A tile is SCAN_SUB sub-tiles of BLOCK x 4 cells; thread t
+ * holds cells 4t..4t+3 of each sub-tile, so each of the four loads of a wave
+ * reads one contiguous KiB. */
+#define SCAN_SUB (SCAN_ITEMS / 4)
+
+__device__ __forceinline__ uint64_t pack_sum4(const uint4 &c) {
+    return pack_count(c.x) + pack_count(c.y) + pack_count(c.z) + pack_count(c.w);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_reduce(const uint32_t *count, uint64_t *block_sum) {
     __shared__ uint64_t lds[4];
-    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    const uint4 *c4 = reinterpret_cast<const uint4 *>(count + (int64_t)blockIdx.x * SCAN_TILE) + threadIdx.x;
+    uint4 c[SCAN_SUB];
+#pragma unroll
+    for (int q = 0; q < SCAN_SUB; q++) c[q] = c4[q * BLOCK];
     uint64_t s = 0;
 #pragma unroll
-    for (int j = 0; j < SCAN_ITEMS; j++)
-        if (base + j < n) s += pack_count(count[base + j]);
+    for (int q = 0; q < SCAN_SUB; q++) s += pack_sum4(c[q]);
     uint64_t total;
     block_exclusive_scan(s, lds, total);
     if (threadIdx.x == 0) block_sum[blockIdx.x] = total;
@@ -186,32 +228,74 @@ __global__ __launch_bounds__(BLOCK) void k_scan_blocks(uint64_t *block_sum, int6
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_scan_apply(const uint32_t *count, int64_t n, const uint64_t *block_sum,
+__device__ __forceinline__ void scan_emit(uint32_t c, int64_t cell, uint64_t &ex, uint32_t &st, uint32_t &sl,
+                                          uint32_t *occ) {
+    st = (uint32_t)ex;
+    const uint32_t r = (uint32_t)(ex >> 32);
+    sl = c ? r : MPM_NONE;
+    if (c) occ[r] = (uint32_t)cell;
+    ex += pack_count(c);
+}
+
+/* Cell starts, slots and the occupied-cell list.  The last thread of the
+ * last block also writes start[padded] and the occupied count, which are
+ * the grand totals (start[necells] equals them too, padding cells being
+ * empty). */
+__global__ __launch_bounds__(BLOCK) void k_scan_apply(const uint32_t *count, const uint64_t *block_sum,
                                                      uint32_t *start, uint32_t *slot, uint32_t *occ, uint32_t *nocc) {
-    __shared__ uint64_t lds[4];
-    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-    uint32_t v[SCAN_ITEMS];
-    uint64_t s = 0;
+    __shared__ uint64_t lds[SCAN_SUB][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t tile = (int64_t)blockIdx.x * SCAN_TILE;
+    const uint4 *c4 = reinterpret_cast<const uint4 *>(count + tile) + threadIdx.x;
+    uint4 c[SCAN_SUB];
 #pragma unroll
-    for (int j = 0; j < SCAN_ITEMS; j++) {
-        v[j] = (base + j < n) ? count[base + j] : 0u;
-        s += pack_count(v[j]);
+    for (int q = 0; q < SCAN_SUB; q++) c[q] = c4[q * BLOCK];
+    uint64_t s[SCAN_SUB], x[SCAN_SUB];
+#pragma unroll
+    for (int q = 0; q < SCAN_SUB; q++) {
+        s[q] = pack_sum4(c[q]);
+        x[q] = s[q];
     }
-    uint64_t total;
-    uint64_t ex = block_exclusive_scan(s, lds, total) + block_sum[blockIdx.x];
+    /* wave-level inclusive scans of the SCAN_SUB sub-tiles side by side */
 #pragma unroll
-    for (int j = 0; j < SCAN_ITEMS; j++) {
-        if (base + j < n) {
-            start[base + j] = (uint32_t)ex;
-            uint32_t r = (uint32_t)(ex >> 32);
-            slot[base + j] = v[j] ? r : MPM_NONE;
-            if (v[j]) occ[r] = (uint32_t)(base + j);
+    for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+        for (int q = 0; q < SCAN_SUB; q++) {
+            const uint64_t y = __shfl_up(x[q], o, 64);
+            if (lane >= o) x[q] += y;
         }
-        ex += pack_count(v[j]);
     }
-    if (base + SCAN_ITEMS >= n && base < n) { /* the thread holding the last cell */
-        start[n] = (uint32_t)ex;
-        *nocc = (uint32_t)(ex >> 32);
+    if (lane == 63) {
+#pragma unroll
+        for (int q = 0; q < SCAN_SUB; q++) lds[q][wave] = x[q];
+    }
+    __syncthreads();
+    uint64_t off = block_sum[blockIdx.x];
+    uint4 *st4 = reinterpret_cast<uint4 *>(start + tile) + threadIdx.x;
+    uint4 *sl4 = reinterpret_cast<uint4 *>(slot + tile) + threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < SCAN_SUB; q++) {
+        uint64_t before = 0, sub = 0;
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            const uint64_t t = lds[q][w];
+            before += w < wave ? t : 0u;
+            sub += t;
+        }
+        uint64_t ex = off + before + x[q] - s[q];
+        const int64_t cell = tile + (int64_t)q * BLOCK * 4 + (int64_t)threadIdx.x * 4;
+        uint4 st, sl;
+        scan_emit(c[q].x, cell + 0, ex, st.x, sl.x, occ);
+        scan_emit(c[q].y, cell + 1, ex, st.y, sl.y, occ);
+        scan_emit(c[q].z, cell + 2, ex, st.z, sl.z, occ);
+        scan_emit(c[q].w, cell + 3, ex, st.w, sl.w, occ);
+        st4[q * BLOCK] = st;
+        sl4[q * BLOCK] = sl;
+        off += sub;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == BLOCK - 1) {
+        start[tile + SCAN_TILE] = (uint32_t)off;
+        *nocc = (uint32_t)(off >> 32);
     }
 }
 
@@ -230,32 +314,22 @@ __device__ __forceinline__ void sort_cell(const BView &B, uint32_t c) {
 }
 
 
-/* Particle ids into their cell's range of the sorted list.  The lanes of a
- * wave that share a cell take consecutive slots in id order with one atomic
- * decrement of the cell's count per wave (which also leaves the counts at
- * zero for the next step); k_cell_sort orders what different waves put in
- * one cell. */
+/* Particle ids into their cell's range of the sorted list.  Each run of
+ * equal keys in a wave takes consecutive slots in id order with one atomic
+ * decrement of the cell's count (which also leaves the counts at zero for
+ * the next step); k_cell_sort orders what different runs put in one cell. */
 __global__ __launch_bounds__(BLOCK) void k_scatter(PView P, BView B) {
     uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     const bool live = i < P.n;
     const uint32_t k = live ? B.key[i] : MPM_NONE;
     const bool want = k != MPM_NONE;
     const int lane = (int)(threadIdx.x & 63);
-    uint64_t pending = __ballot(want);
-    uint32_t slot = 0;
-    while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const uint32_t lk = __shfl(k, leader, 64);
-        const uint64_t same = __ballot(want && k == lk);
-        const uint32_t cnt = (uint32_t)__popcll(same);
-        uint32_t top = 0;
-        if (lane == leader) top = atomicSub(&B.count[lk], cnt);
-        top = __shfl(top, leader, 64);
-        if ((same >> lane) & 1ull) slot = B.start[lk] + top - cnt + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
-        pending &= ~same;
-    }
+    const KeyRun r = wave_key_run(k);
+    uint32_t top = 0;
+    if (want && lane == r.head) top = atomicSub(&B.count[k], r.len);
+    top = __shfl(top, r.head, 64);
     if (!want) return;
-    B.sorted[slot] = i;
+    B.sorted[B.start[k] + top - r.len + (uint32_t)(lane - r.head)] = i;
     if (P.dc_rank[i] >= 0) atomicOr(&B.slot[k], SLOT_DC); /* the cell holds a controlled particle */
 }
 
@@ -313,6 +387,12 @@ __device__ __forceinline__ void node_of_tile(const Batch &bt, const GridGeom &g,
     iz = tz * bt.tz + (int)(threadIdx.x / (TX * TY));
 }
 
+/* Inverse of node_of_tile: the instance-local tile holding node (ix, iy, iz). */
+__device__ __forceinline__ uint32_t tile_of_node(const Batch &bt, const GridGeom &g, int ix, int iy, int iz) {
+    const uint32_t tnx = (uint32_t)((g.nn[0] + bt.tx - 1) / bt.tx), tny = (uint32_t)((g.nn[1] + bt.ty - 1) / bt.ty);
+    return (uint32_t)(ix / bt.tx) + tnx * ((uint32_t)(iy / bt.ty) + tny * (uint32_t)(iz / bt.tz));
+}
+
 /* Track, per axis, the displacement-controlled particle latest in the
  * control list among those touching the node (the last writer of
  * runSimulation_DisplacementControl, DP:143-174, when run in list order). */
@@ -352,9 +432,48 @@ __device__ __forceinline__ void store_g2(const NView &Nd, int64_t node, double m
     o[3] = make_double2(v[2], 0.0);
 }
 
+/* What the node updates read besides the sums: instance scalars, the
+ * node's fixed mask and friction (P2G), its mass and momentum (remap) and,
+ * for the single pass, its force sums.  Callers read it before their first
+ * store, with the node's cell slots, so that every load is in flight at
+ * once (a load after a store cannot be proven unaliased and waits on its
+ * own). */
+struct NodeIn {
+    double dt, damping, mass_min;
+    double friction;
+    double m, p[3], f[3];
+    uint8_t fixed;
+};
+
+template <bool REMAP>
+__device__ __forceinline__ NodeIn node_in(const Batch &bt, const Inst &I, const NView &Nd, int64_t node) {
+    const size_t nn = (size_t)bt.nnodes;
+    NodeIn r;
+    r.dt = I.dt;
+    r.damping = I.damping;
+    r.mass_min = I.mass_min;
+    r.fixed = Nd.fixed[node];
+    r.friction = 0.0;
+    r.m = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) r.p[a] = r.f[a] = 0.0;
+    if (!REMAP) {
+        if (bt.variant == VAR_DP) r.friction = Nd.friction[node];
+    } else {
+        r.m = Nd.m[node];
+#pragma unroll
+        for (int a = 0; a < 3; a++) r.p[a] = Nd.p[a * nn + node];
+        if (bt.variant == VAR_SP) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) r.f[a] = Nd.f[a * nn + node];
+        }
+    }
+    return r;
+}
+
 /* DC override then runSimulation_IntegrateGrid (DP:90-142) on one node's
  * sums, in registers; stores m, p, f, f/m and the reaction. */
-__device__ __forceinline__ void node_integrate(const Inst &I, const NView &Nd, int64_t node, size_t nn, double m,
+__device__ __forceinline__ void node_integrate(const NodeIn &in, const NView &Nd, int64_t node, size_t nn, double m,
                                                double p[3], double f[3], const DCWin &w) {
     double v[3] = {0.0, 0.0, 0.0};
     /* runSimulation_DisplacementControl, DP:143-174 */
@@ -366,24 +485,24 @@ __device__ __forceinline__ void node_integrate(const Inst &I, const NView &Nd, i
             f[a] = 0.0;
         }
     /* runSimulation_IntegrateGrid, DP:90-142 */
-    const double dt = I.dt;
+    const double dt = in.dt;
     {
         double pp = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
         if (sqrt(pp) > 1.0e-24) {
             double inv = 1.0 / sqrt(pp);
             double nx = p[0] * inv, ny = p[1] * inv, nz = p[2] * inv;
-            double sc = I.damping * fabs(f[0] * nx + f[1] * ny + f[2] * nz);
+            double sc = in.damping * fabs(f[0] * nx + f[1] * ny + f[2] * nz);
             f[0] -= sc * nx;
             f[1] -= sc * ny;
             f[2] -= sc * nz;
         }
     }
-    const uint8_t fixed = Nd.fixed[node];
+    const uint8_t fixed = in.fixed;
     double fty = 0.0;
     if (fixed & 1) {
         double pty = p[1] + f[1] * dt;
         if (fabs(pty * dt) > 1.0e-24 && f[0] > 0.0) {
-            double ffy = Nd.friction[node] * glm_sign(-pty) * fabs(f[0]);
+            double ffy = in.friction * glm_sign(-pty) * fabs(f[0]);
             if (fabs(ffy * dt) > fabs(pty)) ffy *= fabs(pty / dt) / fabs(ffy);
             f[1] += ffy;
         }
@@ -420,15 +539,15 @@ __device__ __forceinline__ void node_integrate(const Inst &I, const NView &Nd, i
  * damping along p (as DP:100-101), p += f dt, the fixed components zeroed
  * with the y reaction taken first (no friction), then displacement control
  * on all axes.  Stores m, p, f, f/m and the reaction. */
-__device__ __forceinline__ void node_integrate_contact(const Inst &I, const NView &Nd, int64_t node, size_t nn,
+__device__ __forceinline__ void node_integrate_contact(const NodeIn &in, const NView &Nd, int64_t node, size_t nn,
                                                        double m, double p[3], double f[3], const DCWin &w) {
-    const double dt = I.dt;
+    const double dt = in.dt;
     {
         double pp = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
         if (sqrt(pp) > 1.0e-24) {
             double inv = 1.0 / sqrt(pp);
             double nx = p[0] * inv, ny = p[1] * inv, nz = p[2] * inv;
-            double sc = I.damping * fabs(f[0] * nx + f[1] * ny + f[2] * nz);
+            double sc = in.damping * fabs(f[0] * nx + f[1] * ny + f[2] * nz);
             f[0] -= sc * nx;
             f[1] -= sc * ny;
             f[2] -= sc * nz;
@@ -436,7 +555,7 @@ __device__ __forceinline__ void node_integrate_contact(const Inst &I, const NVie
     }
 #pragma unroll
     for (int a = 0; a < 3; a++) p[a] += f[a] * dt;
-    const uint8_t fixed = Nd.fixed[node];
+    const uint8_t fixed = in.fixed;
     double fty = 0.0;
     if (fixed & 1) {
         p[0] = 0.0;
@@ -470,9 +589,9 @@ __device__ __forceinline__ void node_integrate_contact(const Inst &I, const NVie
 
 /* Single pass, first half of the grid work (SP:108-204): the node keeps
  * the sums; momentum only where the mass exceeds the minimum (SP:170-174). */
-__device__ __forceinline__ void node_store_singlepass(const Inst &I, const NView &Nd, int64_t node, size_t nn,
+__device__ __forceinline__ void node_store_singlepass(const NodeIn &in, const NView &Nd, int64_t node, size_t nn,
                                                       double m, double p[3], double f[3]) {
-    const bool massive = m > I.mass_min;
+    const bool massive = m > in.mass_min;
     Nd.active[node] = 1;
     Nd.m[node] = m;
 #pragma unroll
@@ -487,30 +606,30 @@ __device__ __forceinline__ void node_store_singlepass(const Inst &I, const NView
  * fixed components zeroed with the y reaction taken first, then
  * displacement control on all axes.  Reads the sums stored by
  * node_store_singlepass; stores v, p, f, f/m, p/m and the reaction. */
-__device__ __forceinline__ void node_update_singlepass(const Inst &I, const NView &Nd, int64_t node, size_t nn,
+__device__ __forceinline__ void node_update_singlepass(const NodeIn &in, const NView &Nd, int64_t node, size_t nn,
                                                        double m, double v[3], const DCWin &w) {
-    const double dt = I.dt;
-    double p[3] = {Nd.p[node], Nd.p[nn + node], Nd.p[2 * nn + node]};
-    double f[3] = {Nd.f[node], Nd.f[nn + node], Nd.f[2 * nn + node]};
+    const double dt = in.dt;
+    double p[3] = {in.p[0], in.p[1], in.p[2]};
+    double f[3] = {in.f[0], in.f[1], in.f[2]};
     {
         double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
         if (sqrt(vv) > 1.0e-9) {
             double lf = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
             double inv = 1.0 / sqrt(vv);
             double nx = v[0] * inv, ny = v[1] * inv, nz = v[2] * inv;
-            double sc = I.damping * lf;
+            double sc = in.damping * lf;
             f[0] -= sc * nx;
             f[1] -= sc * ny;
             f[2] -= sc * nz;
         }
     }
-    if (m > I.mass_min) {
+    if (m > in.mass_min) {
 #pragma unroll
         for (int a = 0; a < 3; a++) v[a] += f[a] / m * dt;
     }
 #pragma unroll
     for (int a = 0; a < 3; a++) p[a] += f[a] * dt;
-    const uint8_t fixed = Nd.fixed[node];
+    const uint8_t fixed = in.fixed;
     double fty = 0.0;
     if (fixed & 1) {
         v[0] = 0.0;
@@ -547,15 +666,15 @@ __device__ __forceinline__ void node_update_singlepass(const Inst &I, const NVie
 }
 
 /* The node side of the particle-to-grid phase for each loop. */
-__device__ __forceinline__ void node_p2g_finish(const Batch &bt, const Inst &I, const NView &Nd, int64_t node,
+__device__ __forceinline__ void node_p2g_finish(const Batch &bt, const NodeIn &in, const NView &Nd, int64_t node,
                                                 double m, double p[3], double f[3], const DCWin &w) {
     const size_t nn = (size_t)bt.nnodes;
     if (bt.variant == VAR_DPC)
-        node_integrate_contact(I, Nd, node, nn, m, p, f, w);
+        node_integrate_contact(in, Nd, node, nn, m, p, f, w);
     else if (bt.variant == VAR_SP)
-        node_store_singlepass(I, Nd, node, nn, m, p, f);
+        node_store_singlepass(in, Nd, node, nn, m, p, f);
     else
-        node_integrate(I, Nd, node, nn, m, p, f, w);
+        node_integrate(in, Nd, node, nn, m, p, f, w);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd, BView B) {
@@ -569,6 +688,7 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd,
     const size_t n = P.n;
     const double xn[3] = {node_coord(ix, g.L[0], g.cells[0]), node_coord(iy, g.L[1], g.cells[1]),
                           node_coord(iz, g.L[2], g.cells[2])};
+    const NodeIn in = node_in<false>(bt, I, Nd, node);
     Merge8 M;
     merge_init(M, B, g, I.cell_off, ix, iy, iz);
     uint32_t id = merge_min(M);
@@ -615,13 +735,14 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd,
         merge_advance(M, B, id);
         id = merge_min(M);
     }
-    node_p2g_finish(bt, I, Nd, node, m, p, f, w);
+    node_p2g_finish(bt, in, Nd, node, m, p, f, w);
 }
 
 /* G2P pass 1 for one particle: v_p += N * (f_i / m_i) * dt over the
  * adjacent nodes in GridPoint_Mediator order (x, then y, then z innermost),
  * DP:490-510. */
-__device__ __forceinline__ void g2p_velocity_update(const Inst &I, const NView &Nd, const double x[3], double v[3]) {
+template <class REC>
+__device__ __forceinline__ void g2p_velocity_update_rec(const Inst &I, REC rec, const double x[3], double v[3]) {
     const GridGeom &g = I.g;
     /* grid fields read once, outside any branch */
     const double gL[3] = {g.L[0], g.L[1], g.L[2]}, gh[3] = {g.h[0], g.h[1], g.h[2]};
@@ -654,9 +775,7 @@ __device__ __forceinline__ void g2p_velocity_update(const Inst &I, const NView &
         in[c] = ok[0][ox] & ok[1][oy] & ok[2][oz];
         const size_t node =
             in[c] ? node_off + (size_t)(b[0] + ox) + nx * ((size_t)(b[1] + oy) + ny * (size_t)(b[2] + oz)) : node_off;
-        const double2 *r = reinterpret_cast<const double2 *>(Nd.g1 + node * 4u);
-        ra[c] = r[0];
-        rb[c] = r[1];
+        rec(c, node, ra[c], rb[c]);
     }
 #pragma unroll
     for (int c = 0; c < 8; c++) {
@@ -670,6 +789,18 @@ __device__ __forceinline__ void g2p_velocity_update(const Inst &I, const NView &
     }
 }
 
+__device__ __forceinline__ void g2p_velocity_update(const Inst &I, const NView &Nd, const double x[3], double v[3]) {
+    const double *g1 = Nd.g1;
+    g2p_velocity_update_rec(I,
+                            [=](int, size_t node, double2 &a, double2 &b) {
+                                const double2 *r = reinterpret_cast<const double2 *>(g1 + node * 4u);
+                                a = r[0];
+                                b = r[1];
+                            },
+                            x, v);
+}
+
+template <bool MULTI>
 __global__ __launch_bounds__(BLOCK) void k_g2p_velocity(Batch bt, PView P, NView Nd) {
     uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= P.n) return;
@@ -677,7 +808,7 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_velocity(Batch bt, PView P, NView
     const size_t n = P.n;
     const double x[3] = {P.x[i], P.x[n + i], P.x[2 * n + i]};
     double v[3] = {P.v[i], P.v[n + i], P.v[2 * n + i]};
-    g2p_velocity_update(bt.inst[inst_of_particle(bt, i)], Nd, x, v);
+    g2p_velocity_update(inst_p<MULTI>(bt, i), Nd, x, v);
     P.v[i] = v[0];
     P.v[n + i] = v[1];
     P.v[2 * n + i] = v[2];
@@ -685,10 +816,10 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_velocity(Batch bt, PView P, NView
 
 /* Second DC override (DP:568) and BC re-application (DP:573-599) on the
  * remapped velocity; stores v, the final p, zeroed forces and p/m. */
-__device__ __forceinline__ void node_remap_finish(const NView &Nd, int64_t node, size_t nn, double m, double v[3],
-                                                  const DCWin &w) {
-    double p[3] = {Nd.p[node], Nd.p[nn + node], Nd.p[2 * nn + node]};
-    const uint8_t fixed = Nd.fixed[node];
+__device__ __forceinline__ void node_remap_finish(const NodeIn &in, const NView &Nd, int64_t node, size_t nn,
+                                                  double m, double v[3], const DCWin &w) {
+    double p[3] = {in.p[0], in.p[1], in.p[2]};
+    const uint8_t fixed = in.fixed;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         if (w.rank[a] >= 0) {
@@ -708,10 +839,10 @@ __device__ __forceinline__ void node_remap_finish(const NView &Nd, int64_t node,
 
 /* Contact loop: BC re-applied (DPC:380-411), then displacement control on
  * all axes (DPC:413-433), so control wins over fixed components. */
-__device__ __forceinline__ void node_remap_finish_contact(const NView &Nd, int64_t node, size_t nn, double m,
-                                                          double v[3], const DCWin &w) {
-    double p[3] = {Nd.p[node], Nd.p[nn + node], Nd.p[2 * nn + node]};
-    const uint8_t fixed = Nd.fixed[node];
+__device__ __forceinline__ void node_remap_finish_contact(const NodeIn &in, const NView &Nd, int64_t node, size_t nn,
+                                                          double m, double v[3], const DCWin &w) {
+    double p[3] = {in.p[0], in.p[1], in.p[2]};
+    const uint8_t fixed = in.fixed;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         if ((fixed >> a) & 1) {
@@ -731,8 +862,8 @@ __device__ __forceinline__ void node_remap_finish_contact(const NView &Nd, int64
 
 /* The node side of the remap phase for each loop (the single pass runs it
  * before its only grid-to-particle pass, as its velocity sum). */
-__device__ __forceinline__ void node_remap_dispatch(const Batch &bt, const Inst &I, const NView &Nd, int64_t node,
-                                                    double m, double v[3], const DCWin &w);
+__device__ __forceinline__ void node_remap_dispatch(const Batch &bt, const NodeIn &in, const NView &Nd, int64_t node,
+                                                    double v[3], const DCWin &w);
 
 /* Velocity remap (DP:539-564) gathered per node in ascending particle
  * order, then the second displacement-control override (DP:568) and the BC
@@ -750,8 +881,9 @@ __global__ __launch_bounds__(BLOCK) void k_remap_grid(Batch bt, PView P, NView N
     const size_t n = P.n, nn = (size_t)bt.nnodes;
     const double xn[3] = {node_coord(ix, g.L[0], g.cells[0]), node_coord(iy, g.L[1], g.cells[1]),
                           node_coord(iz, g.L[2], g.cells[2])};
-    const double m = Nd.m[node];
-    const bool massive = m > I.mass_min;
+    const NodeIn in = node_in<true>(bt, I, Nd, node);
+    const double m = in.m;
+    const bool massive = m > in.mass_min;
     Merge8 M;
     merge_init(M, B, g, I.cell_off, ix, iy, iz);
     double v[3] = {0.0, 0.0, 0.0};
@@ -771,18 +903,18 @@ __global__ __launch_bounds__(BLOCK) void k_remap_grid(Batch bt, PView P, NView N
         dc_consider(w, P, id, all_axes);
         merge_advance(M, B, id);
     }
-    node_remap_dispatch(bt, I, Nd, node, m, v, w);
+    node_remap_dispatch(bt, in, Nd, node, v, w);
 }
 
-__device__ __forceinline__ void node_remap_dispatch(const Batch &bt, const Inst &I, const NView &Nd, int64_t node,
-                                                    double m, double v[3], const DCWin &w) {
+__device__ __forceinline__ void node_remap_dispatch(const Batch &bt, const NodeIn &in, const NView &Nd, int64_t node,
+                                                    double v[3], const DCWin &w) {
     const size_t nn = (size_t)bt.nnodes;
     if (bt.variant == VAR_DPC)
-        node_remap_finish_contact(Nd, node, nn, m, v, w);
+        node_remap_finish_contact(in, Nd, node, nn, in.m, v, w);
     else if (bt.variant == VAR_SP)
-        node_update_singlepass(I, Nd, node, nn, m, v, w);
+        node_update_singlepass(in, Nd, node, nn, in.m, v, w);
     else
-        node_remap_finish(Nd, node, nn, m, v, w);
+        node_remap_finish(in, Nd, node, nn, in.m, v, w);
 }
 
 /* ---- cell order: per (occupied cell, corner) partial sums, then nodes ----
@@ -807,7 +939,7 @@ __device__ __forceinline__ void node_remap_dispatch(const Batch &bt, const Inst 
 /* G1 (remap pass of DoublePass and its contact variant): G2P pass 1 runs
  * here too, on each particle as it is staged (each particle is staged by
  * exactly one block), and the updated velocity is written back. */
-template <bool REMAP, bool G1>
+template <bool REMAP, bool G1, bool MULTI>
 __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NView Nd, BView B, CellParts C) {
     /* staged particle inputs: x, m, v, then (P2G) V, sigma, f_ext */
     __shared__ double sx[3][PART_STAGE], sm[PART_STAGE], sv[3][PART_STAGE];
@@ -832,7 +964,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
             const uint32_t c = B.occ[slot];
             s0 = B.start[c];
             s1 = B.start[c + 1];
-            Ip = &bt.inst[inst_of_particle(bt, B.sorted[s0])];
+            Ip = &inst_p<MULTI>(bt, B.sorted[s0]);
             const GridGeom &g = Ip->g;
             const uint32_t e = c - (uint32_t)Ip->cell_off;
             const int ex = (int)(e % (uint32_t)g.ec[0]);
@@ -842,6 +974,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
             iy = ey - 1 + ((corner >> 1) & 1);
             iz = ez - 1 + (corner & 1);
             if (ix < 0 || iy < 0 || iz < 0 || ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) have = false;
+            if (!REMAP && have && bt.tflag) bt.tflag[Ip->tile_off + tile_of_node(bt, g, ix, iy, iz)] = 1;
         }
         double xn[3] = {0.0, 0.0, 0.0}, hh[3] = {1.0, 1.0, 1.0};
         double mn = 0.0;
@@ -871,7 +1004,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
                 double v[3] = {P.v[id], P.v[n + id], P.v[2 * n + id]};
                 const int dcr = (skip_dc || (G1 && bt.variant != VAR_DP)) ? P.dc_rank[id] : -1;
                 if (G1 && !(bt.variant != VAR_DP && dcr >= 0)) { /* DPC:309-310 */
-                    g2p_velocity_update(bt.inst[inst_of_particle(bt, id)], Nd, x, v);
+                    g2p_velocity_update(inst_p<MULTI>(bt, id), Nd, x, v);
                     P.v[id] = v[0];
                     P.v[n + id] = v[1];
                     P.v[2 * n + id] = v[2];
@@ -962,29 +1095,50 @@ __device__ __forceinline__ void dc_scan_cell(DCWin &w, const PView &P, const BVi
 /* Node side of the cell order: adds the 8 partials in neighbour order
  * k = 0..7 (x outer, z inner), then the same grid update as the particle
  * order (P2G) or the same remap finish (REMAP). */
-template <bool REMAP>
+template <bool REMAP, bool MULTI>
 __global__ __launch_bounds__(BLOCK) void k_node_cells(Batch bt, PView P, NView Nd, BView B, CellParts C) {
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-    const Inst &I = bt.inst[inst_of_tile(bt, tile)];
+    /* tiles with no corner of an occupied cell: all their nodes are inactive
+     * (the remap pass also re-arms the flag for the next step, after every
+     * read of this kernel is issued: a store ahead of them would turn them
+     * into vector loads that each wait on their own) */
+    __shared__ uint8_t s_live;
+    if (threadIdx.x == 0) s_live = bt.tflag[tile];
+    __syncthreads();
+    const bool live_tile = s_live != 0;
+    const Inst &I = inst_t<MULTI>(bt, tile);
     const GridGeom &g = I.g;
+    const uint32_t cell_off = (uint32_t)I.cell_off, ecx = (uint32_t)g.ec[0], ecy = (uint32_t)g.ec[1];
     int ix, iy, iz;
     node_of_tile(bt, g, tile - I.tile_off, ix, iy, iz);
-    if (ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) return;
+    const bool in_grid = ix < g.nn[0] && iy < g.nn[1] && iz < g.nn[2];
     const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
-    if (REMAP && !Nd.active[node]) return;
+    const bool work = in_grid && live_tile;
+    bool act = true;
+    uint32_t sl[8];
+    NodeIn in;
+    if (work) {
+        if (REMAP) act = Nd.active[node] != 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint32_t c = cell_off + (uint32_t)(ix + (k >> 2)) +
+                               ecx * ((uint32_t)(iy + ((k >> 1) & 1)) + ecy * (uint32_t)(iz + (k & 1)));
+            sl[k] = B.slot[c];
+        }
+        in = node_in<REMAP>(bt, I, Nd, node);
+    }
+    if (REMAP && threadIdx.x == 0) bt.tflag[tile] = 0;
+    if (!in_grid) return;
+    if (!live_tile) {
+        if (!REMAP) Nd.active[node] = 0;
+        return;
+    }
+    if (REMAP && !act) return;
     double m = 0.0, p[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0};
     DCWin w;
     w.rank[0] = w.rank[1] = w.rank[2] = -1;
     w.v[0] = w.v[1] = w.v[2] = 0.0;
     const bool all_axes = bt.variant != VAR_DP;
-    const uint32_t cell_off = (uint32_t)I.cell_off, ecx = (uint32_t)g.ec[0], ecy = (uint32_t)g.ec[1];
-    uint32_t sl[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const uint32_t c = cell_off + (uint32_t)(ix + (k >> 2)) +
-                           ecx * ((uint32_t)(iy + ((k >> 1) & 1)) + ecy * (uint32_t)(iz + (k & 1)));
-        sl[k] = B.slot[c];
-    }
     bool any = false;
 #pragma unroll
     for (int k = 0; k < 8; k++) any |= sl[k] != MPM_NONE;
@@ -1044,9 +1198,9 @@ __global__ __launch_bounds__(BLOCK) void k_node_cells(Batch bt, PView P, NView N
             dc_scan_cell(w, P, B, c, all_axes);
         }
     if (!REMAP) {
-        node_p2g_finish(bt, I, Nd, node, m, p, f, w);
+        node_p2g_finish(bt, in, Nd, node, m, p, f, w);
     } else {
-        node_remap_dispatch(bt, I, Nd, node, Nd.m[node], f, w);
+        node_remap_dispatch(bt, in, Nd, node, f, w);
     }
 }
 
@@ -1056,55 +1210,83 @@ __device__ __forceinline__ void block_count_add(unsigned long long *dst, uint32_
     if ((threadIdx.x & 63) == 0 && b) atomicAdd(dst, (unsigned long long)__popcll(b));
 }
 
+/* The accumulated fields the state update reads and writes back.  They
+ * are read with the rest of the particle's state, before the kernel's
+ * first store: a read-modify-write in place would make each load wait on
+ * the store before it (the compiler cannot prove the arrays apart). */
+struct PAccum {
+    double eps[6], epsp[6], Ws, Wti, Wmax, Wp;
+};
+
+/* DP reads all of them; the contact and single-pass updates (new stress
+ * energies, strain updated before the material check) need no history
+ * maximum or time integral. */
+__device__ __forceinline__ void load_accum(const PView &P, size_t i, bool dp, PAccum &a) {
+    const size_t n = P.n;
+#pragma unroll
+    for (int k = 0; k < 6; k++) a.eps[k] = P.eps[k * n + i];
+#pragma unroll
+    for (int k = 0; k < 6; k++) a.epsp[k] = P.epsp[k * n + i];
+    a.Ws = P.Ws[i];
+    a.Wp = P.Wp[i];
+    a.Wti = dp ? P.Wti[i] : 0.0;
+    a.Wmax = dp ? P.Wmax[i] : 0.0;
+}
+
 /* State update after the return map (DP:242-278 minus F and V, which the
  * caller has done): eps, sigma, plastic strain, energies, hardening. */
 __device__ __forceinline__ uint32_t finish_update(const PView &P, size_t i, const double de[6], double sig[6],
-                                                  const CRResult &cr, double R, double V, double dt) {
+                                                  const CRResult &cr, double R, double V, double dt,
+                                                  const PAccum &a) {
     const size_t n = P.n;
 #pragma unroll
-    for (int k = 0; k < 6; k++) P.eps[k * n + i] += de[k];
+    for (int k = 0; k < 6; k++) P.eps[k * n + i] = a.eps[k] + de[k];
 #pragma unroll
     for (int k = 0; k < 6; k++) sig[k] += cr.dsig[k];
 #pragma unroll
     for (int k = 0; k < 6; k++) P.sig[k * n + i] = sig[k];
 #pragma unroll
-    for (int k = 0; k < 6; k++) P.epsp[k * n + i] += cr.depsp[k];
+    for (int k = 0; k < 6; k++) P.epsp[k * n + i] = a.epsp[k] + cr.depsp[k];
     double wi = 0.0;
 #pragma unroll
     for (int k = 0; k < 6; k++) wi += (sig[k] - 0.5 * cr.dsig[k]) * de[k] * V;
-    double Ws = P.Ws[i] + wi;
+    double Ws = a.Ws + wi;
     P.Ws[i] = Ws;
-    P.Wti[i] += Ws * dt;
-    if (Ws > P.Wmax[i]) P.Wmax[i] = Ws;
+    P.Wti[i] = a.Wti + Ws * dt;
+    if (Ws > a.Wmax) P.Wmax[i] = Ws;
     double wp = 0.0;
 #pragma unroll
     for (int k = 0; k < 6; k++) wp += (sig[k] - 0.5 * cr.dsig[k]) * cr.depsp[k] * V;
-    P.Wp[i] += wp;
+    P.Wp[i] = a.Wp + wp;
     P.R[i] = R + cr.dR;
     uint32_t bad = 0;
     for (int k = 0; k < 6; k++) bad |= !isfinite(sig[k]);
     return bad;
 }
 
-/* A slot in an append-only queue, one atomic per wave: the active lanes
- * that call it take consecutive slots in lane order after the wave's base. */
-__device__ __forceinline__ uint32_t wave_append_u32(uint32_t *counter) {
-    const uint64_t mask = __ballot(1);
-    const int lane = (int)(threadIdx.x & 63);
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader, 64);
-    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-}
-__device__ __forceinline__ unsigned long long wave_append_u64(unsigned long long *counter) {
-    const uint64_t mask = __ballot(1);
-    const int lane = (int)(threadIdx.x & 63);
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
-    base = __shfl(base, leader, 64);
-    return base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
+/* A slot in an append-only queue, one atomic per block: lanes with want
+ * set take consecutive slots in thread order after the block's base.  Every
+ * thread of the block must call it (it synchronises the block). */
+template <typename T>
+__device__ __forceinline__ T block_append(T *counter, bool want) {
+    __shared__ T s_off[BLOCK / 64], s_base;
+    const uint64_t mask = __ballot(want);
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    if (lane == 0) s_off[wave] = (T)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T tot = 0;
+        for (int w = 0; w < BLOCK / 64; w++) {
+            const T c = s_off[w];
+            s_off[w] = tot;
+            tot += c;
+        }
+        s_base = tot ? atomicAdd(counter, tot) : (T)0;
+    }
+    __syncthreads();
+    const T slot = s_base + s_off[wave] + (T)__popcll(mask & ((1ull << lane) - 1ull));
+    __syncthreads();
+    return slot;
 }
 
 /* Stress, plastic strain, energies and backstress of the contact and
@@ -1113,19 +1295,19 @@ __device__ __forceinline__ unsigned long long wave_append_u64(unsigned long long
  * time integral.  The strain was already updated. */
 __device__ __forceinline__ uint32_t finish_update_new_stress(const PView &P, size_t i, double sig[6],
                                                              const CRResult &cr, double R, double V,
-                                                             const double de[6]) {
+                                                             const double de[6], const PAccum &a) {
     const size_t n = P.n;
 #pragma unroll
     for (int k = 0; k < 6; k++) sig[k] += cr.dsig[k];
 #pragma unroll
     for (int k = 0; k < 6; k++) P.sig[k * n + i] = sig[k];
 #pragma unroll
-    for (int k = 0; k < 6; k++) P.epsp[k * n + i] += cr.depsp[k];
-    double Ws = P.Ws[i];
+    for (int k = 0; k < 6; k++) P.epsp[k * n + i] = a.epsp[k] + cr.depsp[k];
+    double Ws = a.Ws;
 #pragma unroll
     for (int k = 0; k < 6; k++) Ws += sig[k] * de[k] * V;
     P.Ws[i] = Ws;
-    double Wp = P.Wp[i];
+    double Wp = a.Wp;
 #pragma unroll
     for (int k = 0; k < 6; k++) Wp += sig[k] * cr.depsp[k] * V;
     P.Wp[i] = Wp;
@@ -1160,7 +1342,182 @@ __device__ __forceinline__ void trial_state(const MatTable &mt, int m, double R,
  * DP:766-789 and the next step's binning key) and then the return map and
  * state update, a pure per-particle stream.  The strain increment passes
  * between them in P.de. */
-template <int VAR>
+/* The kinematics of one particle: x, L, the strain increment, F, V, the
+ * control schedule and the particle's next cell key.  rec(corner, node,
+ * r0..r3) fetches the packed g2 record of one adjacent node.  Every
+ * instance field and every particle field the pass needs is read before
+ * its first store: reads after a store cannot be proven unaliased, become
+ * vector loads and each waits on its own. */
+template <int VAR, class REC>
+__device__ __forceinline__ void g2p_kinematics_particle(const Batch &bt, const PView &P, const Inst &I, uint32_t i,
+                                                        REC rec, uint32_t &next_key, uint32_t &f_outside) {
+    const size_t n = P.n;
+    const GridGeom g = I.g;
+    const double dt = I.dt, mass_min = I.mass_min;
+    const size_t node_off = (size_t)I.node_off;
+    const uint32_t cell_off = (uint32_t)I.cell_off;
+    const int32_t tl_count = I.tl_count;
+    const double vt[3] = {I.vt[0], I.vt[1], I.vt[2]};
+    const double x0[3] = {P.x[i], P.x[n + i], P.x[2 * n + i]};
+    const int32_t dcr = P.dc_rank[i];
+    const int mat = P.mat[i];
+    double F[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) F[c][r] = P.F[(3 * c + r) * n + i];
+    const double V0 = P.V0[i];
+    const double hx[3] = {g.h[0], g.h[1], g.h[2]};
+    const int gn[3] = {g.nn[0], g.nn[1], g.nn[2]};
+    double x[3] = {x0[0], x0[1], x0[2]};
+    int b[3], lo[3], hi[3];
+    AxisShape as[3][2];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        base_cell(x0[a], hx[a], gn[a], b[a], lo[a], hi[a]);
+#pragma unroll
+        for (int o = 0; o < 2; o++) as[a][o] = axis_shape(x0[a], node_coord(b[a] + o, g.L[a], g.cells[a]), hx[a]);
+    }
+    double L[3][3]; /* glm column-major L[col][row] */
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) L[c][r] = 0.0;
+    bool ok[3][2];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int o = 0; o < 2; o++)
+            ok[a][o] = (o >= lo[a]) & (o < hi[a]) & (b[a] + o >= 0) & (b[a] + o <= gn[a] - 1);
+    f_outside = !((ok[0][0] || ok[0][1]) && (ok[1][0] || ok[1][1]) && (ok[2][0] || ok[2][1]));
+    /* the contact and single-pass loops leave controlled particles out
+     * of the grid-to-particle pass (DPC:443-444, SP:279-280) */
+    const bool ctl = VAR != VAR_DP && dcr >= 0;
+    uint8_t flag = 0;
+    if (!ctl) {
+        const size_t nx = (size_t)gn[0], ny = (size_t)gn[1];
+        /* corners in GridPoint_Mediator order: x outer, z inner;
+         * branch-free over the 8 corners (off-grid corners read the
+         * instance's first node and add nothing), so that all gathers can
+         * be in flight */
+#pragma unroll
+        for (int cc = 0; cc < 8; cc++) {
+            const int ox = cc >> 2, oy = (cc >> 1) & 1, oz = cc & 1;
+            const bool in = ok[0][ox] && ok[1][oy] && ok[2][oz];
+            const size_t node =
+                in ? node_off + (size_t)(b[0] + ox) + nx * ((size_t)(b[1] + oy) + ny * (size_t)(b[2] + oz))
+                   : node_off;
+            double N, gr[3];
+            shape_full(as[0][ox], as[1][oy], as[2][oz], hx, N, gr[0], gr[1], gr[2]);
+            double2 r0, r1, r2, r3;
+            rec(cc, node, r0, r1, r2, r3);
+            const bool take = in && r0.x > mass_min;
+            const double t0 = x[0] + N * r0.y * dt, t1 = x[1] + N * r1.x * dt, t2 = x[2] + N * r1.y * dt;
+            x[0] = take ? t0 : x[0];
+            x[1] = take ? t1 : x[1];
+            x[2] = take ? t2 : x[2];
+            const double vv[3] = {r2.x, r2.y, r3.x};
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    const double t = L[c][r] + vv[r] * gr[c];
+                    L[c][r] = in ? t : L[c][r];
+                }
+        }
+        P.x[i] = x[0];
+        P.x[n + i] = x[1];
+        P.x[2 * n + i] = x[2];
+        const bool valid = mat >= 0 && mat < bt.nmat;
+        double dF[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) dF[c][r] = (c == r ? 1.0 : 0.0) + L[c][r] * dt;
+        double de[6];
+        if (VAR == VAR_DP) {
+            double E[3][3];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) E[c][r] = 0.5 * (dF[r][c] + dF[c][r]) - (c == r ? 1.0 : 0.0);
+            de[0] = E[0][0];
+            de[1] = E[1][1];
+            de[2] = E[2][2];
+            de[3] = E[0][1] + E[1][0];
+            de[4] = E[1][2] + E[2][1];
+            de[5] = E[2][0] + E[0][2];
+        } else {
+            /* DPC:469-477, SP:310-318: straight from dF */
+            de[0] = dF[0][0] - 1.0;
+            de[1] = dF[1][1] - 1.0;
+            de[2] = dF[2][2] - 1.0;
+            de[3] = dF[0][1] + dF[1][0];
+            de[4] = dF[1][2] + dF[2][1];
+            de[5] = dF[2][0] + dF[0][2];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++) P.de[k * n + i] = de[k];
+        /* DP: F = F dF after the material check (DP:242-245); DPC / SP:
+         * F += (L F) dt before it (DPC:464-467, SP:305-308) */
+        if (VAR != VAR_DP || valid) {
+            double Fn[3][3];
+            if (VAR == VAR_DP) {
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+#pragma unroll
+                    for (int r = 0; r < 3; r++)
+                        Fn[c][r] = F[0][r] * dF[c][0] + F[1][r] * dF[c][1] + F[2][r] * dF[c][2];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+#pragma unroll
+                    for (int r = 0; r < 3; r++)
+                        Fn[c][r] = F[c][r] + (L[0][r] * F[c][0] + L[1][r] * F[c][1] + L[2][r] * F[c][2]) * dt;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) P.F[(3 * c + r) * n + i] = Fn[c][r];
+            double det = +Fn[0][0] * (Fn[1][1] * Fn[2][2] - Fn[2][1] * Fn[1][2]) -
+                         Fn[1][0] * (Fn[0][1] * Fn[2][2] - Fn[2][1] * Fn[0][2]) +
+                         Fn[2][0] * (Fn[0][1] * Fn[1][2] - Fn[1][1] * Fn[0][2]);
+            P.V[i] = det * V0;
+        }
+        flag = !(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2])) ? 1 : 0;
+    }
+    P.flag[i] = flag;
+    /* displacement-control schedule: DP:766-789 (flagged axes), or
+     * DPC:531-540 / SP:369-378 (all axes, then x += v dt) */
+    if (VAR == VAR_DP) {
+        if (tl_count != 0 && dcr >= 0) {
+            uint8_t axm = P.dc_axes[i];
+            double mult = (double)P.dc_mult[i];
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                if ((axm >> a) & 1) P.v[a * n + i] = mult * vt[a];
+        }
+    } else if (ctl) {
+        double v[3] = {P.v[i], P.v[n + i], P.v[2 * n + i]};
+        if (tl_count != 0) {
+            double mult = (double)P.dc_mult[i];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                v[a] = mult * vt[a];
+                P.v[a * n + i] = v[a];
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            x[a] = x0[a] + v[a] * dt;
+            P.x[a * n + i] = x[a];
+        }
+    }
+    next_key = cell_key(g, x[0], x[1], x[2]);
+    if (next_key != MPM_NONE) next_key += cell_off;
+}
+
+template <int VAR, bool MULTI>
 __global__ __launch_bounds__(BLOCK) void k_g2p_kinematics(Batch bt, PView P, NView Nd, BView B,
                                                          unsigned long long *health) {
     uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
@@ -1168,170 +1525,16 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_kinematics(Batch bt, PView P, NVi
     uint32_t f_outside = 0;
     uint32_t next_key = MPM_NONE;
     if (live) {
-        const Inst &I = bt.inst[inst_of_particle(bt, i)];
-        const GridGeom &g = I.g;
-        const size_t n = P.n;
-        /* grid fields read once, outside any branch */
-        const double gL[3] = {g.L[0], g.L[1], g.L[2]}, gh[3] = {g.h[0], g.h[1], g.h[2]};
-        const int gc[3] = {g.cells[0], g.cells[1], g.cells[2]}, gn[3] = {g.nn[0], g.nn[1], g.nn[2]};
-        const double x0[3] = {P.x[i], P.x[n + i], P.x[2 * n + i]};
-        double x[3] = {x0[0], x0[1], x0[2]};
-        int b[3], lo[3], hi[3];
-        AxisShape as[3][2];
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            base_cell(x0[a], gh[a], gn[a], b[a], lo[a], hi[a]);
-#pragma unroll
-            for (int o = 0; o < 2; o++) as[a][o] = axis_shape(x0[a], node_coord(b[a] + o, gL[a], gc[a]), gh[a]);
-        }
-        double L[3][3]; /* glm column-major L[col][row] */
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-#pragma unroll
-            for (int r = 0; r < 3; r++) L[c][r] = 0.0;
-        const double dt = I.dt;
-        bool ok[3][2];
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-#pragma unroll
-            for (int o = 0; o < 2; o++)
-                ok[a][o] = (o >= lo[a]) & (o < hi[a]) & (b[a] + o >= 0) & (b[a] + o <= gn[a] - 1);
-        f_outside = !((ok[0][0] || ok[0][1]) && (ok[1][0] || ok[1][1]) && (ok[2][0] || ok[2][1]));
-        /* the contact and single-pass loops leave controlled particles out
-         * of the grid-to-particle pass (DPC:443-444, SP:279-280) */
-        const bool ctl = VAR != VAR_DP && P.dc_rank[i] >= 0;
-        uint8_t flag = 0;
-        if (!ctl) {
-            /* instance fields read once: inside the per-corner branches they
-             * would be reloaded (and waited for) corner by corner */
-            const double mass_min = I.mass_min;
-            const double hx[3] = {gh[0], gh[1], gh[2]};
-            const size_t node_off = (size_t)I.node_off, nx = (size_t)gn[0], ny = (size_t)gn[1];
-            /* corners in GridPoint_Mediator order: x outer, z inner */
-            /* branch-free over the 8 corners (off-grid corners read the
-             * instance's first node and add nothing), so that all gathers can
-             * be in flight */
-#pragma unroll
-            for (int cc = 0; cc < 8; cc++) {
-                const int ox = cc >> 2, oy = (cc >> 1) & 1, oz = cc & 1;
-                const bool in = ok[0][ox] && ok[1][oy] && ok[2][oz];
-                const size_t node =
-                    in ? node_off + (size_t)(b[0] + ox) + nx * ((size_t)(b[1] + oy) + ny * (size_t)(b[2] + oz))
-                       : node_off;
-                double N, gr[3];
-                shape_full(as[0][ox], as[1][oy], as[2][oz], hx, N, gr[0], gr[1], gr[2]);
-                const double2 *rec = reinterpret_cast<const double2 *>(Nd.g2 + node * 8u);
-                const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
-                const bool take = in && r0.x > mass_min;
-                const double t0 = x[0] + N * r0.y * dt, t1 = x[1] + N * r1.x * dt, t2 = x[2] + N * r1.y * dt;
-                x[0] = take ? t0 : x[0];
-                x[1] = take ? t1 : x[1];
-                x[2] = take ? t2 : x[2];
-                const double vv[3] = {r2.x, r2.y, r3.x};
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-#pragma unroll
-                    for (int r = 0; r < 3; r++) {
-                        const double t = L[c][r] + vv[r] * gr[c];
-                        L[c][r] = in ? t : L[c][r];
-                    }
-            }
-            P.x[i] = x[0];
-            P.x[n + i] = x[1];
-            P.x[2 * n + i] = x[2];
-            const int mat = P.mat[i];
-            const bool valid = mat >= 0 && mat < bt.nmat;
-            double dF[3][3];
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-#pragma unroll
-                for (int r = 0; r < 3; r++) dF[c][r] = (c == r ? 1.0 : 0.0) + L[c][r] * dt;
-            double de[6];
-            if (VAR == VAR_DP) {
-                double E[3][3];
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-#pragma unroll
-                    for (int r = 0; r < 3; r++) E[c][r] = 0.5 * (dF[r][c] + dF[c][r]) - (c == r ? 1.0 : 0.0);
-                de[0] = E[0][0];
-                de[1] = E[1][1];
-                de[2] = E[2][2];
-                de[3] = E[0][1] + E[1][0];
-                de[4] = E[1][2] + E[2][1];
-                de[5] = E[2][0] + E[0][2];
-            } else {
-                /* DPC:469-477, SP:310-318: straight from dF */
-                de[0] = dF[0][0] - 1.0;
-                de[1] = dF[1][1] - 1.0;
-                de[2] = dF[2][2] - 1.0;
-                de[3] = dF[0][1] + dF[1][0];
-                de[4] = dF[1][2] + dF[2][1];
-                de[5] = dF[2][0] + dF[0][2];
-            }
-#pragma unroll
-            for (int k = 0; k < 6; k++) P.de[k * n + i] = de[k];
-            /* DP: F = F dF after the material check (DP:242-245); DPC / SP:
-             * F += (L F) dt before it (DPC:464-467, SP:305-308) */
-            if (VAR != VAR_DP || valid) {
-                double F[3][3], Fn[3][3];
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-#pragma unroll
-                    for (int r = 0; r < 3; r++) F[c][r] = P.F[(3 * c + r) * n + i];
-                if (VAR == VAR_DP) {
-#pragma unroll
-                    for (int c = 0; c < 3; c++)
-#pragma unroll
-                        for (int r = 0; r < 3; r++)
-                            Fn[c][r] = F[0][r] * dF[c][0] + F[1][r] * dF[c][1] + F[2][r] * dF[c][2];
-                } else {
-#pragma unroll
-                    for (int c = 0; c < 3; c++)
-#pragma unroll
-                        for (int r = 0; r < 3; r++)
-                            Fn[c][r] = F[c][r] + (L[0][r] * F[c][0] + L[1][r] * F[c][1] + L[2][r] * F[c][2]) * dt;
-                }
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-#pragma unroll
-                    for (int r = 0; r < 3; r++) P.F[(3 * c + r) * n + i] = Fn[c][r];
-                double det = +Fn[0][0] * (Fn[1][1] * Fn[2][2] - Fn[2][1] * Fn[1][2]) -
-                             Fn[1][0] * (Fn[0][1] * Fn[2][2] - Fn[2][1] * Fn[0][2]) +
-                             Fn[2][0] * (Fn[0][1] * Fn[1][2] - Fn[1][1] * Fn[0][2]);
-                P.V[i] = det * P.V0[i];
-            }
-            flag = !(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2])) ? 1 : 0;
-        }
-        P.flag[i] = flag;
-        /* displacement-control schedule: DP:766-789 (flagged axes), or
-         * DPC:531-540 / SP:369-378 (all axes, then x += v dt) */
-        if (VAR == VAR_DP) {
-            if (I.tl_count != 0 && P.dc_rank[i] >= 0) {
-                uint8_t axm = P.dc_axes[i];
-                double mult = (double)P.dc_mult[i];
-#pragma unroll
-                for (int a = 0; a < 3; a++)
-                    if ((axm >> a) & 1) P.v[a * n + i] = mult * I.vt[a];
-            }
-        } else if (ctl) {
-            double v[3] = {P.v[i], P.v[n + i], P.v[2 * n + i]};
-            if (I.tl_count != 0) {
-                double mult = (double)P.dc_mult[i];
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    v[a] = mult * I.vt[a];
-                    P.v[a * n + i] = v[a];
-                }
-            }
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                x[a] = x0[a] + v[a] * dt;
-                P.x[a * n + i] = x[a];
-            }
-        }
-        /* next step's binning */
-        next_key = cell_key(g, x[0], x[1], x[2]);
-        if (next_key != MPM_NONE) next_key += (uint32_t)I.cell_off;
+        const double *g2 = Nd.g2;
+        g2p_kinematics_particle<VAR>(bt, P, inst_p<MULTI>(bt, i), i,
+                                     [=](int, size_t node, double2 &r0, double2 &r1, double2 &r2, double2 &r3) {
+                                         const double2 *q = reinterpret_cast<const double2 *>(g2 + node * 8u);
+                                         r0 = q[0];
+                                         r1 = q[1];
+                                         r2 = q[2];
+                                         r3 = q[3];
+                                     },
+                                     next_key, f_outside);
         B.key[i] = next_key;
     }
     wave_key_add(B.count, next_key, next_key != MPM_NONE);
@@ -1352,83 +1555,94 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_constitutive(Batch bt, PView P, M
     uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     uint32_t f_null = 0, f_plastic = 0, f_nonfinite = 0, f_queued = 0;
     const size_t n = P.n;
-    if (i < P.n && !(VAR != VAR_DP && P.dc_rank[i] >= 0)) {
-        const double dt = bt.inst[inst_of_particle(bt, i)].dt;
-        const int mat = P.mat[i];
-        const bool valid = mat >= 0 && mat < mt.n;
-        double de[6];
+    const bool live = i < P.n && !(VAR != VAR_DP && P.dc_rank[i] >= 0);
+    double de[6], sig[6], R = 0.0, V = 0.0, dt = 0.0;
+    PAccum acc;
+    int mat = -1;
+    bool valid = false;
+    if (live) {
+        dt = bt.inst[inst_of_particle(bt, i)].dt;
+        mat = P.mat[i];
+        valid = mat >= 0 && mat < mt.n;
 #pragma unroll
         for (int k = 0; k < 6; k++) de[k] = P.de[k * n + i];
-        if (VAR != VAR_DP) {
-#pragma unroll
-            for (int k = 0; k < 6; k++) P.eps[k * n + i] += de[k];
-        }
-        const double V = P.V[i];
-        double sig[6];
+        V = P.V[i];
 #pragma unroll
         for (int k = 0; k < 6; k++) sig[k] = P.sig[k * n + i];
-        const double R = P.R[i];
+        R = P.R[i];
         f_nonfinite = P.flag[i] & 1u;
+        load_accum(P, i, VAR == VAR_DP, acc);
+        if (VAR != VAR_DP) {
+            /* the strain is updated before the material check */
+#pragma unroll
+            for (int k = 0; k < 6; k++) P.eps[k * n + i] = acc.eps[k] + de[k];
+        }
         if (!valid) f_null = 1;
+    }
+    /* the learned map takes the points of its material whose trial state yields */
+    bool queue = false;
+    if (MODE == MODE_SURROGATE && live && valid && mat == sg.material) {
+        double dtr[6], J2, K;
+        trial_state(mt, mat, R, sig, de, dtr, J2, K);
+        queue = !(J2 <= K * K);
+    }
+    if (MODE == MODE_SURROGATE) {
+        const uint32_t slot = block_append<uint32_t>(sg.qcount, queue);
+        if (queue) {
+            sg.qid[slot] = i;
+            const size_t cap = (size_t)P.n;
+            float feat[MLP_IN];
+            for (int k = 0; k < 6; k++) feat[k] = (float)(sig[k] / sg.stress_scale);
+            for (int k = 0; k < 6; k++) feat[6 + k] = (float)(de[k] / sg.strain_scale);
+            feat[12] = (float)(R / sg.stress_scale);
+            uint8_t inside = 1;
+            for (int k = 0; k < MLP_IN; k++) {
+                float z = (feat[k] - sg.norm.in_mean[k]) / sg.norm.in_std[k];
+                inside &= (z >= sg.norm.in_lo[k] && z <= sg.norm.in_hi[k]) ? 1 : 0;
+                sg.qx[(size_t)k * cap + slot] = z; /* feature-major: coalesced */
+            }
+            sg.qenv[slot] = inside;
+            f_queued = 1;
+            f_plastic = 1;
+            /* stress and the rest are finished by the post kernel */
+        }
+    }
+    bool capture = false;
+    CRResult cr;
+    if (live && !queue) {
         if (!valid && VAR != VAR_SP) {
             /* DP:226-230, DPC:491-495: message and skip */
             for (int k = 0; k < 6; k++) f_nonfinite |= !isfinite(sig[k]);
-        } else if (!valid) {
-            /* SP with no material: the per-particle defaults (elastic,
-             * E = nu = 0, MaterialPoint_BC.h:48-57) */
-            CRResult cr;
-            trial_increment(0.0, 0.0, de, cr.dsig);
-            for (int k = 0; k < 6; k++) cr.depsp[k] = 0.0;
-            cr.dR = 0.0;
-            cr.yielded = false;
-            f_nonfinite |= finish_update_new_stress(P, i, sig, cr, R, V, de);
         } else {
-            bool queued = false;
-            if (MODE == MODE_SURROGATE && mat == sg.material) {
-                double dtr[6], J2, K;
-                trial_state(mt, mat, R, sig, de, dtr, J2, K);
-                if (!(J2 <= K * K)) {
-                    /* queue for the network: exact strain increment + normalised features */
-                    uint32_t slot = wave_append_u32(sg.qcount);
-                    sg.qid[slot] = i;
-                    const size_t cap = (size_t)P.n;
-                    float feat[MLP_IN];
-                    for (int k = 0; k < 6; k++) feat[k] = (float)(sig[k] / sg.stress_scale);
-                    for (int k = 0; k < 6; k++) feat[6 + k] = (float)(de[k] / sg.strain_scale);
-                    feat[12] = (float)(R / sg.stress_scale);
-                    uint8_t inside = 1;
-                    for (int k = 0; k < MLP_IN; k++) {
-                        float z = (feat[k] - sg.norm.in_mean[k]) / sg.norm.in_std[k];
-                        inside &= (z >= sg.norm.in_lo[k] && z <= sg.norm.in_hi[k]) ? 1 : 0;
-                        sg.qx[(size_t)k * cap + slot] = z; /* feature-major: coalesced */
-                    }
-                    sg.qenv[slot] = inside;
-                    queued = true;
-                    f_queued = 1;
-                    f_plastic = 1;
-                    /* stress and the rest are finished by the post kernel */
-                }
-            }
-            if (!queued) {
-                CRResult cr;
+            if (!valid) {
+                /* SP with no material: the per-particle defaults (elastic,
+                 * E = nu = 0, MaterialPoint_BC.h:48-57) */
+                trial_increment(0.0, 0.0, de, cr.dsig);
+                for (int k = 0; k < 6; k++) cr.depsp[k] = 0.0;
+                cr.dR = 0.0;
+                cr.yielded = false;
+            } else {
                 constitutive(mt, mat, R, sig, de, cr);
                 f_plastic = cr.yielded;
-                if (MODE == MODE_CAPTURE && cr.yielded) {
-                    unsigned long long slot = wave_append_u64(sg.capcount);
-                    if (slot < (unsigned long long)sg.capcap) {
-                        double *row = sg.cap + slot * 14;
-                        for (int k = 0; k < 6; k++) row[k] = sig[k];
-                        for (int k = 0; k < 6; k++) row[6 + k] = de[k];
-                        row[12] = R;
-                        row[13] = (double)mat;
-                    }
-                }
-                if (VAR == VAR_DP)
-                    f_nonfinite |= finish_update(P, i, de, sig, cr, R, V, dt);
-                else
-                    f_nonfinite |= finish_update_new_stress(P, i, sig, cr, R, V, de);
+                capture = MODE == MODE_CAPTURE && cr.yielded;
             }
         }
+    }
+    if (MODE == MODE_CAPTURE) {
+        const unsigned long long slot = block_append<unsigned long long>(sg.capcount, capture);
+        if (capture && slot < (unsigned long long)sg.capcap) {
+            double *row = sg.cap + slot * 14;
+            for (int k = 0; k < 6; k++) row[k] = sig[k];
+            for (int k = 0; k < 6; k++) row[6 + k] = de[k];
+            row[12] = R;
+            row[13] = (double)mat;
+        }
+    }
+    if (live && !queue && (valid || VAR == VAR_SP)) {
+        if (VAR == VAR_DP)
+            f_nonfinite |= finish_update(P, i, de, sig, cr, R, V, dt, acc);
+        else
+            f_nonfinite |= finish_update_new_stress(P, i, sig, cr, R, V, de, acc);
     }
     block_count_add(&health[HEALTH_NULL], f_null);
     block_count_add(&health[HEALTH_PLASTIC], f_plastic);
@@ -1510,10 +1724,12 @@ __global__ __launch_bounds__(BLOCK) void k_surrogate_post(Batch bt, PView P, Mat
                 constitutive(mt, m, R, sig, de, cr);
                 f_fallback = 1;
             }
+            PAccum acc;
+            load_accum(P, i, bt.variant == VAR_DP, acc);
             if (bt.variant == VAR_DP)
-                f_nonfinite = finish_update(P, i, de, sig, cr, R, P.V[i], bt.inst[inst_of_particle(bt, i)].dt);
+                f_nonfinite = finish_update(P, i, de, sig, cr, R, P.V[i], bt.inst[inst_of_particle(bt, i)].dt, acc);
             else
-                f_nonfinite = finish_update_new_stress(P, i, sig, cr, R, P.V[i], de);
+                f_nonfinite = finish_update_new_stress(P, i, sig, cr, R, P.V[i], de, acc);
         }
         block_count_add(&health[HEALTH_SURROGATE_FALLBACKS], f_fallback);
         block_count_add(&health[HEALTH_NONFINITE], f_nonfinite);
@@ -1676,13 +1892,14 @@ hipError_t launch_keys(const Batch &bt, const PView &P, const BView &B, hipStrea
 }
 
 int64_t scan_block_count(int64_t necells) { return (necells + SCAN_TILE - 1) / SCAN_TILE; }
+int64_t scan_padded_cells(int64_t necells) { return std::max<int64_t>(scan_block_count(necells), 1) * SCAN_TILE; }
 
 hipError_t launch_bin(const PView &P, const BView &B, uint64_t *block_sum, hipStream_t s) {
-    int64_t nb = scan_block_count(B.necells);
-    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(BLOCK), 0, s, B.count, B.necells, block_sum);
+    const int64_t nb = scan_padded_cells(B.necells) / SCAN_TILE;
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(BLOCK), 0, s, B.count, block_sum);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(BLOCK), 0, s, block_sum, nb);
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(BLOCK), 0, s, B.count, B.necells, block_sum, B.start,
-                       B.slot, B.occ, B.nocc);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(BLOCK), 0, s, B.count, block_sum, B.start, B.slot,
+                       B.occ, B.nocc);
     if (P.n > 0) hipLaunchKernelGGL(k_scatter, dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, P, B);
     hipLaunchKernelGGL(k_cell_sort, dim3((unsigned)std::min<int64_t>(blocks_for(std::max<int64_t>(P.n, 1), BLOCK), 4096)),
                        dim3(BLOCK), 0, s, B);
@@ -1696,11 +1913,26 @@ static unsigned partial_grid(int64_t n) {
     return (unsigned)std::min<int64_t>(b, 8192);
 }
 
+/* Launches KERNEL with M = whether the batch has more than one instance. */
+#define LAUNCH_MULTI(bt, KERNEL, ...)                                   \
+    do {                                                                 \
+        if ((bt).ninst > 1) {                                            \
+            constexpr bool M = true;                                     \
+            hipLaunchKernelGGL(KERNEL, __VA_ARGS__);                     \
+        } else {                                                         \
+            constexpr bool M = false;                                    \
+            hipLaunchKernelGGL(KERNEL, __VA_ARGS__);                     \
+        }                                                                \
+    } while (0)
+
 template <int VAR>
 static void launch_g2p_strain(int mode, dim3 grid, hipStream_t s, const Batch &bt, const PView &P, const NView &N,
                               const BView &B, const MatTable &mt, unsigned long long *health, const SurrogateArgs &sg,
                               hipEvent_t mid) {
-    hipLaunchKernelGGL((k_g2p_kinematics<VAR>), grid, dim3(BLOCK), 0, s, bt, P, N, B, health);
+    if (bt.ninst > 1)
+        hipLaunchKernelGGL((k_g2p_kinematics<VAR, true>), grid, dim3(BLOCK), 0, s, bt, P, N, B, health);
+    else
+        hipLaunchKernelGGL((k_g2p_kinematics<VAR, false>), grid, dim3(BLOCK), 0, s, bt, P, N, B, health);
     if (mid) (void)hipEventRecord(mid, s);
     if (mode == MODE_SURROGATE)
         hipLaunchKernelGGL((k_g2p_constitutive<MODE_SURROGATE, VAR>), grid, dim3(BLOCK), 0, s, bt, P, mt, health, sg);
@@ -1721,10 +1953,10 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
         }
         if (order == ORDER_CELL) {
             if (P.n > 0)
-                hipLaunchKernelGGL((k_cell_partials<false, false>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt, P,
-                                   N, B, C);
+                LAUNCH_MULTI(bt, (k_cell_partials<false, false, M>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt,
+                             P, N, B, C);
             if (mid) (void)hipEventRecord(mid, s);
-            hipLaunchKernelGGL(k_node_cells<false>, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
+            LAUNCH_MULTI(bt, (k_node_cells<false, M>), dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
         } else {
             hipLaunchKernelGGL(k_p2g_grid, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B);
             if (mid) (void)hipEventRecord(mid, s);
@@ -1734,7 +1966,7 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
         /* in the cell order of DoublePass and its contact variant the remap's
          * partials kernel runs this pass */
         if (P.n > 0 && !(order == ORDER_CELL && bt.variant != VAR_SP))
-            hipLaunchKernelGGL(k_g2p_velocity, dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, bt, P, N);
+            LAUNCH_MULTI(bt, (k_g2p_velocity<M>), dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, bt, P, N);
         if (mid) (void)hipEventRecord(mid, s);
         break;
     case PHASE_REMAP_GRID:
@@ -1745,14 +1977,14 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
         if (order == ORDER_CELL) {
             if (P.n > 0) {
                 if (bt.variant != VAR_SP)
-                    hipLaunchKernelGGL((k_cell_partials<true, true>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt,
-                                       P, N, B, C);
+                    LAUNCH_MULTI(bt, (k_cell_partials<true, true, M>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt,
+                                 P, N, B, C);
                 else
-                    hipLaunchKernelGGL((k_cell_partials<true, false>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt,
-                                       P, N, B, C);
+                    LAUNCH_MULTI(bt, (k_cell_partials<true, false, M>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt,
+                                 P, N, B, C);
             }
             if (mid) (void)hipEventRecord(mid, s);
-            hipLaunchKernelGGL(k_node_cells<true>, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
+            LAUNCH_MULTI(bt, (k_node_cells<true, M>), dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
         } else {
             hipLaunchKernelGGL(k_remap_grid, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B);
             if (mid) (void)hipEventRecord(mid, s);
