@@ -36,9 +36,12 @@ PEAK_HBM_GBS = 8000.0
 #   remap_nodes: per active node reads m p3, writes v3 p/m3 (10 words)
 #   g2p_kinematics: reads x3 F9 V0, writes x3 F9 V (26 words) + material id
 #     and control flags (5 B); per active node reads p/m3 v3 m (7 words)
-#   g2p_constitutive: reads and writes sigma6 eps6 epsp6 R W4 (46 words)
+#   g2p_constitutive: reads sigma6 eps6 R Ws Wti Wmax and writes sigma6
+#     eps6 Ws Wti (30 words); a point that yields also reads and writes
+#     epsp6 Wp and writes R (15 words; elsewhere they are unchanged)
 PER_PARTICLE = {"p2g_partials": 14 * 8, "p2g_nodes": 0, "g2p_velocity": 9 * 8, "remap_partials": 7 * 8,
-                "remap_nodes": 0, "g2p_kinematics": 26 * 8 + 5, "g2p_constitutive": 46 * 8, "bin": 0}
+                "remap_nodes": 0, "g2p_kinematics": 26 * 8 + 5, "g2p_constitutive": 30 * 8, "bin": 0}
+PER_PLASTIC = {"g2p_constitutive": 15 * 8}
 PER_NODE = {"p2g_partials": 0, "p2g_nodes": 20 * 8, "g2p_velocity": 4 * 8, "remap_partials": 0,
             "remap_nodes": 10 * 8, "g2p_kinematics": 7 * 8, "g2p_constitutive": 0, "bin": 0}
 FUSED_CELL = {"remap_partials": (10 * 8, 4 * 8)}
@@ -235,7 +238,7 @@ def main():
         per_p, per_n = PER_PARTICLE[k], PER_NODE[k]
         if args.order == "cell" and k in FUSED_CELL:
             per_p, per_n = FUSED_CELL[k]
-        b = per_p * scene.n + per_n * active
+        b = per_p * scene.n + per_n * active + round(PER_PLASTIC.get(k, 0) * plastic_frac * scene.n)
         t = [measured.get(kn) for kn in kernels[k].split("+")]
         per_kernel[k] = dict(ms=round(ms, 4), kernel=kernels[k], algorithmic_bytes=b,
                              gbs=round(b / (ms * 1e-3) / 1e9, 1) if (b and ms > 0) else None,

@@ -208,7 +208,7 @@ struct mpm_engine {
             free_parts();
             return rc;
         }
-        parts.cap8 = (int64_t)c8;
+        parts.cap = cap;
         parts_cap = cap;
         drop_graphs();
         return MPM_OK;

@@ -84,15 +84,14 @@ enum { ORDER_PARTICLE = 0, ORDER_CELL = 1 };
 /* step loops: DoublePass, DoublePass_Contact, SinglePass (same codes as MPM_VARIANT_*) */
 enum { VAR_DP = 0, VAR_DPC = 1, VAR_SP = 2 };
 
-/* Per (occupied cell, corner) partial sums of the cell order, component
- * stride cap8 = 8 x occupied-cell capacity: P2G m, p3, f3; the remap reuses
- * f for velocity; dc = id of the latest controlled particle per axis. */
-/* Cell-order partial sums: one record per (occupied cell, corner), item
- * t = 8 * occupied rank + corner; 8 doubles (m, p, f, pad) in the P2G pass,
- * 4 (v, pad) in the remap pass. */
+/* Cell-order partial sums: one record per (occupied cell, corner), stored
+ * corner-major, item t = corner * cap + occupied rank, so that neighbouring
+ * nodes, which read the same neighbour index of neighbouring cells, read
+ * neighbouring records; 8 doubles (m, p, f, pad) in the P2G pass, 4 (v, pad)
+ * in the remap pass. */
 struct CellParts {
     double *w;
-    int64_t cap8;
+    int64_t cap; /* occupied-cell capacity */
 };
 
 /* slot[] of a cell: its rank among occupied cells, MPM_NONE when empty,

@@ -117,6 +117,31 @@ __device__ __forceinline__ uint32_t cell_key(const GridGeom &g, double x, double
     return (uint32_t)c[0] + (uint32_t)g.ec[0] * ((uint32_t)c[1] + (uint32_t)g.ec[1] * (uint32_t)c[2]);
 }
 
+/* A slot in an append-only queue, one atomic per block: lanes with want
+ * set take consecutive slots in thread order after the block's base.  Every
+ * thread of the block must call it (it synchronises the block). */
+template <typename T>
+__device__ __forceinline__ T block_append(T *counter, bool want) {
+    __shared__ T s_off[BLOCK / 64], s_base;
+    const uint64_t mask = __ballot(want);
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    if (lane == 0) s_off[wave] = (T)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T tot = 0;
+        for (int w = 0; w < BLOCK / 64; w++) {
+            const T c = s_off[w];
+            s_off[w] = tot;
+            tot += c;
+        }
+        s_base = tot ? atomicAdd(counter, tot) : (T)0;
+    }
+    __syncthreads();
+    const T slot = s_base + s_off[wave] + (T)__popcll(mask & ((1ull << lane) - 1ull));
+    __syncthreads();
+    return slot;
+}
+
 /* Runs of equal keys over adjacent lanes of a wave: each lane's run head
  * (first lane) and run length, from one ballot.  Neighbouring particle ids
  * mostly share a cell, so a wave holds few runs and their heads can issue
@@ -365,6 +390,12 @@ __device__ __forceinline__ uint32_t merge_min(const Merge8 &M) {
     return min(a, b);
 }
 
+/* Inverse of node_of_tile: the instance-local tile holding node (ix, iy, iz). */
+__device__ __forceinline__ uint32_t tile_of_node(const Batch &bt, const GridGeom &g, int ix, int iy, int iz) {
+    const uint32_t tnx = (uint32_t)((g.nn[0] + bt.tx - 1) / bt.tx), tny = (uint32_t)((g.nn[1] + bt.ty - 1) / bt.ty);
+    return (uint32_t)(ix / bt.tx) + tnx * ((uint32_t)(iy / bt.ty) + tny * (uint32_t)(iz / bt.tz));
+}
+
 __device__ __forceinline__ void merge_advance(Merge8 &M, const BView &B, uint32_t id) {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
@@ -387,11 +418,6 @@ __device__ __forceinline__ void node_of_tile(const Batch &bt, const GridGeom &g,
     iz = tz * bt.tz + (int)(threadIdx.x / (TX * TY));
 }
 
-/* Inverse of node_of_tile: the instance-local tile holding node (ix, iy, iz). */
-__device__ __forceinline__ uint32_t tile_of_node(const Batch &bt, const GridGeom &g, int ix, int iy, int iz) {
-    const uint32_t tnx = (uint32_t)((g.nn[0] + bt.tx - 1) / bt.tx), tny = (uint32_t)((g.nn[1] + bt.ty - 1) / bt.ty);
-    return (uint32_t)(ix / bt.tx) + tnx * ((uint32_t)(iy / bt.ty) + tny * (uint32_t)(iz / bt.tz));
-}
 
 /* Track, per axis, the displacement-controlled particle latest in the
  * control list among those touching the node (the last writer of
@@ -742,18 +768,19 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd,
  * adjacent nodes in GridPoint_Mediator order (x, then y, then z innermost),
  * DP:490-510. */
 template <class REC>
-__device__ __forceinline__ void g2p_velocity_update_rec(const Inst &I, REC rec, const double x[3], double v[3]) {
+__device__ __forceinline__ void g2p_velocity_update_rec(const Inst &I, REC rec, const double x[3], double v[3],
+                                                        double s[3][2]) {
     const GridGeom &g = I.g;
     /* grid fields read once, outside any branch */
     const double gL[3] = {g.L[0], g.L[1], g.L[2]}, gh[3] = {g.h[0], g.h[1], g.h[2]};
     const int gc[3] = {g.cells[0], g.cells[1], g.cells[2]}, gn[3] = {g.nn[0], g.nn[1], g.nn[2]};
     int b[3], lo[3], hi[3];
-    double s[3][2];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         base_cell(x[a], gh[a], gn[a], b[a], lo[a], hi[a]);
 #pragma unroll
-        for (int o = 0; o < 2; o++) s[a][o] = axis_shape(x[a], node_coord(b[a] + o, gL[a], gc[a]), gh[a]).s;
+        for (int o = 0; o < 2; o++)
+            s[a][o] = axis_shape(x[a], node_coord(b[a] + o, gL[a], gc[a]), gh[a]).s;
     }
     bool ok[3][2];
 #pragma unroll
@@ -789,7 +816,10 @@ __device__ __forceinline__ void g2p_velocity_update_rec(const Inst &I, REC rec, 
     }
 }
 
-__device__ __forceinline__ void g2p_velocity_update(const Inst &I, const NView &Nd, const double x[3], double v[3]) {
+/* s receives the shape factors of the particle's base-cell corner nodes
+ * (axis a, offset o along it), which the remap partials reuse. */
+__device__ __forceinline__ void g2p_velocity_update(const Inst &I, const NView &Nd, const double x[3], double v[3],
+                                                    double s[3][2]) {
     const double *g1 = Nd.g1;
     g2p_velocity_update_rec(I,
                             [=](int, size_t node, double2 &a, double2 &b) {
@@ -797,7 +827,7 @@ __device__ __forceinline__ void g2p_velocity_update(const Inst &I, const NView &
                                 a = r[0];
                                 b = r[1];
                             },
-                            x, v);
+                            x, v, s);
 }
 
 template <bool MULTI>
@@ -808,7 +838,8 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_velocity(Batch bt, PView P, NView
     const size_t n = P.n;
     const double x[3] = {P.x[i], P.x[n + i], P.x[2 * n + i]};
     double v[3] = {P.v[i], P.v[n + i], P.v[2 * n + i]};
-    g2p_velocity_update(inst_p<MULTI>(bt, i), Nd, x, v);
+    double s[3][2];
+    g2p_velocity_update(inst_p<MULTI>(bt, i), Nd, x, v, s);
     P.v[i] = v[0];
     P.v[n + i] = v[1];
     P.v[2 * n + i] = v[2];
@@ -929,26 +960,56 @@ __device__ __forceinline__ void node_remap_dispatch(const Batch &bt, const NodeI
 
 /* Each block takes groups of PART_CELLS consecutive occupied cells (8
  * lanes per cell, one per corner).  The group's particles are contiguous in
- * the sorted list; they are staged through LDS in chunks of PART_STAGE with
- * one coalesced pass, and every lane then sums its cell's particles for its
- * corner from LDS in ascending id, the same order and arithmetic as reading
- * them from memory. */
+ * the sorted list; they are staged through LDS in chunks with one coalesced
+ * pass, and every lane then sums its cell's particles for its corner from
+ * LDS in ascending id, the same order and arithmetic as reading them from
+ * memory.
+ *
+ * In the remap pass, what is common to a particle's 8 corners is computed
+ * once, by the thread that stages it: the six axis shape factors s[a][o] of
+ * its cell's corner nodes (axis a, offset o); corner (ox, oy, oz) takes
+ * N = s_x[ox] s_y[oy] s_z[oz], the product the lane would form.  The staged
+ * particle's base cell is its bin (floor(x / h), as the binning computed it
+ * from the same x), so its corners are exactly the lanes' nodes.  (P2G
+ * keeps the per-lane shapes: its shared data, with the twelve gradient
+ * quotients (s_j s_l) / h_k, does not fit LDS for a full staging chunk,
+ * and staging half chunks with those divisions in one thread per particle
+ * measured slower.) */
 #define PART_CELLS (BLOCK / 8)
 #define PART_STAGE 256
 
+/* Shape factors s[a][o] of a particle's base-cell corner nodes (axis a,
+ * offset o), the values axis_shape gives each corner lane. */
+__device__ __forceinline__ void particle_factors(const GridGeom &g, const double x[3], double s[3][2]) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int b, lo, hi;
+        base_cell(x[a], g.h[a], g.nn[a], b, lo, hi);
+#pragma unroll
+        for (int o = 0; o < 2; o++)
+            s[a][o] = axis_shape(x[a], node_coord(b + o, g.L[a], g.cells[a]), g.h[a]).s;
+    }
+}
+
 /* G1 (remap pass of DoublePass and its contact variant): G2P pass 1 runs
  * here too, on each particle as it is staged (each particle is staged by
- * exactly one block), and the updated velocity is written back. */
+ * exactly one block), and the updated velocity is written back; its shape
+ * factors are the ones the remap lanes need. */
 template <bool REMAP, bool G1, bool MULTI>
 __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NView Nd, BView B, CellParts C) {
-    /* staged particle inputs: x, m, v, then (P2G) V, sigma, f_ext */
-    __shared__ double sx[3][PART_STAGE], sm[PART_STAGE], sv[3][PART_STAGE];
-    __shared__ double sV[REMAP ? 1 : PART_STAGE], ss[REMAP ? 1 : 6][REMAP ? 1 : PART_STAGE];
-    __shared__ double sf[REMAP ? 1 : 3][REMAP ? 1 : PART_STAGE];
-    __shared__ int sdc[PART_STAGE];
+    /* staged per particle: m v (3) and either the shape factors s (6, remap)
+     * or x (3, P2G), then for P2G m, V, sigma (6), f_ext (3); rows padded by
+     * one double so that the corner-dependent rows of one particle fall in
+     * different banks */
+    constexpr int PS = PART_STAGE, PR = PS + 1;
+    __shared__ double sS[REMAP ? 6 : 3][PR], spv[3][PR];
+    __shared__ double sm[REMAP ? 1 : PR], sV[REMAP ? 1 : PR], ss[REMAP ? 1 : 6][REMAP ? 1 : PR];
+    __shared__ double sf[REMAP ? 1 : 3][REMAP ? 1 : PR];
+    __shared__ int sdc[PS];
     const uint32_t nocc = *B.nocc;
     const size_t n = P.n;
     const int lc = threadIdx.x >> 3, corner = threadIdx.x & 7;
+    const int ox = corner >> 2, oy = (corner >> 1) & 1, oz = corner & 1;
     const bool skip_dc = REMAP ? bt.variant != VAR_DP : bt.variant == VAR_SP;
     const bool fext = !REMAP && bt.fext;
     for (uint32_t g0 = blockIdx.x * PART_CELLS; g0 < nocc; g0 += gridDim.x * PART_CELLS) {
@@ -958,66 +1019,49 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
         const uint32_t slot = g0 + (uint32_t)lc;
         bool have = slot < g1;
         uint32_t s0 = 0, s1 = 0;
-        int ix = 0, iy = 0, iz = 0;
-        const Inst *Ip = nullptr;
+        double mn = 0.0, xn[3] = {0.0, 0.0, 0.0}, hh[3] = {1.0, 1.0, 1.0};
+        bool massive = false;
         if (have) {
             const uint32_t c = B.occ[slot];
             s0 = B.start[c];
             s1 = B.start[c + 1];
-            Ip = &inst_p<MULTI>(bt, B.sorted[s0]);
-            const GridGeom &g = Ip->g;
-            const uint32_t e = c - (uint32_t)Ip->cell_off;
+            const Inst &I = inst_p<MULTI>(bt, B.sorted[s0]);
+            const GridGeom &g = I.g;
+            const uint32_t e = c - (uint32_t)I.cell_off;
             const int ex = (int)(e % (uint32_t)g.ec[0]);
             const int ey = (int)((e / (uint32_t)g.ec[0]) % (uint32_t)g.ec[1]);
             const int ez = (int)(e / ((uint32_t)g.ec[0] * (uint32_t)g.ec[1]));
-            ix = ex - 1 + (corner >> 2);
-            iy = ey - 1 + ((corner >> 1) & 1);
-            iz = ez - 1 + (corner & 1);
+            const int ix = ex - 1 + ox, iy = ey - 1 + oy, iz = ez - 1 + oz;
             if (ix < 0 || iy < 0 || iz < 0 || ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) have = false;
-            if (!REMAP && have && bt.tflag) bt.tflag[Ip->tile_off + tile_of_node(bt, g, ix, iy, iz)] = 1;
-        }
-        double xn[3] = {0.0, 0.0, 0.0}, hh[3] = {1.0, 1.0, 1.0};
-        double mn = 0.0;
-        bool massive = false;
-        if (have) {
-            const GridGeom &g = Ip->g;
-            xn[0] = node_coord(ix, g.L[0], g.cells[0]);
-            xn[1] = node_coord(iy, g.L[1], g.cells[1]);
-            xn[2] = node_coord(iz, g.L[2], g.cells[2]);
-            hh[0] = g.h[0];
-            hh[1] = g.h[1];
-            hh[2] = g.h[2];
-            if (REMAP) {
-                const int64_t node =
-                    Ip->node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
+            if (REMAP && have) {
+                const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
                 mn = Nd.m[node];
-                massive = mn > Ip->mass_min;
+                massive = mn > I.mass_min;
+            }
+            if (!REMAP && have) {
+                xn[0] = node_coord(ix, g.L[0], g.cells[0]);
+                xn[1] = node_coord(iy, g.L[1], g.cells[1]);
+                xn[2] = node_coord(iz, g.L[2], g.cells[2]);
+                hh[0] = g.h[0];
+                hh[1] = g.h[1];
+                hh[2] = g.h[2];
+                if (bt.tflag) bt.tflag[I.tile_off + tile_of_node(bt, g, ix, iy, iz)] = 1;
             }
         }
         double m = 0.0, p[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0};
-        for (uint32_t base = r0; base < r1; base += PART_STAGE) {
-            const uint32_t top = min(base + (uint32_t)PART_STAGE, r1);
+        for (uint32_t base = r0; base < r1; base += PS) {
+            const uint32_t top = min(base + (uint32_t)PS, r1);
             __syncthreads();
             for (uint32_t q = base + threadIdx.x; q < top; q += BLOCK) {
                 const uint32_t id = B.sorted[q], j = q - base;
                 const double x[3] = {P.x[id], P.x[n + id], P.x[2 * n + id]};
                 double v[3] = {P.v[id], P.v[n + id], P.v[2 * n + id]};
+                const double mp = P.m[id];
                 const int dcr = (skip_dc || (G1 && bt.variant != VAR_DP)) ? P.dc_rank[id] : -1;
-                if (G1 && !(bt.variant != VAR_DP && dcr >= 0)) { /* DPC:309-310 */
-                    g2p_velocity_update(inst_p<MULTI>(bt, id), Nd, x, v);
-                    P.v[id] = v[0];
-                    P.v[n + id] = v[1];
-                    P.v[2 * n + id] = v[2];
-                }
-                sx[0][j] = x[0];
-                sx[1][j] = x[1];
-                sx[2][j] = x[2];
-                sm[j] = P.m[id];
-                sv[0][j] = v[0];
-                sv[1][j] = v[1];
-                sv[2][j] = v[2];
-                sdc[j] = skip_dc ? dcr : -1;
                 if (!REMAP) {
+#pragma unroll
+                    for (int a = 0; a < 3; a++) sS[a][j] = x[a];
+                    sm[j] = mp;
                     sV[j] = P.V[id];
 #pragma unroll
                     for (int k = 0; k < 6; k++) ss[k][j] = P.sig[k * n + id];
@@ -1026,7 +1070,27 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
                         sf[1][j] = P.fext[n + id];
                         sf[2][j] = P.fext[2 * n + id];
                     }
+                } else {
+                    double sh[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+                    const Inst &I = inst_p<MULTI>(bt, id);
+                    if (G1) {
+                        if (!(bt.variant != VAR_DP && dcr >= 0)) { /* DPC:309-310 */
+                            g2p_velocity_update(I, Nd, x, v, sh);
+                            P.v[id] = v[0];
+                            P.v[n + id] = v[1];
+                            P.v[2 * n + id] = v[2];
+                        }
+                    } else {
+                        particle_factors(I.g, x, sh);
+                    }
+#pragma unroll
+                    for (int a = 0; a < 3; a++)
+#pragma unroll
+                        for (int o = 0; o < 2; o++) sS[2 * a + o][j] = sh[a][o];
                 }
+#pragma unroll
+                for (int k = 0; k < 3; k++) spv[k][j] = mp * v[k];
+                sdc[j] = skip_dc ? dcr : -1;
             }
             __syncthreads();
             if (!have) continue;
@@ -1035,16 +1099,15 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
                 for (uint32_t q = a; q < b; q++) {
                     const uint32_t j = q - base;
                     if (sdc[j] >= 0) continue; /* SP:116-117, 157-158 */
-                    AxisShape ax = axis_shape(sx[0][j], xn[0], hh[0]);
-                    AxisShape ay = axis_shape(sx[1][j], xn[1], hh[1]);
-                    AxisShape az = axis_shape(sx[2][j], xn[2], hh[2]);
+                    AxisShape ax = axis_shape(sS[0][j], xn[0], hh[0]);
+                    AxisShape ay = axis_shape(sS[1][j], xn[1], hh[1]);
+                    AxisShape az = axis_shape(sS[2][j], xn[2], hh[2]);
                     double N, gx, gy, gz;
                     shape_full(ax, ay, az, hh, N, gx, gy, gz);
-                    const double mp = sm[j];
-                    m += N * mp;
-                    p[0] += N * (mp * sv[0][j]);
-                    p[1] += N * (mp * sv[1][j]);
-                    p[2] += N * (mp * sv[2][j]);
+                    m += N * sm[j];
+                    p[0] += N * spv[0][j];
+                    p[1] += N * spv[1][j];
+                    p[2] += N * spv[2][j];
                     const double V = sV[j];
                     f[0] += -V * (gx * ss[0][j] + gy * ss[3][j] + gz * ss[5][j]);
                     f[1] += -V * (gy * ss[1][j] + gx * ss[3][j] + gz * ss[4][j]);
@@ -1059,17 +1122,15 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
                 for (uint32_t q = a; q < b; q++) {
                     const uint32_t j = q - base;
                     if (sdc[j] >= 0) continue; /* DPC:358-359, SP:157-158 */
-                    double N = axis_shape(sx[0][j], xn[0], hh[0]).s * axis_shape(sx[1][j], xn[1], hh[1]).s *
-                               axis_shape(sx[2][j], xn[2], hh[2]).s;
-                    const double mp = sm[j];
-                    f[0] += N * (mp * sv[0][j]) / mn;
-                    f[1] += N * (mp * sv[1][j]) / mn;
-                    f[2] += N * (mp * sv[2][j]) / mn;
+                    const double N = sS[0 + ox][j] * sS[2 + oy][j] * sS[4 + oz][j];
+                    f[0] += N * spv[0][j] / mn;
+                    f[1] += N * spv[1][j] / mn;
+                    f[2] += N * spv[2][j] / mn;
                 }
             }
         }
         if (!have) continue;
-        const size_t t = (size_t)slot * 8u + (size_t)corner;
+        const size_t t = (size_t)corner * (size_t)C.cap + (size_t)slot;
         if (!REMAP) {
             /* one 64-byte record per (cell, corner): m, p, f */
             double2 *o = reinterpret_cast<double2 *>(C.w + t * 8u);
@@ -1152,7 +1213,7 @@ __global__ __launch_bounds__(BLOCK) void k_node_cells(Batch bt, PView P, NView N
         double2 ra[8], rb[8], rc[8], rd[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            const size_t j = sl[k] == MPM_NONE ? 0 : (size_t)(sl[k] & ~SLOT_DC) * 8u + (size_t)(7 - k);
+            const size_t j = sl[k] == MPM_NONE ? 0 : (size_t)(7 - k) * (size_t)C.cap + (size_t)(sl[k] & ~SLOT_DC);
             const double2 *q = reinterpret_cast<const double2 *>(C.w + j * 8u);
             ra[k] = q[0];
             rb[k] = q[1];
@@ -1176,7 +1237,7 @@ __global__ __launch_bounds__(BLOCK) void k_node_cells(Batch bt, PView P, NView N
         double2 ra[8], rb[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            const size_t j = sl[k] == MPM_NONE ? 0 : (size_t)(sl[k] & ~SLOT_DC) * 8u + (size_t)(7 - k);
+            const size_t j = sl[k] == MPM_NONE ? 0 : (size_t)(7 - k) * (size_t)C.cap + (size_t)(sl[k] & ~SLOT_DC);
             const double2 *q = reinterpret_cast<const double2 *>(C.w + j * 4u);
             ra[k] = q[0];
             rb[k] = q[1];
@@ -1213,9 +1274,18 @@ __device__ __forceinline__ void block_count_add(unsigned long long *dst, uint32_
 /* The accumulated fields the state update reads and writes back.  They
  * are read with the rest of the particle's state, before the kernel's
  * first store: a read-modify-write in place would make each load wait on
- * the store before it (the compiler cannot prove the arrays apart). */
+ * the store before it (the compiler cannot prove the arrays apart).
+ *
+ * The plastic strain, the plastic work and the hardening variable change
+ * only when the point yields: otherwise dεp = +0, dR = +0 and the plastic
+ * work increment is ±0, and adding them gives back the stored doubles (a
+ * stored -0.0 would become +0.0 in the reference; these fields start at
+ * +0.0 and a sum is -0.0 only when both terms are, so that needs a caller
+ * to store -0.0 there).  Those three are read and written only for points
+ * that yield or whose plastic work increment is not zero (NaN included),
+ * which is most of the state update's traffic on an elastic step. */
 struct PAccum {
-    double eps[6], epsp[6], Ws, Wti, Wmax, Wp;
+    double eps[6], Ws, Wti, Wmax;
 };
 
 /* DP reads all of them; the contact and single-pass updates (new stress
@@ -1225,12 +1295,25 @@ __device__ __forceinline__ void load_accum(const PView &P, size_t i, bool dp, PA
     const size_t n = P.n;
 #pragma unroll
     for (int k = 0; k < 6; k++) a.eps[k] = P.eps[k * n + i];
-#pragma unroll
-    for (int k = 0; k < 6; k++) a.epsp[k] = P.epsp[k * n + i];
     a.Ws = P.Ws[i];
-    a.Wp = P.Wp[i];
     a.Wti = dp ? P.Wti[i] : 0.0;
     a.Wmax = dp ? P.Wmax[i] : 0.0;
+}
+
+/* The plastic fields, when they change: reads εp and Wp, writes εp + dεp,
+ * Wp + wp and R + dR. */
+__device__ __forceinline__ void update_plastic(const PView &P, size_t i, const CRResult &cr, double R, double wp) {
+    const size_t n = P.n;
+    const bool change = cr.yielded || !(wp == 0.0) || !(cr.dR == 0.0);
+    if (!change) return;
+    double epsp[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) epsp[k] = P.epsp[k * n + i];
+    const double Wp = P.Wp[i];
+#pragma unroll
+    for (int k = 0; k < 6; k++) P.epsp[k * n + i] = epsp[k] + cr.depsp[k];
+    P.Wp[i] = Wp + wp;
+    P.R[i] = R + cr.dR;
 }
 
 /* State update after the return map (DP:242-278 minus F and V, which the
@@ -1240,54 +1323,27 @@ __device__ __forceinline__ uint32_t finish_update(const PView &P, size_t i, cons
                                                   const PAccum &a) {
     const size_t n = P.n;
 #pragma unroll
-    for (int k = 0; k < 6; k++) P.eps[k * n + i] = a.eps[k] + de[k];
-#pragma unroll
     for (int k = 0; k < 6; k++) sig[k] += cr.dsig[k];
-#pragma unroll
-    for (int k = 0; k < 6; k++) P.sig[k * n + i] = sig[k];
-#pragma unroll
-    for (int k = 0; k < 6; k++) P.epsp[k * n + i] = a.epsp[k] + cr.depsp[k];
     double wi = 0.0;
 #pragma unroll
     for (int k = 0; k < 6; k++) wi += (sig[k] - 0.5 * cr.dsig[k]) * de[k] * V;
-    double Ws = a.Ws + wi;
-    P.Ws[i] = Ws;
-    P.Wti[i] = a.Wti + Ws * dt;
-    if (Ws > a.Wmax) P.Wmax[i] = Ws;
+    const double Ws = a.Ws + wi;
     double wp = 0.0;
 #pragma unroll
     for (int k = 0; k < 6; k++) wp += (sig[k] - 0.5 * cr.dsig[k]) * cr.depsp[k] * V;
-    P.Wp[i] = a.Wp + wp;
-    P.R[i] = R + cr.dR;
+    update_plastic(P, i, cr, R, wp);
+#pragma unroll
+    for (int k = 0; k < 6; k++) P.eps[k * n + i] = a.eps[k] + de[k];
+#pragma unroll
+    for (int k = 0; k < 6; k++) P.sig[k * n + i] = sig[k];
+    P.Ws[i] = Ws;
+    P.Wti[i] = a.Wti + Ws * dt;
+    if (Ws > a.Wmax) P.Wmax[i] = Ws;
     uint32_t bad = 0;
     for (int k = 0; k < 6; k++) bad |= !isfinite(sig[k]);
     return bad;
 }
 
-/* A slot in an append-only queue, one atomic per block: lanes with want
- * set take consecutive slots in thread order after the block's base.  Every
- * thread of the block must call it (it synchronises the block). */
-template <typename T>
-__device__ __forceinline__ T block_append(T *counter, bool want) {
-    __shared__ T s_off[BLOCK / 64], s_base;
-    const uint64_t mask = __ballot(want);
-    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
-    if (lane == 0) s_off[wave] = (T)__popcll(mask);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        T tot = 0;
-        for (int w = 0; w < BLOCK / 64; w++) {
-            const T c = s_off[w];
-            s_off[w] = tot;
-            tot += c;
-        }
-        s_base = tot ? atomicAdd(counter, tot) : (T)0;
-    }
-    __syncthreads();
-    const T slot = s_base + s_off[wave] + (T)__popcll(mask & ((1ull << lane) - 1ull));
-    __syncthreads();
-    return slot;
-}
 
 /* Stress, plastic strain, energies and backstress of the contact and
  * single-pass loops (DPC:514-524, SP:351-362): energies accumulate
@@ -1299,19 +1355,35 @@ __device__ __forceinline__ uint32_t finish_update_new_stress(const PView &P, siz
     const size_t n = P.n;
 #pragma unroll
     for (int k = 0; k < 6; k++) sig[k] += cr.dsig[k];
-#pragma unroll
-    for (int k = 0; k < 6; k++) P.sig[k * n + i] = sig[k];
-#pragma unroll
-    for (int k = 0; k < 6; k++) P.epsp[k * n + i] = a.epsp[k] + cr.depsp[k];
     double Ws = a.Ws;
 #pragma unroll
     for (int k = 0; k < 6; k++) Ws += sig[k] * de[k] * V;
-    P.Ws[i] = Ws;
-    double Wp = a.Wp;
+    /* the plastic work increment, added to Wp one component at a time as
+     * the reference does (Wp + t0 + t1 ...), is formed here as a sum of the
+     * terms only to decide whether any is non-zero; update_plastic is given
+     * the exact sequential result below */
+    double t[6];
+    bool any = false;
 #pragma unroll
-    for (int k = 0; k < 6; k++) Wp += sig[k] * cr.depsp[k] * V;
-    P.Wp[i] = Wp;
-    P.R[i] = R + cr.dR;
+    for (int k = 0; k < 6; k++) {
+        t[k] = sig[k] * cr.depsp[k] * V;
+        any |= !(t[k] == 0.0);
+    }
+    if (cr.yielded || any || !(cr.dR == 0.0)) {
+        double epsp[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) epsp[k] = P.epsp[k * n + i];
+        double Wp = P.Wp[i];
+#pragma unroll
+        for (int k = 0; k < 6; k++) Wp += t[k];
+#pragma unroll
+        for (int k = 0; k < 6; k++) P.epsp[k * n + i] = epsp[k] + cr.depsp[k];
+        P.Wp[i] = Wp;
+        P.R[i] = R + cr.dR;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) P.sig[k * n + i] = sig[k];
+    P.Ws[i] = Ws;
     uint32_t bad = 0;
     for (int k = 0; k < 6; k++) bad |= !isfinite(sig[k]);
     return bad;
@@ -1376,7 +1448,8 @@ __device__ __forceinline__ void g2p_kinematics_particle(const Batch &bt, const P
     for (int a = 0; a < 3; a++) {
         base_cell(x0[a], hx[a], gn[a], b[a], lo[a], hi[a]);
 #pragma unroll
-        for (int o = 0; o < 2; o++) as[a][o] = axis_shape(x0[a], node_coord(b[a] + o, g.L[a], g.cells[a]), hx[a]);
+        for (int o = 0; o < 2; o++)
+            as[a][o] = axis_shape(x0[a], node_coord(b[a] + o, g.L[a], g.cells[a]), hx[a]);
     }
     double L[3][3]; /* glm column-major L[col][row] */
 #pragma unroll
@@ -1572,11 +1645,6 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_constitutive(Batch bt, PView P, M
         R = P.R[i];
         f_nonfinite = P.flag[i] & 1u;
         load_accum(P, i, VAR == VAR_DP, acc);
-        if (VAR != VAR_DP) {
-            /* the strain is updated before the material check */
-#pragma unroll
-            for (int k = 0; k < 6; k++) P.eps[k * n + i] = acc.eps[k] + de[k];
-        }
         if (!valid) f_null = 1;
     }
     /* the learned map takes the points of its material whose trial state yields */
@@ -1643,6 +1711,12 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_constitutive(Batch bt, PView P, M
             f_nonfinite |= finish_update(P, i, de, sig, cr, R, V, dt, acc);
         else
             f_nonfinite |= finish_update_new_stress(P, i, sig, cr, R, V, de, acc);
+    }
+    if (live && VAR != VAR_DP) {
+        /* the strain is updated before the material check, for every point
+         * the pass visits (stored last: no read follows it) */
+#pragma unroll
+        for (int k = 0; k < 6; k++) P.eps[k * n + i] = acc.eps[k] + de[k];
     }
     block_count_add(&health[HEALTH_NULL], f_null);
     block_count_add(&health[HEALTH_PLASTIC], f_plastic);
