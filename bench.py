@@ -41,21 +41,26 @@ PEAK_HBM_GBS = 8000.0
 #     epsp6 Wp and writes R (15 words; elsewhere they are unchanged)
 PER_PARTICLE = {"p2g_partials": 14 * 8, "p2g_nodes": 0, "g2p_velocity": 9 * 8, "remap_partials": 7 * 8,
                 "remap_nodes": 0, "g2p_kinematics": 26 * 8 + 5, "g2p_constitutive": 30 * 8, "bin": 0}
-PER_PLASTIC = {"g2p_constitutive": 15 * 8}
+PER_PLASTIC = {"g2p_constitutive": 15 * 8, "g2p_kinematics": 15 * 8}
 PER_NODE = {"p2g_partials": 0, "p2g_nodes": 20 * 8, "g2p_velocity": 4 * 8, "remap_partials": 0,
             "remap_nodes": 10 * 8, "g2p_kinematics": 7 * 8, "g2p_constitutive": 0, "bin": 0}
 FUSED_CELL = {"remap_partials": (10 * 8, 4 * 8)}
+# With the exact return map and one instance, G2P pass 2 is one kernel
+# (k_g2p_fused), booked in the kinematics slot: its algorithmic bytes are
+# the kinematics' and the state update's together (the strain increment no
+# longer crosses memory), and the constitutive slot launches nothing.
+FUSED_G2P = {"g2p_kinematics": (26 * 8 + 5 + 30 * 8, 7 * 8)}
 # kernel of each slot, per summation order (names as rocprofv3 reports them)
 KERNELS = {
     "cell": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
              "p2g_partials": "k_cell_partials<false, false, false>", "p2g_nodes": "k_node_cells<false, false>",
              "g2p_velocity": None, "remap_partials": "k_cell_partials<true, true, false>",
-             "remap_nodes": "k_node_cells<true, false>", "g2p_kinematics": "k_g2p_kinematics<0, false>",
-             "g2p_constitutive": "k_g2p_constitutive<0, 0>"},
+             "remap_nodes": "k_node_cells<true, false>", "g2p_kinematics": "k_g2p_fused<0, false>",
+             "g2p_constitutive": None},
     "particle": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
                  "p2g_partials": "k_p2g_grid", "p2g_nodes": None, "g2p_velocity": "k_g2p_velocity<false>",
                  "remap_partials": "k_remap_grid", "remap_nodes": None,
-                 "g2p_kinematics": "k_g2p_kinematics<0, false>", "g2p_constitutive": "k_g2p_constitutive<0, 0>"},
+                 "g2p_kinematics": "k_g2p_fused<0, false>", "g2p_constitutive": None},
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_honeycomb8m.txt")
 
@@ -238,6 +243,8 @@ def main():
         per_p, per_n = PER_PARTICLE[k], PER_NODE[k]
         if args.order == "cell" and k in FUSED_CELL:
             per_p, per_n = FUSED_CELL[k]
+        if k in FUSED_G2P:
+            per_p, per_n = FUSED_G2P[k]
         b = per_p * scene.n + per_n * active + round(PER_PLASTIC.get(k, 0) * plastic_frac * scene.n)
         t = [measured.get(kn) for kn in kernels[k].split("+")]
         per_kernel[k] = dict(ms=round(ms, 4), kernel=kernels[k], algorithmic_bytes=b,

@@ -1420,9 +1420,22 @@ __device__ __forceinline__ void trial_state(const MatTable &mt, int m, double R,
  * instance field and every particle field the pass needs is read before
  * its first store: reads after a store cannot be proven unaliased, become
  * vector loads and each waits on its own. */
-template <int VAR, class REC>
+/* What the kinematics hands to the state update. */
+struct KinOut {
+    double de[6]; /* strain increment */
+    double V;     /* volume after the pass */
+    double dt;
+    int mat;
+    uint32_t flag; /* position not finite */
+    bool ctl;      /* left out of the pass (controlled, contact and single-pass loops) */
+};
+
+/* HANDOFF: store the strain increment and the flag for k_g2p_constitutive
+ * (the fused pass keeps them in registers). */
+template <int VAR, bool HANDOFF, class REC>
 __device__ __forceinline__ void g2p_kinematics_particle(const Batch &bt, const PView &P, const Inst &I, uint32_t i,
-                                                        REC rec, uint32_t &next_key, uint32_t &f_outside) {
+                                                        REC rec, uint32_t &next_key, uint32_t &f_outside,
+                                                        KinOut &ko) {
     const size_t n = P.n;
     const GridGeom g = I.g;
     const double dt = I.dt, mass_min = I.mass_min;
@@ -1467,6 +1480,10 @@ __device__ __forceinline__ void g2p_kinematics_particle(const Batch &bt, const P
      * of the grid-to-particle pass (DPC:443-444, SP:279-280) */
     const bool ctl = VAR != VAR_DP && dcr >= 0;
     uint8_t flag = 0;
+    ko.ctl = ctl;
+    ko.mat = mat;
+    ko.dt = dt;
+    ko.V = 0.0;
     if (!ctl) {
         const size_t nx = (size_t)gn[0], ny = (size_t)gn[1];
         /* corners in GridPoint_Mediator order: x outer, z inner;
@@ -1530,7 +1547,10 @@ __device__ __forceinline__ void g2p_kinematics_particle(const Batch &bt, const P
             de[5] = dF[2][0] + dF[0][2];
         }
 #pragma unroll
-        for (int k = 0; k < 6; k++) P.de[k * n + i] = de[k];
+        for (int k = 0; k < 6; k++) {
+            ko.de[k] = de[k];
+            if (HANDOFF) P.de[k * n + i] = de[k];
+        }
         /* DP: F = F dF after the material check (DP:242-245); DPC / SP:
          * F += (L F) dt before it (DPC:464-467, SP:305-308) */
         if (VAR != VAR_DP || valid) {
@@ -1555,11 +1575,13 @@ __device__ __forceinline__ void g2p_kinematics_particle(const Batch &bt, const P
             double det = +Fn[0][0] * (Fn[1][1] * Fn[2][2] - Fn[2][1] * Fn[1][2]) -
                          Fn[1][0] * (Fn[0][1] * Fn[2][2] - Fn[2][1] * Fn[0][2]) +
                          Fn[2][0] * (Fn[0][1] * Fn[1][2] - Fn[1][1] * Fn[0][2]);
-            P.V[i] = det * V0;
+            ko.V = det * V0;
+            P.V[i] = ko.V;
         }
         flag = !(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2])) ? 1 : 0;
     }
-    P.flag[i] = flag;
+    ko.flag = flag;
+    if (HANDOFF) P.flag[i] = flag;
     /* displacement-control schedule: DP:766-789 (flagged axes), or
      * DPC:531-540 / SP:369-378 (all axes, then x += v dt) */
     if (VAR == VAR_DP) {
@@ -1599,7 +1621,8 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_kinematics(Batch bt, PView P, NVi
     uint32_t next_key = MPM_NONE;
     if (live) {
         const double *g2 = Nd.g2;
-        g2p_kinematics_particle<VAR>(bt, P, inst_p<MULTI>(bt, i), i,
+        KinOut ko;
+        g2p_kinematics_particle<VAR, true>(bt, P, inst_p<MULTI>(bt, i), i,
                                      [=](int, size_t node, double2 &r0, double2 &r1, double2 &r2, double2 &r3) {
                                          const double2 *q = reinterpret_cast<const double2 *>(g2 + node * 8u);
                                          r0 = q[0];
@@ -1607,7 +1630,7 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_kinematics(Batch bt, PView P, NVi
                                          r2 = q[2];
                                          r3 = q[3];
                                      },
-                                     next_key, f_outside);
+                                     next_key, f_outside, ko);
         B.key[i] = next_key;
     }
     wave_key_add(B.count, next_key, next_key != MPM_NONE);
@@ -1722,6 +1745,84 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_constitutive(Batch bt, PView P, M
     block_count_add(&health[HEALTH_PLASTIC], f_plastic);
     block_count_add(&health[HEALTH_NONFINITE], f_nonfinite);
     if (MODE == MODE_SURROGATE) block_count_add(&health[HEALTH_SURROGATE_CALLS], f_queued);
+}
+
+/* The exact state update of one particle that the pass visits, from the
+ * kinematics' outputs in registers: the same steps as k_g2p_constitutive's
+ * exact mode (material check, return map, finish, strain). */
+template <int VAR>
+__device__ __forceinline__ void state_update_particle(const PView &P, const MatTable &mt, uint32_t i, const KinOut &ko,
+                                                      uint32_t &f_null, uint32_t &f_plastic, uint32_t &f_nonfinite) {
+    const size_t n = P.n;
+    const int mat = ko.mat;
+    const bool valid = mat >= 0 && mat < mt.n;
+    double sig[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) sig[k] = P.sig[k * n + i];
+    const double R = P.R[i];
+    PAccum acc;
+    load_accum(P, i, VAR == VAR_DP, acc);
+    f_nonfinite = ko.flag;
+    if (!valid) f_null = 1;
+    if (!valid && VAR != VAR_SP) {
+        /* DP:226-230, DPC:491-495: message and skip */
+        for (int k = 0; k < 6; k++) f_nonfinite |= !isfinite(sig[k]);
+    } else {
+        CRResult cr;
+        if (!valid) {
+            /* SP with no material: the per-particle defaults (elastic,
+             * E = nu = 0, MaterialPoint_BC.h:48-57) */
+            trial_increment(0.0, 0.0, ko.de, cr.dsig);
+            for (int k = 0; k < 6; k++) cr.depsp[k] = 0.0;
+            cr.dR = 0.0;
+            cr.yielded = false;
+        } else {
+            constitutive(mt, mat, R, sig, ko.de, cr);
+            f_plastic = cr.yielded;
+        }
+        if (VAR == VAR_DP)
+            f_nonfinite |= finish_update(P, i, ko.de, sig, cr, R, ko.V, ko.dt, acc);
+        else
+            f_nonfinite |= finish_update_new_stress(P, i, sig, cr, R, ko.V, ko.de, acc);
+    }
+    if (VAR != VAR_DP) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) P.eps[k * n + i] = acc.eps[k] + ko.de[k];
+    }
+}
+
+/* G2P pass 2 in one kernel for the exact return map: kinematics, then the
+ * state update of the same particle, with the strain increment, the new
+ * volume and the flags kept in registers (the two-kernel form hands them
+ * over through memory; the surrogate and capture modes use it for their
+ * queues). */
+template <int VAR, bool MULTI>
+__global__ __launch_bounds__(BLOCK) void k_g2p_fused(Batch bt, PView P, NView Nd, BView B, MatTable mt,
+                                                    unsigned long long *health) {
+    uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    const bool live = i < P.n;
+    uint32_t f_outside = 0, f_null = 0, f_plastic = 0, f_nonfinite = 0;
+    uint32_t next_key = MPM_NONE;
+    if (live) {
+        const double *g2 = Nd.g2;
+        KinOut ko;
+        g2p_kinematics_particle<VAR, false>(bt, P, inst_p<MULTI>(bt, i), i,
+                                            [=](int, size_t node, double2 &r0, double2 &r1, double2 &r2, double2 &r3) {
+                                                const double2 *q = reinterpret_cast<const double2 *>(g2 + node * 8u);
+                                                r0 = q[0];
+                                                r1 = q[1];
+                                                r2 = q[2];
+                                                r3 = q[3];
+                                            },
+                                            next_key, f_outside, ko);
+        B.key[i] = next_key;
+        if (!ko.ctl) state_update_particle<VAR>(P, mt, i, ko, f_null, f_plastic, f_nonfinite);
+    }
+    wave_key_add(B.count, next_key, next_key != MPM_NONE);
+    block_count_add(&health[HEALTH_OUTSIDE], f_outside);
+    block_count_add(&health[HEALTH_NULL], f_null);
+    block_count_add(&health[HEALTH_PLASTIC], f_plastic);
+    block_count_add(&health[HEALTH_NONFINITE], f_nonfinite);
 }
 
 /* Applies the network's output to the queued points: the stress increment
@@ -2003,6 +2104,13 @@ template <int VAR>
 static void launch_g2p_strain(int mode, dim3 grid, hipStream_t s, const Batch &bt, const PView &P, const NView &N,
                               const BView &B, const MatTable &mt, unsigned long long *health, const SurrogateArgs &sg,
                               hipEvent_t mid) {
+    /* one instance, exact map: the fused pass (with a batch's per-particle
+     * instance lookups it no longer fits the registers of two waves) */
+    if (mode == MODE_EXACT && bt.ninst == 1) {
+        hipLaunchKernelGGL((k_g2p_fused<VAR, false>), grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health);
+        if (mid) (void)hipEventRecord(mid, s);
+        return;
+    }
     if (bt.ninst > 1)
         hipLaunchKernelGGL((k_g2p_kinematics<VAR, true>), grid, dim3(BLOCK), 0, s, bt, P, N, B, health);
     else
