@@ -1001,15 +1001,29 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
      * or x (3, P2G), then for P2G m, V, sigma (6), f_ext (3); rows padded by
      * one double so that the corner-dependent rows of one particle fall in
      * different banks */
-    constexpr int PS = PART_STAGE, PR = PS + 1;
+    constexpr int PS = PART_STAGE, PR = REMAP ? PS + 1 : PS;
     __shared__ double sS[REMAP ? 6 : 3][PR], spv[3][PR];
     __shared__ double sm[REMAP ? 1 : PR], sV[REMAP ? 1 : PR], ss[REMAP ? 1 : 6][REMAP ? 1 : PR];
     __shared__ double sf[REMAP ? 1 : 3][REMAP ? 1 : PR];
-    __shared__ int sdc[PS];
+    __shared__ int8_t sdc[PS];
+    /* P2G: what the 8 lanes of a cell share for the particle at hand: the
+     * six axis shapes (factor, gradient-taken and sign bits) and the twelve
+     * gradient quotients */
+    __shared__ double xs_s[REMAP ? 1 : PART_CELLS][6], xs_q[REMAP ? 1 : PART_CELLS][12];
+    __shared__ uint8_t xs_b[REMAP ? 1 : PART_CELLS][6];
     const uint32_t nocc = *B.nocc;
     const size_t n = P.n;
     const int lc = threadIdx.x >> 3, corner = threadIdx.x & 7;
     const int ox = corner >> 2, oy = (corner >> 1) & 1, oz = corner & 1;
+    /* P2G sharing: the axis-shape pair of lanes 0..5, and the (type, index)
+     * of the gradient quotients each lane computes (type 0 x, 1 y, 2 z; -1
+     * none); every quotient is computed by a lane that holds its factors */
+    const int pa = corner < 6 ? corner >> 1 : 0, po = corner & 1;
+    /* two bits per corner, 3 = none: first {x z z y y y z -}, second
+     * {y x x x z - - -} */
+    const int qt1 = (int)((0xE568u >> (2 * corner)) & 3u), qt2 = (int)((0xFE01u >> (2 * corner)) & 3u);
+    auto qindex = [&](int t) { return t == 0 ? 2 * oy + oz : (t == 1 ? 4 + 2 * ox + oz : 8 + 2 * ox + oy); };
+    const int qk1 = qindex(qt1), qk2 = qindex(qt2);
     const bool skip_dc = REMAP ? bt.variant != VAR_DP : bt.variant == VAR_SP;
     const bool fext = !REMAP && bt.fext;
     for (uint32_t g0 = blockIdx.x * PART_CELLS; g0 < nocc; g0 += gridDim.x * PART_CELLS) {
@@ -1017,11 +1031,12 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
         const uint32_t r0 = B.start[B.occ[g0]], r1 = B.start[B.occ[g1 - 1] + 1];
         /* this lane's (cell, corner) */
         const uint32_t slot = g0 + (uint32_t)lc;
-        bool have = slot < g1;
+        const bool cell = slot < g1;
+        bool have = cell;
         uint32_t s0 = 0, s1 = 0;
-        double mn = 0.0, xn[3] = {0.0, 0.0, 0.0}, hh[3] = {1.0, 1.0, 1.0};
+        double mn = 0.0, xpair = 0.0, hpair = 1.0, hx_ = 1.0, hy_ = 1.0, hz_ = 1.0;
         bool massive = false;
-        if (have) {
+        if (cell) {
             const uint32_t c = B.occ[slot];
             s0 = B.start[c];
             s1 = B.start[c + 1];
@@ -1033,20 +1048,24 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
             const int ez = (int)(e / ((uint32_t)g.ec[0] * (uint32_t)g.ec[1]));
             const int ix = ex - 1 + ox, iy = ey - 1 + oy, iz = ez - 1 + oz;
             if (ix < 0 || iy < 0 || iz < 0 || ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) have = false;
+            if (!REMAP) {
+                /* the node coordinate of this lane's shape pair: axis pa,
+                 * the cell's node pa-index e - 1 + po */
+                const int ep = pa == 0 ? ex : (pa == 1 ? ey : ez);
+                const double Lp = pa == 0 ? g.L[0] : (pa == 1 ? g.L[1] : g.L[2]);
+                const int cp = pa == 0 ? g.cells[0] : (pa == 1 ? g.cells[1] : g.cells[2]);
+                xpair = node_coord(ep - 1 + po, Lp, cp);
+                hx_ = g.h[0];
+                hy_ = g.h[1];
+                hz_ = g.h[2];
+                hpair = pa == 0 ? hx_ : (pa == 1 ? hy_ : hz_);
+            }
             if (REMAP && have) {
                 const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
                 mn = Nd.m[node];
                 massive = mn > I.mass_min;
             }
-            if (!REMAP && have) {
-                xn[0] = node_coord(ix, g.L[0], g.cells[0]);
-                xn[1] = node_coord(iy, g.L[1], g.cells[1]);
-                xn[2] = node_coord(iz, g.L[2], g.cells[2]);
-                hh[0] = g.h[0];
-                hh[1] = g.h[1];
-                hh[2] = g.h[2];
-                if (bt.tflag) bt.tflag[I.tile_off + tile_of_node(bt, g, ix, iy, iz)] = 1;
-            }
+            if (!REMAP && have && bt.tflag) bt.tflag[I.tile_off + tile_of_node(bt, g, ix, iy, iz)] = 1;
         }
         double m = 0.0, p[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0};
         for (uint32_t base = r0; base < r1; base += PS) {
@@ -1090,20 +1109,42 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
                 }
 #pragma unroll
                 for (int k = 0; k < 3; k++) spv[k][j] = mp * v[k];
-                sdc[j] = skip_dc ? dcr : -1;
+                sdc[j] = (skip_dc && dcr >= 0) ? 1 : 0;
             }
             __syncthreads();
-            if (!have) continue;
+            if (REMAP ? !have : !cell) continue;
             const uint32_t a = max(s0, base), b = min(s1, top);
             if (!REMAP) {
                 for (uint32_t q = a; q < b; q++) {
                     const uint32_t j = q - base;
-                    if (sdc[j] >= 0) continue; /* SP:116-117, 157-158 */
-                    AxisShape ax = axis_shape(sS[0][j], xn[0], hh[0]);
-                    AxisShape ay = axis_shape(sS[1][j], xn[1], hh[1]);
-                    AxisShape az = axis_shape(sS[2][j], xn[2], hh[2]);
-                    double N, gx, gy, gz;
-                    shape_full(ax, ay, az, hh, N, gx, gy, gz);
+                    if (sdc[j]) continue; /* SP:116-117, 157-158 */
+                    /* lanes 0..5 of the cell: the axis shape of pair (axis
+                     * corner >> 1, offset corner & 1), as axis_shape */
+                    if (corner < 6) {
+                        const AxisShape A = axis_shape(sS[pa][j], xpair, hpair);
+                        xs_s[lc][corner] = A.s;
+                        xs_b[lc][corner] = (uint8_t)((A.grad ? 1u : 0u) | (A.sgn > 0.0 ? 2u : 0u));
+                    }
+                    const double sx = xs_s[lc][0 + ox], sy = xs_s[lc][2 + oy], sz = xs_s[lc][4 + oz];
+                    const uint32_t bx = xs_b[lc][0 + ox], by = xs_b[lc][2 + oy], bz = xs_b[lc][4 + oz];
+                    /* the twelve quotients, at most two per lane, each from
+                     * the lane's own factors: x (s_y s_z) / h_x, y (s_x s_z)
+                     * / h_y, z (s_x s_y) / h_z */
+                    {
+                        const double pyz = sy * sz, pxz = sx * sz, pxy = sx * sy;
+                        const double n1 = qt1 == 0 ? pyz : (qt1 == 1 ? pxz : pxy), d1 = qt1 == 0 ? hx_ : (qt1 == 1 ? hy_ : hz_);
+                        const double n2 = qt2 == 0 ? pyz : (qt2 == 1 ? pxz : pxy), d2 = qt2 == 0 ? hx_ : (qt2 == 1 ? hy_ : hz_);
+                        if (qt1 != 3) xs_q[lc][qk1] = n1 / d1;
+                        if (qt2 != 3) xs_q[lc][qk2] = n2 / d2;
+                    }
+                    /* shape_full's gradient (-s_j s_l sgn) / h is exactly
+                     * -sgn ((s_j s_l) / h): sgn = +-1 where it is taken */
+                    const double qx = xs_q[lc][0 + 2 * oy + oz], qy = xs_q[lc][4 + 2 * ox + oz],
+                                 qz = xs_q[lc][8 + 2 * ox + oy];
+                    const double gx = (bx & 1u) ? ((bx & 2u) ? -qx : qx) : 0.0;
+                    const double gy = (by & 1u) ? ((by & 2u) ? -qy : qy) : 0.0;
+                    const double gz = (bz & 1u) ? ((bz & 2u) ? -qz : qz) : 0.0;
+                    const double N = sx * sy * sz;
                     m += N * sm[j];
                     p[0] += N * spv[0][j];
                     p[1] += N * spv[1][j];
@@ -1121,7 +1162,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
             } else if (massive) {
                 for (uint32_t q = a; q < b; q++) {
                     const uint32_t j = q - base;
-                    if (sdc[j] >= 0) continue; /* DPC:358-359, SP:157-158 */
+                    if (sdc[j]) continue; /* DPC:358-359, SP:157-158 */
                     const double N = sS[0 + ox][j] * sS[2 + oy][j] * sS[4 + oz][j];
                     f[0] += N * spv[0][j] / mn;
                     f[1] += N * spv[1][j] / mn;
