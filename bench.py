@@ -29,10 +29,7 @@ PEAK_HBM_GBS = 8000.0
 #   p2g_nodes: per active node writes m p3 f3 (7 words, the P2G flush) and
 #     reads fixed/friction and writes p3 f3 f/m3 reaction (13, grid update)
 #   g2p_velocity: reads x3 v3, writes v3; per active node reads f/m3 m
-#   remap_partials: reads x3 v3 m (7 words); in the cell order G2P pass 1
-#     runs inside it on each staged particle, which adds the v3 write (10
-#     words) and per active node the f/m3 m reads (4 words), and the
-#     g2p_velocity slot launches nothing
+#   remap_partials: reads x3 v3 m (7 words)
 #   remap_nodes: per active node reads m p3, writes v3 p/m3 (10 words)
 #   g2p_kinematics: reads x3 F9 V0, writes x3 F9 V (26 words) + material id
 #     and control flags (5 B); per active node reads p/m3 v3 m (7 words)
@@ -44,7 +41,6 @@ PER_PARTICLE = {"p2g_partials": 14 * 8, "p2g_nodes": 0, "g2p_velocity": 9 * 8, "
 PER_PLASTIC = {"g2p_constitutive": 15 * 8, "g2p_kinematics": 15 * 8}
 PER_NODE = {"p2g_partials": 0, "p2g_nodes": 20 * 8, "g2p_velocity": 4 * 8, "remap_partials": 0,
             "remap_nodes": 10 * 8, "g2p_kinematics": 7 * 8, "g2p_constitutive": 0, "bin": 0}
-FUSED_CELL = {"remap_partials": (10 * 8, 4 * 8)}
 # With the exact return map and one instance, G2P pass 2 is one kernel
 # (k_g2p_fused), booked in the kinematics slot: its algorithmic bytes are
 # the kinematics' and the state update's together (the strain increment no
@@ -53,8 +49,8 @@ FUSED_G2P = {"g2p_kinematics": (26 * 8 + 5 + 30 * 8, 7 * 8)}
 # kernel of each slot, per summation order (names as rocprofv3 reports them)
 KERNELS = {
     "cell": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
-             "p2g_partials": "k_cell_partials<false, false, false>", "p2g_nodes": "k_node_cells<false, false>",
-             "g2p_velocity": None, "remap_partials": "k_cell_partials<true, true, false>",
+             "p2g_partials": "k_cell_partials<false, false>", "p2g_nodes": "k_node_cells<false, false>",
+             "g2p_velocity": "k_g2p_velocity<false>", "remap_partials": "k_cell_partials<true, false>",
              "remap_nodes": "k_node_cells<true, false>", "g2p_kinematics": "k_g2p_fused<0, false>",
              "g2p_constitutive": None},
     "particle": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
@@ -241,8 +237,6 @@ def main():
         if kernels[k] is None:
             continue
         per_p, per_n = PER_PARTICLE[k], PER_NODE[k]
-        if args.order == "cell" and k in FUSED_CELL:
-            per_p, per_n = FUSED_CELL[k]
         if k in FUSED_G2P:
             per_p, per_n = FUSED_G2P[k]
         b = per_p * scene.n + per_n * active + round(PER_PLASTIC.get(k, 0) * plastic_frac * scene.n)

@@ -768,19 +768,18 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd,
  * adjacent nodes in GridPoint_Mediator order (x, then y, then z innermost),
  * DP:490-510. */
 template <class REC>
-__device__ __forceinline__ void g2p_velocity_update_rec(const Inst &I, REC rec, const double x[3], double v[3],
-                                                        double s[3][2]) {
+__device__ __forceinline__ void g2p_velocity_update_rec(const Inst &I, REC rec, const double x[3], double v[3]) {
     const GridGeom &g = I.g;
     /* grid fields read once, outside any branch */
     const double gL[3] = {g.L[0], g.L[1], g.L[2]}, gh[3] = {g.h[0], g.h[1], g.h[2]};
     const int gc[3] = {g.cells[0], g.cells[1], g.cells[2]}, gn[3] = {g.nn[0], g.nn[1], g.nn[2]};
     int b[3], lo[3], hi[3];
+    double s[3][2];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         base_cell(x[a], gh[a], gn[a], b[a], lo[a], hi[a]);
 #pragma unroll
-        for (int o = 0; o < 2; o++)
-            s[a][o] = axis_shape(x[a], node_coord(b[a] + o, gL[a], gc[a]), gh[a]).s;
+        for (int o = 0; o < 2; o++) s[a][o] = axis_shape(x[a], node_coord(b[a] + o, gL[a], gc[a]), gh[a]).s;
     }
     bool ok[3][2];
 #pragma unroll
@@ -816,10 +815,7 @@ __device__ __forceinline__ void g2p_velocity_update_rec(const Inst &I, REC rec, 
     }
 }
 
-/* s receives the shape factors of the particle's base-cell corner nodes
- * (axis a, offset o along it), which the remap partials reuse. */
-__device__ __forceinline__ void g2p_velocity_update(const Inst &I, const NView &Nd, const double x[3], double v[3],
-                                                    double s[3][2]) {
+__device__ __forceinline__ void g2p_velocity_update(const Inst &I, const NView &Nd, const double x[3], double v[3]) {
     const double *g1 = Nd.g1;
     g2p_velocity_update_rec(I,
                             [=](int, size_t node, double2 &a, double2 &b) {
@@ -827,7 +823,7 @@ __device__ __forceinline__ void g2p_velocity_update(const Inst &I, const NView &
                                 a = r[0];
                                 b = r[1];
                             },
-                            x, v, s);
+                            x, v);
 }
 
 template <bool MULTI>
@@ -838,8 +834,7 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_velocity(Batch bt, PView P, NView
     const size_t n = P.n;
     const double x[3] = {P.x[i], P.x[n + i], P.x[2 * n + i]};
     double v[3] = {P.v[i], P.v[n + i], P.v[2 * n + i]};
-    double s[3][2];
-    g2p_velocity_update(inst_p<MULTI>(bt, i), Nd, x, v, s);
+    g2p_velocity_update(inst_p<MULTI>(bt, i), Nd, x, v);
     P.v[i] = v[0];
     P.v[n + i] = v[1];
     P.v[2 * n + i] = v[2];
@@ -970,11 +965,10 @@ __device__ __forceinline__ void node_remap_dispatch(const Batch &bt, const NodeI
  * its cell's corner nodes (axis a, offset o); corner (ox, oy, oz) takes
  * N = s_x[ox] s_y[oy] s_z[oz], the product the lane would form.  The staged
  * particle's base cell is its bin (floor(x / h), as the binning computed it
- * from the same x), so its corners are exactly the lanes' nodes.  (P2G
- * keeps the per-lane shapes: its shared data, with the twelve gradient
- * quotients (s_j s_l) / h_k, does not fit LDS for a full staging chunk,
- * and staging half chunks with those divisions in one thread per particle
- * measured slower.) */
+ * from the same x), so its corners are exactly the lanes' nodes.  In the
+ * P2G pass, which also needs the twelve gradient quotients (s_j s_l) / h_k
+ * (too much LDS per staged particle), the 8 lanes of a cell share that work
+ * for the particle at hand instead (see the P2G loop). */
 #define PART_CELLS (BLOCK / 8)
 #define PART_STAGE 256
 
@@ -991,11 +985,7 @@ __device__ __forceinline__ void particle_factors(const GridGeom &g, const double
     }
 }
 
-/* G1 (remap pass of DoublePass and its contact variant): G2P pass 1 runs
- * here too, on each particle as it is staged (each particle is staged by
- * exactly one block), and the updated velocity is written back; its shape
- * factors are the ones the remap lanes need. */
-template <bool REMAP, bool G1, bool MULTI>
+template <bool REMAP, bool MULTI>
 __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NView Nd, BView B, CellParts C) {
     /* staged per particle: m v (3) and either the shape factors s (6, remap)
      * or x (3, P2G), then for P2G m, V, sigma (6), f_ext (3); rows padded by
@@ -1074,9 +1064,9 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
             for (uint32_t q = base + threadIdx.x; q < top; q += BLOCK) {
                 const uint32_t id = B.sorted[q], j = q - base;
                 const double x[3] = {P.x[id], P.x[n + id], P.x[2 * n + id]};
-                double v[3] = {P.v[id], P.v[n + id], P.v[2 * n + id]};
+                const double v[3] = {P.v[id], P.v[n + id], P.v[2 * n + id]};
                 const double mp = P.m[id];
-                const int dcr = (skip_dc || (G1 && bt.variant != VAR_DP)) ? P.dc_rank[id] : -1;
+                const int dcr = skip_dc ? P.dc_rank[id] : -1;
                 if (!REMAP) {
 #pragma unroll
                     for (int a = 0; a < 3; a++) sS[a][j] = x[a];
@@ -1090,18 +1080,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
                         sf[2][j] = P.fext[2 * n + id];
                     }
                 } else {
-                    double sh[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-                    const Inst &I = inst_p<MULTI>(bt, id);
-                    if (G1) {
-                        if (!(bt.variant != VAR_DP && dcr >= 0)) { /* DPC:309-310 */
-                            g2p_velocity_update(I, Nd, x, v, sh);
-                            P.v[id] = v[0];
-                            P.v[n + id] = v[1];
-                            P.v[2 * n + id] = v[2];
-                        }
-                    } else {
-                        particle_factors(I.g, x, sh);
-                    }
+                    double sh[3][2];
+                    particle_factors(inst_p<MULTI>(bt, id).g, x, sh);
 #pragma unroll
                     for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -2176,7 +2156,7 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
         }
         if (order == ORDER_CELL) {
             if (P.n > 0)
-                LAUNCH_MULTI(bt, (k_cell_partials<false, false, M>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt,
+                LAUNCH_MULTI(bt, (k_cell_partials<false, M>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt,
                              P, N, B, C);
             if (mid) (void)hipEventRecord(mid, s);
             LAUNCH_MULTI(bt, (k_node_cells<false, M>), dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
@@ -2186,9 +2166,7 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
         }
         break;
     case PHASE_G2P_VELOCITY:
-        /* in the cell order of DoublePass and its contact variant the remap's
-         * partials kernel runs this pass */
-        if (P.n > 0 && !(order == ORDER_CELL && bt.variant != VAR_SP))
+        if (P.n > 0)
             LAUNCH_MULTI(bt, (k_g2p_velocity<M>), dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, bt, P, N);
         if (mid) (void)hipEventRecord(mid, s);
         break;
@@ -2199,12 +2177,8 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
         }
         if (order == ORDER_CELL) {
             if (P.n > 0) {
-                if (bt.variant != VAR_SP)
-                    LAUNCH_MULTI(bt, (k_cell_partials<true, true, M>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt,
-                                 P, N, B, C);
-                else
-                    LAUNCH_MULTI(bt, (k_cell_partials<true, false, M>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt,
-                                 P, N, B, C);
+                LAUNCH_MULTI(bt, (k_cell_partials<true, M>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt, P, N,
+                             B, C);
             }
             if (mid) (void)hipEventRecord(mid, s);
             LAUNCH_MULTI(bt, (k_node_cells<true, M>), dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
