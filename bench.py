@@ -166,12 +166,120 @@ def surrogate_bench(scene, steps, train_steps=1500):
             "train_loss_first_last": [float(hist[:10].mean()), float(hist[-10:].mean())]}
 
 
+def _train_surrogate(mat, train_steps):
+    from code_mpm_amd import surrogate as su
+    s = su.Surrogate(seed=11)
+    data = su.make_training_set(mat, su.synthetic_states(mat, 200000, seed=3))
+    s.fit_normalization(data)
+    xs, ts = s.standardise(data)
+    hist = s.train(xs, ts, steps=train_steps, batch=4096, lr=2e-3, seed=12)
+    return s, hist
+
+
+def _run_curve(scene, surrogate, steps, every, material=0):
+    """steps steps (5 warm-up steps first), monitors read every `every`
+    steps; returns (ms/step over the timed steps, monitor rows, health,
+    active nodes)."""
+    import numpy as np
+    from code_mpm_amd import Engine
+    eng = Engine(device=0, order="cell")
+    eng.load_scene(scene)
+    if surrogate is not None:
+        eng.set_surrogate(surrogate, material=material, residual_tolerance=1e-4)
+    eng.step(scene.dt, 5)
+    eng.synchronize()
+    rows = []
+    t0 = time.perf_counter()
+    for _ in range(steps // every):
+        eng.step(scene.dt, every)
+        m = eng.monitors()
+        rows.append([m["position_y"], m["force_y"], m["strain_energy"], m["plastic_energy"], m["momentum_y"]])
+    eng.synchronize()
+    el = time.perf_counter() - t0
+    health = eng.health()
+    active = int(eng.get_node("ACTIVE").sum())
+    eng.close()
+    return 1e3 * el / (every * (steps // every)), np.array(rows), health, active
+
+
+def _curve_error(a, b):
+    """max |a - b| / max |a| and ||a - b|| / ||a|| of two sampled curves."""
+    import numpy as np
+    peak, norm = float(np.abs(a).max()), float(np.linalg.norm(a))
+    return {"max_over_peak": float(np.abs(a - b).max() / peak) if peak > 0 else None,
+            "l2": float(np.linalg.norm(a - b) / norm) if norm > 0 else None}
+
+
+def c4_bench(steps, fd_steps, every=50, fd_every=200, train_steps=1500):
+    """C4: the half ring crushed by a foam impactor (scenes.half_ring_impact),
+    exact FP64 return map against the learned one on the ring's steel.
+    At the full ~48 M-particle size, `steps` steps each way: step time, the
+    step's algorithmic HBM rate, the network's fallback rate and the error of
+    the ring's energy curves.  The stress wave needs thousands of steps to
+    reach the support, so the force-displacement comparison runs the same
+    set-up at 4 h (about 0.8 M particles) for fd_steps steps: support reaction
+    against platen displacement, exact vs learned."""
+    import numpy as np
+    from code_mpm_amd import scenes
+    t = time.perf_counter()
+    sc = scenes.half_ring_impact()
+    print(f"# c4: {sc.n} particles, {sc.nnodes} nodes, setup {time.perf_counter() - t:.1f} s",
+          file=sys.stderr, flush=True)
+    s, hist = _train_surrogate(sc.materials[0], train_steps)
+    full = {}
+    for mode in ("exact", "surrogate"):
+        ms, rows, health, active = _run_curve(sc, s if mode == "surrogate" else None, steps, every)
+        full[mode] = dict(ms=ms, rows=rows, health=health, active=active)
+        print(f"# c4 {mode}: {ms:.3f} ms/step", file=sys.stderr, flush=True)
+    n, active = sc.n, full["exact"]["active"]
+    step_bytes = 693 * n + 328 * active
+    h = full["surrogate"]["health"]
+    calls, falls = h["surrogate_calls"], h["surrogate_fallbacks"]
+    e_rows, s_rows = full["exact"]["rows"], full["surrogate"]["rows"]
+    del sc
+    small = scenes.half_ring_impact(h=2.0e-4, depth_cells=23, foam_cells=20)
+    fd = {}
+    for mode in ("exact", "surrogate"):
+        fd[mode] = _run_curve(small, s if mode == "surrogate" else None, fd_steps, fd_every)
+    ea, sa = fd["exact"][1], fd["surrogate"][1]
+    y0 = small.fields["POSITION"][1][small.dc_index].mean()
+    return {
+        "workload": f"half ring (steel, von Mises hardening) crushed by an aluminium foam impactor at "
+                    f"{sc_speed(small)} m/s, {n} particles ({steps} steps each way, exact FP64 vs the learned "
+                    "return map on the ring's steel, residual tolerance 1e-4 sigma_y)",
+        "particles": n, "active_nodes": active, "steps": steps,
+        "exact_ms_per_step": round(full["exact"]["ms"], 4), "surrogate_ms_per_step": round(full["surrogate"]["ms"], 4),
+        "exact_particle_steps_per_s": round(n / (full["exact"]["ms"] * 1e-3), 1),
+        "exact_algorithmic_gbs": round(step_bytes / (full["exact"]["ms"] * 1e-3) / 1e9, 1),
+        "network_calls": calls, "fallback_rate": round(falls / max(calls, 1), 4),
+        "plastic_points": full["exact"]["health"]["plastic_points"],
+        "ring_strain_energy_error": _curve_error(e_rows[:, 2], s_rows[:, 2]),
+        "ring_plastic_energy_error": _curve_error(e_rows[:, 3], s_rows[:, 3]),
+        "force_displacement": {
+            "workload": f"the same set-up at h = 2e-4 m, {small.n} particles, {fd_steps} steps, "
+                        f"support reaction sampled every {fd_every} steps",
+            "displacement_end_m": float(y0 - ea[-1, 0]), "force_peak_exact_N": float(np.abs(ea[:, 1]).max()),
+            "force_error": _curve_error(ea[:, 1], sa[:, 1]),
+            "network_calls": fd["surrogate"][2]["surrogate_calls"],
+            "fallback_rate": round(fd["surrogate"][2]["surrogate_fallbacks"] /
+                                   max(fd["surrogate"][2]["surrogate_calls"], 1), 4)},
+        "train_steps": train_steps, "train_loss_first_last": [float(hist[:10].mean()), float(hist[-10:].mean())],
+    }
+
+
+def sc_speed(scene):
+    return scene.info["impact_speed"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--launch", default="eager", choices=["graph", "eager"],
+                    help="time the K steps replayed from one HIP graph (recorded and run once first) or "
+                         "launched kernel by kernel; the other is reported beside it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--order", default="cell", choices=["cell", "particle"],
@@ -183,6 +291,10 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--surrogate-steps", type=int, default=10,
                     help="steps of the learned-return-map comparison (0: skip)")
+    ap.add_argument("--c4-steps", type=int, default=1000,
+                    help="C4 half-ring impact at ~48 M particles: steps each way, exact and learned (0: skip)")
+    ap.add_argument("--c4-fd-steps", type=int, default=8000,
+                    help="C4 force-displacement comparison at 4 h: steps each way")
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="first run one device copy of this many GiB (the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -216,13 +328,27 @@ def main():
     eng.step(scene.dt, args.warmup)
     eng.synchronize()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    eng.step(scene.dt, args.steps)
-    eng.synchronize()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+
+    def timed(launch):
+        if launch == "graph":
+            eng.step_graph(args.steps)  # records the K steps and runs them once
+            eng.synchronize()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        if launch == "graph":
+            eng.step_graph(args.steps)
+        else:
+            eng.step(scene.dt, args.steps)
+        eng.synchronize()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    other = "eager" if args.launch == "graph" else "graph"
+    elapsed_other = timed(other)
+    elapsed = timed(args.launch)
     value = scene.n * args.steps / elapsed
-    print(f"# timed {args.steps} steps in {elapsed:.4f} s", file=sys.stderr, flush=True)
+    print(f"# timed {args.steps} steps in {elapsed:.4f} s ({args.launch}); {elapsed_other:.4f} s ({other})",
+          file=sys.stderr, flush=True)
 
     kms = eng.step_timed_kernels(scene.dt, args.profile_steps)
     avg_ms = {k: v / args.profile_steps for k, v in kms.items()}
@@ -273,6 +399,7 @@ def main():
 
     batch = batch_bench(args.batch_instances, args.batch_steps) if args.batch_instances > 0 else None
     sur = surrogate_bench(scenes.honeycomb_crush(), args.surrogate_steps) if args.surrogate_steps > 0 else None
+    c4 = c4_bench(args.c4_steps, args.c4_fd_steps) if args.c4_steps > 0 else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -290,7 +417,11 @@ def main():
         "config": {"workload": "J2 honeycomb crush, 512x512x32 cells, 8.0M particles, exact FP64 return map "
                                "(von Mises hardening), platen 10 m/s, seed 1",
                    "particles": scene.n, "active_nodes": active, "plastic_fraction": round(plastic_frac, 5),
-                   "return_map": "exact", "summation_order": args.order},
+                   "return_map": "exact", "summation_order": args.order,
+                   "launch": ("one HIP graph of the K steps, replayed" if args.launch == "graph"
+                              else "kernel launches on one stream")},
+        "other_launch": {"launch": other, "value": round(scene.n * args.steps / elapsed_other, 1),
+                         "ms_per_step": round(1e3 * elapsed_other / args.steps, 4)},
         "roofline": {"bound": "hbm", "kernel": kernels[dominant], "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes": dom_bytes,
@@ -300,6 +431,7 @@ def main():
         "variants": variants,
         "batch": batch,
         "surrogate": sur,
+        "c4": c4,
         "cpu_baseline": cpu,
         "gpu_over_cpu": round(value / cpu["value"], 1) if cpu else None,
     }

@@ -295,6 +295,110 @@ def honeycomb_crush(cells=(512, 512, 32), h=1.0e-4, hex_cells=(6, 6), z_layers=2
                   wave_speed=c))
 
 
+def half_ring_impact(h=5.0e-5, depth_cells=91, foam_cells=80, porosity=0.5, impact_speed=20.0, seed=4,
+                     platen_cells=2, steps=1000):
+    """C4: the half ring of
+    PhysicsEngine_initializeWorld_Classic_HalfRing_Xiang_PlainStress.cpp:4-195
+    crushed by a foam impactor.  The ring (inner diameter 47.9 mm, wall
+    1.48 mm, steel, von Mises hardening with the reference's constants) is cut
+    at its symmetry plane: it stands on the y support (nodes y < 1.5 h fixed in
+    y, reaction measured), its cut face lies on the nodes x < 1.5 h, fixed in
+    x, and every node is fixed in z (the reference's plane-stress set-up,
+    here extruded over depth_cells cells).  Shifts: +h in x as in the
+    reference; +h in y (the reference's +2h leaves the ring's lowest cells
+    one node row above the fixed rows y < 1.5 h, so it would meet the support
+    only after moving a cell).  On top of the ring, in contact with it, a block of aluminium
+    foam (the reference's second material) foam_cells tall over 0.95 of the
+    ring's width, with spherical voids (radii lognormal, mu = ln 3h,
+    sigma = 0.3, from ``seed``) up to ``porosity``, moves down at
+    impact_speed; its top platen_cells layers are displacement-controlled at
+    that speed.  2x2x2 particles per solid cell; the default h gives about
+    48 M particles."""
+    rng = np.random.default_rng(seed)
+    r_in = 0.5 * 0.0479
+    r_out = r_in + 0.00148
+    ring_cx, ring_cy = 1.0 * h, 1.0 * h + r_out
+    nz = int(depth_cells) + 2
+    ring_top = ring_cy + r_out
+    nx = int(np.ceil((ring_cx + r_out) / h)) + 4
+    ny_ring = int(np.ceil(ring_top / h)) + 1
+    # ring: cells whose centre lies in the half annulus (x >= the cut), same in every z layer
+    gy, gx = np.meshgrid(np.arange(ny_ring), np.arange(nx), indexing="ij")
+    cxp, cyp = (gx + 0.5) * h, (gy + 0.5) * h
+    rr = np.hypot(cxp - ring_cx, cyp - ring_cy)
+    ring2d = (rr >= r_in) & (rr <= r_out) & (cxp >= ring_cx)
+    # the foam's first cell row sits on the ring's top row: the two share that
+    # row of nodes, so they are in contact from the start
+    fy0 = int(np.nonzero(ring2d)[0].max()) + 1
+    fy1 = fy0 + int(foam_cells)                   # foam rows [fy0, fy1), platen on top
+    ny = fy1 + platen_cells + 3
+    cells = np.array([nx, ny, nz], dtype=np.int32)
+    hh = np.full(3, h)
+    length = cells * hh
+    # foam: a block of cells, voids carved by bounding box
+    fx0 = int(np.floor(ring_cx / h))
+    fx1 = fx0 + int(np.floor(0.95 * r_out / h))
+    body = np.array([fx1 - fx0, fy1 - fy0, nz - 2])
+    solid = np.ones((body[2], body[1], body[0]), bool)          # [z, y, x]
+    target = porosity * solid.size
+    removed = 0
+    lo = np.array([fx0, fy0, 1], np.float64) * h
+    while removed < target:
+        r = float(np.exp(rng.normal(np.log(3.0 * h), 0.3)))
+        c = lo + rng.uniform(0, 1, 3) * body * h
+        a0 = np.clip(np.floor((c - r - lo) / h).astype(int), 0, body - 1)
+        a1 = np.clip(np.floor((c + r - lo) / h).astype(int) + 1, 0, body)
+        sz, sy, sx = (slice(a0[2], a1[2]), slice(a0[1], a1[1]), slice(a0[0], a1[0]))
+        zc = (np.arange(a0[2], a1[2]) + 0.5) * h + lo[2] - c[2]
+        yc = (np.arange(a0[1], a1[1]) + 0.5) * h + lo[1] - c[1]
+        xc = (np.arange(a0[0], a1[0]) + 0.5) * h + lo[0] - c[0]
+        ball = (zc[:, None, None] ** 2 + yc[None, :, None] ** 2 + xc[None, None, :] ** 2) <= r * r
+        sub = solid[sz, sy, sx]
+        removed += int((sub & ball).sum())
+        sub &= ~ball
+    # cells of each body in emission order (z, then y, then x)
+    ry, rx = np.nonzero(ring2d)
+    ring_cells = np.concatenate([np.stack([rx, ry, np.full(len(rx), z)], 1) for z in range(1, nz - 1)])
+    fz, fyy, fxx = np.nonzero(solid)
+    foam_cells_idx = np.stack([fxx + fx0, fyy + fy0, fz + 1], 1)
+    pz, py_, px_ = np.meshgrid(np.arange(1, nz - 1), np.arange(fy1, fy1 + platen_cells), np.arange(fx0, fx1),
+                               indexing="ij")
+    platen_idx = np.stack([px_.ravel(), py_.ravel(), pz.ravel()], 1)
+    x = np.concatenate([lattice_in_cells(ring_cells, hh), lattice_in_cells(foam_cells_idx, hh),
+                        lattice_in_cells(platen_idx, hh)], 1)
+    n_ring, n_foam, n_platen = 8 * len(ring_cells), 8 * len(foam_cells_idx), 8 * len(platen_idx)
+    n = x.shape[1]
+    v = np.zeros((3, n))
+    v[1, n_ring:] = -impact_speed
+    steel = dict(type=VONMISESHARDENING, density=7800.0, elastic_modulus=210.0e9, poisson_ratio=0.3,
+                 yield_stress=310.0e6, hardening_c0=4.0, hardening_c1=150.0e6)
+    aluminium = dict(type=VONMISESHARDENING, density=2760.0, elastic_modulus=70.0e9, poisson_ratio=0.3,
+                     yield_stress=190.0e6, hardening_c0=30.0, hardening_c1=50.0e6)
+    material = np.zeros(n, np.int32)
+    material[n_ring:] = 1
+    fields = default_fields(x, v, (h / 2.0) ** 3, 1.0)
+    fields["MASS"] = np.where(material == 0, steel["density"], aluminium["density"]) * fields["VOLUME_INITIAL"]
+    nodes = node_positions(length, cells)
+    fixed = (np.where(nodes[0] < 1.5 * h, AXIS_X, 0) | np.where(nodes[1] < 1.5 * h, AXIS_Y, 0) | AXIS_Z)
+    fixed = fixed.astype(np.uint8)
+    platen = np.arange(n_ring + n_foam, n, dtype=np.int64)
+    c = np.sqrt(steel["elastic_modulus"] / steel["density"])
+    dt = 0.1 * h / c
+    return Scene(
+        name=f"half_ring_impact_{nx}x{ny}x{nz}", length=length, cells=cells, fixed=fixed,
+        friction=np.zeros(fixed.shape), materials=[steel, aluminium], fields=fields, material=material,
+        dt=dt, time_end=steps * dt,
+        dc_index=platen, dc_axes=np.full(len(platen), AXIS_X | AXIS_Y | AXIS_Z, np.uint8),
+        dc_mult=np.full(len(platen), -1.0, np.float32),
+        monitors={"DISPLACEMENT": platen, "MOMENTUM": np.arange(n_ring + n_foam, dtype=np.int64),
+                  "ENERGY": np.arange(n_ring, dtype=np.int64)},
+        timeline_t=np.array([0.0, 10.0]),
+        timeline_v=np.array([[0.0, impact_speed, 0.0], [0.0, impact_speed, 0.0]]),
+        console_interval=100 * dt,
+        info=dict(h=h, particles=n, ring_particles=n_ring, foam_particles=n_foam, platen_particles=n_platen,
+                  porosity=removed / solid.size, seed=seed, impact_speed=impact_speed, wave_speed=c))
+
+
 def random_block(seed=0, cells=(6, 5, 4), n=300, h=1.0e-3, features=True):
     """Small randomized scene exercising every branch of the step: three
     materials (elastic, perfectly plastic, hardening) plus an unknown type and

@@ -273,3 +273,19 @@ def test_variant_run_loop(variant):
     assert g.getTime_Current() == c.getTime_Current()
     assert g.variant == variant
     _assert_same(g, c, f"{variant} run")
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_half_ring_impact_small(order):
+    """The C4 set-up at a small size: two hardening materials, a platen
+    controlled on all axes, an impactor with initial velocity, x-, y- and
+    z-fixed nodes; bit for bit against the oracle, with yielding reached."""
+    from code_mpm_amd import scenes
+    scene = scenes.half_ring_impact(h=4.0e-4, depth_cells=4, foam_cells=8, impact_speed=60.0, steps=90)
+    g, c = _pair(scene, order)
+    for k in range(3):
+        g.step(scene.dt, 30)
+        c.step(scene.dt, 30)
+        _assert_monitors(g, c, f"half ring after {30 * (k + 1)} steps")
+    _assert_same(g, c, "half ring 90 steps")
+    assert g.health()["plastic_points"] > 0
