@@ -12,10 +12,12 @@ def test_half_ring_impact_geometry():
     x = sc.fields["POSITION"]
     n_ring = sc.info["ring_particles"]
     r_in, r_out = 0.5 * 0.0479, 0.5 * 0.0479 + 0.00148
-    # ring particles: in the half annulus around (h, 2h + r_out), within half a cell of the walls
-    rr = np.hypot(x[0, :n_ring] - h, x[1, :n_ring] - (2 * h + r_out))
+    # ring particles: in the half annulus around (h, h + r_out), within a cell of the walls
+    rr = np.hypot(x[0, :n_ring] - h, x[1, :n_ring] - (h + r_out))
     assert rr.min() > r_in - h and rr.max() < r_out + h
     assert x[0, :n_ring].min() >= h
+    # it stands on the support: its lowest cells touch the fixed node row y = h
+    assert x[1, :n_ring].min() < 2 * h
     # materials: steel ring, aluminium foam and platen
     assert (sc.material[:n_ring] == 0).all() and (sc.material[n_ring:] == 1).all()
     assert sc.materials[0]["yield_stress"] == 310.0e6 and sc.materials[1]["yield_stress"] == 190.0e6
