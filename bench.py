@@ -178,8 +178,12 @@ def _train_surrogate(mat, train_steps):
 
 def _run_curve(scene, surrogate, steps, every, material=0):
     """steps steps (5 warm-up steps first), monitors read every `every`
-    steps; returns (ms/step over the timed steps, monitor rows, health,
-    active nodes)."""
+    steps; returns (ms/step of the steps alone, monitor rows, health, active
+    nodes).  The monitors are the reference's sequential sums over their
+    particle lists (reportConsole order, so they match the oracle bit for
+    bit), one thread per sum: on tens of millions of particles a read takes
+    seconds, so it is kept out of the timed window, as the reference keeps
+    its console output out of d_Runtime_Total."""
     import numpy as np
     from code_mpm_amd import Engine
     eng = Engine(device=0, order="cell")
@@ -189,13 +193,14 @@ def _run_curve(scene, surrogate, steps, every, material=0):
     eng.step(scene.dt, 5)
     eng.synchronize()
     rows = []
-    t0 = time.perf_counter()
+    el = 0.0
     for _ in range(steps // every):
+        t0 = time.perf_counter()
         eng.step(scene.dt, every)
+        eng.synchronize()
+        el += time.perf_counter() - t0
         m = eng.monitors()
         rows.append([m["position_y"], m["force_y"], m["strain_energy"], m["plastic_energy"], m["momentum_y"]])
-    eng.synchronize()
-    el = time.perf_counter() - t0
     health = eng.health()
     active = int(eng.get_node("ACTIVE").sum())
     eng.close()
@@ -210,7 +215,7 @@ def _curve_error(a, b):
             "l2": float(np.linalg.norm(a - b) / norm) if norm > 0 else None}
 
 
-def c4_bench(steps, fd_steps, every=50, fd_every=200, train_steps=1500):
+def c4_bench(steps, fd_steps, every=250, fd_every=200, train_steps=1500):
     """C4: the half ring crushed by a foam impactor (scenes.half_ring_impact),
     exact FP64 return map against the learned one on the ring's steel.
     At the full ~48 M-particle size, `steps` steps each way: step time, the
@@ -248,6 +253,9 @@ def c4_bench(steps, fd_steps, every=50, fd_every=200, train_steps=1500):
                     f"{sc_speed(small)} m/s, {n} particles ({steps} steps each way, exact FP64 vs the learned "
                     "return map on the ring's steel, residual tolerance 1e-4 sigma_y)",
         "particles": n, "active_nodes": active, "steps": steps,
+        "timing": "the steps alone, synchronised every monitor sample; monitor reads (sequential sums over "
+                  "the lists) are outside it",
+        "energy_samples_every_steps": every,
         "exact_ms_per_step": round(full["exact"]["ms"], 4), "surrogate_ms_per_step": round(full["surrogate"]["ms"], 4),
         "exact_particle_steps_per_s": round(n / (full["exact"]["ms"] * 1e-3), 1),
         "exact_algorithmic_gbs": round(step_bytes / (full["exact"]["ms"] * 1e-3) / 1e9, 1),
@@ -277,6 +285,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default node tile shape instead of the one mpm_autotune picks (one step)")
     ap.add_argument("--launch", default="eager", choices=["graph", "eager"],
                     help="time the K steps replayed from one HIP graph (recorded and run once first) or "
                          "launched kernel by kernel; the other is reported beside it")
@@ -325,6 +335,10 @@ def main():
     print(f"# scene {scene.name}: {scene.n} particles, setup {time.perf_counter() - t_setup:.1f} s",
           file=sys.stderr, flush=True)
 
+    tile = eng.tile_shape()
+    if not args.no_autotune:
+        tile, _ = eng.autotune()   # one ordinary step that also picks the node tile shape
+        print(f"# autotuned node tile {tile}", file=sys.stderr, flush=True)
     eng.step(scene.dt, args.warmup)
     eng.synchronize()
     torch.cuda.synchronize()
@@ -418,6 +432,7 @@ def main():
                                "(von Mises hardening), platen 10 m/s, seed 1",
                    "particles": scene.n, "active_nodes": active, "plastic_fraction": round(plastic_frac, 5),
                    "return_map": "exact", "summation_order": args.order,
+                   "node_tile": list(tile) if tile is not None else None,
                    "launch": ("one HIP graph of the K steps, replayed" if args.launch == "graph"
                               else "kernel launches on one stream")},
         "other_launch": {"launch": other, "value": round(scene.n * args.steps / elapsed_other, 1),
