@@ -409,6 +409,23 @@ def main():
                          "ms_per_step": round(1e3 * el / args.variant_steps, 4), "steps": args.variant_steps}
         print(f"# {var}: {variants[var]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     eng.set_variant("doublepass")
+    large = None
+    if args.variant_steps > 0:
+        # the large-strain grid-to-particle pass (logarithmic strains) on the same state
+        eng.set_strain_formulation("large")
+        eng.step(scene.dt, 2)
+        eng.synchronize()
+        t1 = time.perf_counter()
+        eng.step(scene.dt, args.variant_steps)
+        eng.synchronize()
+        el = time.perf_counter() - t1
+        kl = eng.step_timed_kernels(scene.dt, 3)
+        large = {"value": round(scene.n * args.variant_steps / el, 1), "unit": "particle-steps/s",
+                 "ms_per_step": round(1e3 * el / args.variant_steps, 4), "steps": args.variant_steps,
+                 "kernel": "k_g2p_large<false>", "kernel_ms": round(kl["g2p_kinematics"] / 3, 4)}
+        print(f"# large strain: {large['ms_per_step']:.3f} ms/step, pass {large['kernel_ms']:.3f} ms",
+              file=sys.stderr, flush=True)
+        eng.set_strain_formulation("small")
     eng.close()
 
     batch = batch_bench(args.batch_instances, args.batch_steps) if args.batch_instances > 0 else None
@@ -444,6 +461,7 @@ def main():
         "step_effective_gbs": round(step_bytes / (elapsed / args.steps) / 1e9, 1),
         "kernels": per_kernel,
         "variants": variants,
+        "large_strain": large,
         "batch": batch,
         "surrogate": sur,
         "c4": c4,

@@ -22,10 +22,12 @@ PF = dict(
     POSITION=(0, 3), VELOCITY=(1, 3), FORCE_EXTERNAL=(2, 3), MASS=(3, 1), VOLUME=(4, 1), VOLUME_INITIAL=(5, 1),
     DEFORMATION_GRADIENT=(6, 9), STRAIN=(7, 6), STRESS=(8, 6), STRAIN_PLASTIC=(9, 6), BACKSTRESS=(10, 1),
     ENERGY_STRAIN=(11, 1), ENERGY_STRAIN_MAX=(12, 1), ENERGY_STRAIN_TI=(13, 1), ENERGY_PLASTIC=(14, 1),
+    DEFORMATION_GRADIENT_ELASTIC=(15, 9),
 )
 NF = dict(MASS=(0, 1), MOMENTUM=(1, 3), VELOCITY=(2, 3), FORCE=(3, 3), FORCE_TEMP=(4, 3), ACTIVE=(5, 1))
 MON = dict(DISPLACEMENT=0, MOMENTUM=1, ENERGY=2, STRESS=3)
 VARIANT = dict(doublepass=0, doublepass_contact=1, singlepass=2)
+STRAIN = dict(small=0, large=1)
 ORDER = dict(particle=0, cell=1)
 KSLOTS = ("bin", "p2g_partials", "p2g_nodes", "g2p_velocity", "remap_partials", "remap_nodes", "g2p_kinematics",
           "g2p_constitutive")
@@ -113,6 +115,8 @@ SIGNATURES = {
     "mpm_set_summation_order": (I32, [P, I32]),
     "mpm_set_variant": (I32, [P, I32]),
     "mpm_get_variant": (I32, [P, PI32]),
+    "mpm_set_strain_formulation": (I32, [P, I32]),
+    "mpm_get_strain_formulation": (I32, [P, PI32]),
     "mpm_set_tile_shape": (I32, [P, PI32]),
     "mpm_get_tile_shape": (I32, [P, PI32]),
     "mpm_autotune": (I32, [P, I32, PI32, PDBL]),

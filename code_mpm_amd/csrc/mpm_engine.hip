@@ -27,7 +27,7 @@
 
 namespace {
 
-const int kFieldComponents[MPM_PF_COUNT] = {3, 3, 3, 1, 1, 1, 9, 6, 6, 6, 1, 1, 1, 1, 1};
+const int kFieldComponents[MPM_PF_COUNT] = {3, 3, 3, 1, 1, 1, 9, 6, 6, 6, 1, 1, 1, 1, 1, 9};
 
 std::string g_create_error = "no error";
 
@@ -117,6 +117,7 @@ struct mpm_engine {
     /* node summation order and the cell-order partial sums */
     int order = ORDER_CELL;
     int variant = VAR_DP;
+    int strain = STRAIN_SMALL;
     uint8_t *dtflag = nullptr; /* node tile flags, see Batch::tflag */
     int64_t tflag_cap = 0;
     CellParts parts{};
@@ -263,6 +264,7 @@ struct mpm_engine {
         P.V = pf[MPM_PF_VOLUME];
         P.V0 = pf[MPM_PF_VOLUME_INITIAL];
         P.F = pf[MPM_PF_DEFORMATION_GRADIENT];
+        P.Fe = pf[MPM_PF_DEFORMATION_GRADIENT_ELASTIC];
         P.eps = pf[MPM_PF_STRAIN];
         P.sig = pf[MPM_PF_STRESS];
         P.epsp = pf[MPM_PF_STRAIN_PLASTIC];
@@ -306,6 +308,7 @@ struct mpm_engine {
         b.ty = tile[1];
         b.tz = tile[2];
         b.variant = variant;
+        b.strain = strain;
         b.fext = has_fext ? 1 : 0;
         b.nmat = mt.n;
         b.tflag = order == ORDER_CELL ? dtflag : nullptr;
@@ -477,6 +480,8 @@ struct mpm_engine {
 
     int begin_steps() {
         if (!layout_set) return fail(MPM_ERR_STATE, "step before set_grid / set_instances");
+        if (strain == STRAIN_LARGE && (variant != VAR_DP || sur || cap))
+            return fail(MPM_ERR_STATE, "the large-strain pass runs with the DoublePass loop and the exact return map only");
         int rc;
         if ((rc = upload_table())) return rc;
         if (sur && (rc = ensure_queue())) return rc;
@@ -711,6 +716,8 @@ struct mpm_engine {
         std::vector<int32_t> minus1(sn, -1);
         std::vector<float> one(sn, 1.0f);
         if ((rc = hip(hipMemcpy(pf[MPM_PF_DEFORMATION_GRADIENT], F.data(), 8 * 9 * sn, hipMemcpyHostToDevice), "init F")) ||
+            (rc = hip(hipMemcpy(pf[MPM_PF_DEFORMATION_GRADIENT_ELASTIC], F.data(), 8 * 9 * sn, hipMemcpyHostToDevice),
+                      "init Fe")) ||
             (rc = hip(hipMemcpy(pmat, minus1.data(), 4 * sn, hipMemcpyHostToDevice), "init materials")) ||
             (rc = hip(hipMemcpy(pdc_rank, minus1.data(), 4 * sn, hipMemcpyHostToDevice), "init control ranks")) ||
             (rc = hip(hipMemset(pdc_axes, 0, sn), "init control axes")) ||
@@ -1066,6 +1073,25 @@ int mpm_set_variant(mpm_engine *e, int32_t variant) {
     (void)hipStreamSynchronize(e->stream);
     e->drop_graphs();
     e->variant = variant;
+    return MPM_OK;
+}
+
+int mpm_set_strain_formulation(mpm_engine *e, int32_t formulation) {
+    CHECK_E(e);
+    if (formulation != MPM_STRAIN_SMALL && formulation != MPM_STRAIN_LARGE)
+        return e->fail(MPM_ERR_ARG, "unknown strain formulation %d", formulation);
+    static_assert(MPM_STRAIN_SMALL == STRAIN_SMALL && MPM_STRAIN_LARGE == STRAIN_LARGE, "strain codes disagree");
+    if (formulation != e->strain) {
+        (void)hipStreamSynchronize(e->stream);
+        e->drop_graphs();
+    }
+    e->strain = formulation;
+    return MPM_OK;
+}
+
+int mpm_get_strain_formulation(const mpm_engine *e, int32_t *formulation) {
+    if (!e || !formulation) return MPM_ERR_ARG;
+    *formulation = e->strain;
     return MPM_OK;
 }
 

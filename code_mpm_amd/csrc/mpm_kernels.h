@@ -44,6 +44,7 @@ struct PView {
     double *x, *v, *fext;          /* 3n each */
     double *m, *V, *V0;            /* n */
     double *F;                     /* 9n, glm column-major components */
+    double *Fe;                    /* 9n, elastic deformation gradient (large-strain pass) */
     double *eps, *sig, *epsp;      /* 6n */
     double *R, *Ws, *Wmax, *Wti, *Wp;
     int32_t *mat;                  /* material index, -1 = none */
@@ -83,6 +84,8 @@ struct BView {
 enum { ORDER_PARTICLE = 0, ORDER_CELL = 1 };
 /* step loops: DoublePass, DoublePass_Contact, SinglePass (same codes as MPM_VARIANT_*) */
 enum { VAR_DP = 0, VAR_DPC = 1, VAR_SP = 2 };
+/* G2P pass 2 of the DoublePass loop (same codes as MPM_STRAIN_*) */
+enum { STRAIN_SMALL = 0, STRAIN_LARGE = 1 };
 
 /* Cell-order partial sums: one record per (occupied cell, corner), stored
  * corner-major, item t = corner * cap + occupied rank, so that neighbouring
@@ -123,6 +126,7 @@ struct Batch {
     int64_t nnodes;           /* all instances: component stride of node arrays */
     int tx, ty, tz;           /* node tile shape, tx * ty * tz == NODE_BLOCK */
     int variant;              /* VAR_* */
+    int strain;               /* STRAIN_SMALL / STRAIN_LARGE: G2P pass 2 of the DoublePass loop */
     int fext;                 /* any particle carries an external force */
     int nmat;                 /* materials in the table */
     uint8_t *tflag;           /* per node tile: 1 when a corner of an occupied cell lies in it

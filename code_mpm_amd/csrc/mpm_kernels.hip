@@ -1445,6 +1445,7 @@ __device__ __forceinline__ void trial_state(const MatTable &mt, int m, double R,
 struct KinOut {
     double de[6]; /* strain increment */
     double V;     /* volume after the pass */
+    double L[3][3], F[3][3], V0; /* large-strain pass: velocity gradient, F before the step, V0 */
     double dt;
     int mat;
     uint32_t flag; /* position not finite */
@@ -1452,8 +1453,9 @@ struct KinOut {
 };
 
 /* HANDOFF: store the strain increment and the flag for k_g2p_constitutive
- * (the fused pass keeps them in registers). */
-template <int VAR, bool HANDOFF, class REC>
+ * (the fused pass keeps them in registers).  LARGE: the large-strain pass,
+ * which takes L, F and V0 in ko and does the F and V update itself. */
+template <int VAR, bool HANDOFF, bool LARGE = false, class REC>
 __device__ __forceinline__ void g2p_kinematics_particle(const Batch &bt, const PView &P, const Inst &I, uint32_t i,
                                                         REC rec, uint32_t &next_key, uint32_t &f_outside,
                                                         KinOut &ko) {
@@ -1539,6 +1541,16 @@ __device__ __forceinline__ void g2p_kinematics_particle(const Batch &bt, const P
         P.x[i] = x[0];
         P.x[n + i] = x[1];
         P.x[2 * n + i] = x[2];
+        if (LARGE) {
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    ko.L[c][r] = L[c][r];
+                    ko.F[c][r] = F[c][r];
+                }
+            ko.V0 = V0;
+        } else {
         const bool valid = mat >= 0 && mat < bt.nmat;
         double dF[3][3];
 #pragma unroll
@@ -1598,6 +1610,7 @@ __device__ __forceinline__ void g2p_kinematics_particle(const Batch &bt, const P
                          Fn[2][0] * (Fn[0][1] * Fn[1][2] - Fn[1][1] * Fn[0][2]);
             ko.V = det * V0;
             P.V[i] = ko.V;
+        }
         }
         flag = !(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2])) ? 1 : 0;
     }
@@ -1838,6 +1851,226 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_fused(Batch bt, PView P, NView Nd
                                             next_key, f_outside, ko);
         B.key[i] = next_key;
         if (!ko.ctl) state_update_particle<VAR>(P, mt, i, ko, f_null, f_plastic, f_nonfinite);
+    }
+    wave_key_add(B.count, next_key, next_key != MPM_NONE);
+    block_count_add(&health[HEALTH_OUTSIDE], f_outside);
+    block_count_add(&health[HEALTH_NULL], f_null);
+    block_count_add(&health[HEALTH_PLASTIC], f_plastic);
+    block_count_add(&health[HEALTH_NONFINITE], f_nonfinite);
+}
+
+/* ---- large-strain pass ---------------------------------------------------- */
+
+/* glm mat3 product a * b: Result[c][r] = a[0][r]*b[c][0] + a[1][r]*b[c][1] + a[2][r]*b[c][2]. */
+__device__ __forceinline__ void m3_mul(const double a[3][3], const double b[3][3], double out[3][3]) {
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) out[c][r] = a[0][r] * b[c][0] + a[1][r] * b[c][1] + a[2][r] * b[c][2];
+}
+
+/* 0.5 * getLog(A * transpose(A)) (DP:323, DP:342-343): getLog is the series
+ * sum over t = 1..8 of ((B - I)^t * (1/t)) * (-1)^(t+1), the powers formed as
+ * I * D * D ... (getPower), added term by term
+ * (ConstitutiveRelation_MatrixAlgebra.cpp:12-23, 39-53).  P^t is P^(t-1) * D,
+ * the same products getPower repeats. */
+__device__ __forceinline__ void half_log_aat(const double A[3][3], double out[3][3]) {
+    double D[3][3], P[3][3], res[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            D[c][r] = (A[0][r] * A[0][c] + A[1][r] * A[1][c] + A[2][r] * A[2][c]) - (c == r ? 1.0 : 0.0);
+            P[c][r] = c == r ? 1.0 : 0.0;
+            res[c][r] = 0.0;
+        }
+#pragma unroll
+    for (int t = 1; t <= 8; t++) {
+        double Q[3][3];
+        m3_mul(P, D, Q);
+        const double s = 1.0 / (double)t;
+        const double sg = (t & 1) ? 1.0 : -1.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                P[c][r] = Q[c][r];
+                res[c][r] += Q[c][r] * s * sg;
+            }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) out[c][r] = 0.5 * res[c][r];
+}
+
+/* getExp (ConstitutiveRelation_MatrixAlgebra.cpp:25-37): the sum over
+ * t = 0..8 of A^t / t!, added term by term. */
+__device__ __forceinline__ void exp_series(const double A[3][3], double out[3][3]) {
+    double P[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            P[c][r] = c == r ? 1.0 : 0.0;
+            out[c][r] = 0.0 + P[c][r] / 1.0;
+        }
+    double f = 1.0;
+#pragma unroll
+    for (int t = 1; t <= 8; t++) {
+        double Q[3][3];
+        m3_mul(P, A, Q);
+        f *= (double)t;
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                P[c][r] = Q[c][r];
+                out[c][r] += Q[c][r] / f;
+            }
+    }
+}
+
+/* Voigt vector with engineering shears (DP:325-330, 348-361). */
+__device__ __forceinline__ void m3_voigt(const double m[3][3], double v[6]) {
+    v[0] = m[0][0];
+    v[1] = m[1][1];
+    v[2] = m[2][2];
+    v[3] = m[0][1] + m[1][0];
+    v[4] = m[1][2] + m[2][1];
+    v[5] = m[2][0] + m[0][2];
+}
+
+/* runSimulation_G2P_P2_LargeStrain after the node gather (DP:303-435) for
+ * one particle: stress reset, dF = I + L dt, the log strains of F (before
+ * the step), of dF Fe and of dF; then, for a particle with a material, the
+ * return map on the elastic trial strain from zero stress, Fe = exp(trial
+ * minus plastic strain, tensor shears), F = F dF, V = det F V0, strain =
+ * log strain of the old F, stress = returned stress / det F, plastic strain,
+ * energies (sigma[k] * increment[k] * V added one component at a time) and
+ * hardening.  A particle with no material keeps everything but the zeroed
+ * stress (DP:373-377). */
+__device__ __forceinline__ void large_strain_update(const PView &P, const MatTable &mt, uint32_t i, const KinOut &ko,
+                                                    uint32_t &f_null, uint32_t &f_plastic, uint32_t &f_nonfinite) {
+    const size_t n = P.n;
+    const int mat = ko.mat;
+    const bool valid = mat >= 0 && mat < mt.n;
+    f_nonfinite = ko.flag;
+    if (!valid) {
+        f_null = 1;
+#pragma unroll
+        for (int k = 0; k < 6; k++) P.sig[k * n + i] = 0.0;
+        return;
+    }
+    double Fe[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) Fe[c][r] = P.Fe[(3 * c + r) * n + i];
+    const double R = P.R[i];
+    double epsp[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) epsp[k] = P.epsp[k * n + i];
+    const double Ws0 = P.Ws[i], Wp0 = P.Wp[i];
+    const double dt = ko.dt;
+    double dF[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) dF[c][r] = (c == r ? 1.0 : 0.0) + ko.L[c][r] * dt;
+    double S[3][3], etot[6], etr[6], einc[6];
+    half_log_aat(ko.F, S);
+    m3_voigt(S, etot);
+    {
+        double FeT[3][3];
+        m3_mul(dF, Fe, FeT);
+        half_log_aat(FeT, S);
+        m3_voigt(S, etr);
+    }
+    half_log_aat(dF, S);
+    m3_voigt(S, einc);
+    const double zero[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    CRResult cr;
+    constitutive(mt, mat, R, zero, etr, cr);
+    f_plastic = cr.yielded;
+    double eel[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) eel[k] = etr[k] - cr.depsp[k];
+    {
+        double Se[3][3];
+        Se[0][0] = eel[0];
+        Se[1][1] = eel[1];
+        Se[2][2] = eel[2];
+        Se[0][1] = 0.5 * eel[3];
+        Se[1][0] = 0.5 * eel[3];
+        Se[1][2] = 0.5 * eel[4];
+        Se[2][1] = 0.5 * eel[4];
+        Se[2][0] = 0.5 * eel[5];
+        Se[0][2] = 0.5 * eel[5];
+        exp_series(Se, Fe);
+    }
+    double Fn[3][3];
+    m3_mul(ko.F, dF, Fn);
+    const double det = +Fn[0][0] * (Fn[1][1] * Fn[2][2] - Fn[2][1] * Fn[1][2]) -
+                       Fn[1][0] * (Fn[0][1] * Fn[2][2] - Fn[2][1] * Fn[0][2]) +
+                       Fn[2][0] * (Fn[0][1] * Fn[1][2] - Fn[1][1] * Fn[0][2]);
+    const double V = det * ko.V0;
+    double sig[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) sig[k] = (1.0 / det) * cr.dsig[k];
+    double Ws = Ws0, Wp = Wp0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) Ws += sig[k] * einc[k] * V;
+#pragma unroll
+    for (int k = 0; k < 6; k++) Wp += sig[k] * cr.depsp[k] * V;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            P.Fe[(3 * c + r) * n + i] = Fe[c][r];
+            P.F[(3 * c + r) * n + i] = Fn[c][r];
+        }
+    P.V[i] = V;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        P.eps[k * n + i] = etot[k];
+        P.sig[k * n + i] = sig[k];
+        P.epsp[k * n + i] = epsp[k] + cr.depsp[k];
+    }
+    P.Ws[i] = Ws;
+    P.Wp[i] = Wp;
+    P.R[i] = R + cr.dR;
+    uint32_t bad = 0;
+    for (int k = 0; k < 6; k++) bad |= !isfinite(sig[k]);
+    f_nonfinite |= bad;
+}
+
+/* G2P pass 2 of the DoublePass loop with the large-strain formulation: the
+ * kinematics of k_g2p_fused (x, L, control schedule, next cell) and then
+ * large_strain_update in the same thread. */
+template <bool MULTI>
+__global__ __launch_bounds__(BLOCK) void k_g2p_large(Batch bt, PView P, NView Nd, BView B, MatTable mt,
+                                                    unsigned long long *health) {
+    uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    const bool live = i < P.n;
+    uint32_t f_outside = 0, f_null = 0, f_plastic = 0, f_nonfinite = 0;
+    uint32_t next_key = MPM_NONE;
+    if (live) {
+        const double *g2 = Nd.g2;
+        KinOut ko;
+        g2p_kinematics_particle<VAR_DP, false, true>(bt, P, inst_p<MULTI>(bt, i), i,
+                                                     [=](int, size_t node, double2 &r0, double2 &r1, double2 &r2,
+                                                         double2 &r3) {
+                                                         const double2 *q =
+                                                             reinterpret_cast<const double2 *>(g2 + node * 8u);
+                                                         r0 = q[0];
+                                                         r1 = q[1];
+                                                         r2 = q[2];
+                                                         r3 = q[3];
+                                                     },
+                                                     next_key, f_outside, ko);
+        B.key[i] = next_key;
+        large_strain_update(P, mt, i, ko, f_null, f_plastic, f_nonfinite);
     }
     wave_key_add(B.count, next_key, next_key != MPM_NONE);
     block_count_add(&health[HEALTH_OUTSIDE], f_outside);
@@ -2125,6 +2358,12 @@ template <int VAR>
 static void launch_g2p_strain(int mode, dim3 grid, hipStream_t s, const Batch &bt, const PView &P, const NView &N,
                               const BView &B, const MatTable &mt, unsigned long long *health, const SurrogateArgs &sg,
                               hipEvent_t mid) {
+    if (VAR == VAR_DP && bt.strain == STRAIN_LARGE) {
+        /* the engine allows the large-strain pass with the exact map only */
+        LAUNCH_MULTI(bt, (k_g2p_large<M>), grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health);
+        if (mid) (void)hipEventRecord(mid, s);
+        return;
+    }
     /* one instance, exact map: the fused pass (with a batch's per-particle
      * instance lookups it no longer fits the registers of two waves) */
     if (mode == MODE_EXACT && bt.ninst == 1) {

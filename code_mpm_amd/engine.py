@@ -55,6 +55,15 @@ class Engine:
             self._check(self._lib.mpm_set_variant(self._h, capi.VARIANT[variant]), "set_variant")
             self.variant = variant
 
+    def set_strain_formulation(self, formulation):
+        """G2P pass 2 of the DoublePass loop: "small"
+        (runSimulation_G2P_P2_SmallStrain, the reference's) or "large"
+        (runSimulation_G2P_P2_LargeStrain, logarithmic strains; DoublePass
+        loop and exact return map only)."""
+        self._check(self._lib.mpm_set_strain_formulation(self._h, capi.STRAIN[formulation]),
+                    "set_strain_formulation")
+        self.strain = formulation
+
     # -- lifetime ---------------------------------------------------------
     def close(self):
         if getattr(self, "_h", None):

@@ -82,7 +82,9 @@ enum {
     MPM_PF_ENERGY_STRAIN_MAX = 12,  /* 1  d_Energy_Strain_MaxHistory */
     MPM_PF_ENERGY_STRAIN_TI = 13,   /* 1  d_Energy_Strain_TimeIntegral */
     MPM_PF_ENERGY_PLASTIC = 14,     /* 1 */
-    MPM_PF_COUNT = 15
+    MPM_PF_DEFORMATION_GRADIENT_ELASTIC = 15, /* 9  d33_DeformationGradient_Elastic (MaterialPoint_BC.h:65),
+                                                 used by the large-strain pass only */
+    MPM_PF_COUNT = 16
 };
 
 /* Node fields (GridPoint, GridPoint.h:6-45), FP64, state after the last step.
@@ -251,6 +253,25 @@ int mpm_set_summation_order(mpm_engine *e, int32_t order);
 enum { MPM_VARIANT_DOUBLEPASS = 0, MPM_VARIANT_DOUBLEPASS_CONTACT = 1, MPM_VARIANT_SINGLEPASS = 2 };
 int mpm_set_variant(mpm_engine *e, int32_t variant);
 int mpm_get_variant(const mpm_engine *e, int32_t *variant);
+
+/* Which grid-to-particle pass 2 the DoublePass loop runs:
+ * MPM_STRAIN_SMALL (default)
+ *   runSimulation_G2P_P2_SmallStrain
+ *   (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:175-280);
+ * MPM_STRAIN_LARGE
+ *   runSimulation_G2P_P2_LargeStrain (same file :281-437; the loop's call
+ *   to it, :605, is commented out in the reference): logarithmic strains
+ *   from 8-term series (ConstitutiveRelation_MatrixAlgebra.cpp:25-53), the
+ *   return map on the elastic trial strain from zero stress, Fe = exp(the
+ *   elastic strain), stress = returned stress / det F, strain = the log
+ *   strain of F before the step.  A particle with no material gets zero
+ *   stress and is otherwise left as the reference leaves it.
+ * The large-strain pass exists for the DoublePass loop and the exact return
+ * map only: a step with another variant, a surrogate or sample capture set
+ * fails with MPM_ERR_STATE. */
+enum { MPM_STRAIN_SMALL = 0, MPM_STRAIN_LARGE = 1 };
+int mpm_set_strain_formulation(mpm_engine *e, int32_t formulation);
+int mpm_get_strain_formulation(const mpm_engine *e, int32_t *formulation);
 
 /* --- node tiles and the autotuner (no counterpart in the reference) -------- */
 

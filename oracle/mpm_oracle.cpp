@@ -75,6 +75,71 @@ static inline double m3_det(const M3 &m) {
            - m.m[1][0] * (m.m[0][1] * m.m[2][2] - m.m[2][1] * m.m[0][2])
            + m.m[2][0] * (m.m[0][1] * m.m[1][2] - m.m[1][1] * m.m[0][2]);
 }
+static inline M3 m3_transpose(const M3 &a) {
+    M3 r;
+    for (int c = 0; c < 3; c++)
+        for (int k = 0; k < 3; k++) r.m[c][k] = a.m[k][c];
+    return r;
+}
+
+/* getFactorial, ConstitutiveRelation_MatrixAlgebra.cpp:3-11. */
+static double series_factorial(int i) {
+    double r = 1.0;
+    for (int k = 1; k <= i; k++) r *= k;
+    return r;
+}
+/* getPower, ConstitutiveRelation_MatrixAlgebra.cpp:12-23: the identity
+ * multiplied on the right by A, p times (I * A is formed, not taken as A). */
+static M3 m3_power(const M3 &A, int p) {
+    M3 r = m3_identity();
+    for (int k = 1; k <= p; k++) r = m3_mul(r, A);
+    return r;
+}
+/* getExp, ConstitutiveRelation_MatrixAlgebra.cpp:25-37: the sum over
+ * t = 0..8 of A^t / t!, added term by term, element by element. */
+static M3 m3_exp_series(const M3 &A) {
+    M3 r = m3_zero();
+    for (int t = 0; t <= 8; t++) {
+        M3 P = m3_power(A, t);
+        double f = series_factorial(t);
+        for (int c = 0; c < 3; c++)
+            for (int k = 0; k < 3; k++) r.m[c][k] += P.m[c][k] / f;
+    }
+    return r;
+}
+/* getLog, ConstitutiveRelation_MatrixAlgebra.cpp:39-53: the sum over
+ * t = 1..8 of ((A - I)^t * (1/t)) * (-1)^(t+1). */
+static M3 m3_log_series(const M3 &A) {
+    M3 I = m3_identity(), D;
+    for (int c = 0; c < 3; c++)
+        for (int k = 0; k < 3; k++) D.m[c][k] = A.m[c][k] - I.m[c][k];
+    M3 r = m3_zero();
+    for (int t = 1; t <= 8; t++) {
+        M3 P = m3_power(D, t);
+        double s = 1.0 / t;
+        double sg = std::pow(-1.0, t + 1);
+        for (int c = 0; c < 3; c++)
+            for (int k = 0; k < 3; k++) r.m[c][k] += P.m[c][k] * s * sg;
+    }
+    return r;
+}
+/* 0.5 * getLog(A * transpose(A)), the strain measure of DP:323 and DP:342-343. */
+static M3 m3_half_log_aat(const M3 &A) {
+    M3 r = m3_log_series(m3_mul(A, m3_transpose(A)));
+    for (int c = 0; c < 3; c++)
+        for (int k = 0; k < 3; k++) r.m[c][k] = 0.5 * r.m[c][k];
+    return r;
+}
+/* Voigt vector with engineering shears, DP:325-330 / DP:348-361. */
+static void m3_voigt(const M3 &m, double v[6]) {
+    v[0] = m.m[0][0];
+    v[1] = m.m[1][1];
+    v[2] = m.m[2][2];
+    v[3] = m.m[0][1] + m.m[1][0];
+    v[4] = m.m[1][2] + m.m[2][1];
+    v[5] = m.m[2][0] + m.m[0][2];
+}
+
 static inline double glm_sign(double x) { return (double)(0.0 < x) - (double)(x < 0.0); }
 static inline double glm_dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
@@ -99,6 +164,7 @@ struct Particle {
     double m = 0.0, V = 0.0, V0 = 0.0;
     V3 x, v, fext;
     M3 F = m3_identity();
+    M3 Fe = m3_identity(); /* d33_DeformationGradient_Elastic, large-strain pass only */
     double eps[6] = {0, 0, 0, 0, 0, 0};
     double sig[6] = {0, 0, 0, 0, 0, 0};
     double epsp[6] = {0, 0, 0, 0, 0, 0};
@@ -222,6 +288,47 @@ static void cr_dispatch(CR &o, const Material &mt, double R, const double sig[6]
         cr_vmh(o, mt, R, sig, de);
     else
         cr_plastic(o, mt, sig, de);
+}
+
+/* runSimulation_G2P_P2_LargeStrain after the node gather, DP:303-435, for
+ * one particle with velocity gradient L; mt is NULL for a particle with no
+ * material (DP:373-377: its stress was already reset, nothing else
+ * changes).  Returns false for that case. */
+static bool large_strain_point(Particle *p, const Material *mt, const M3 &L, double dt, CR &cr) {
+    for (int k = 0; k < 6; k++) p->sig[k] = 0.0; /* DP:305-306 */
+    M3 I = m3_identity(), dF;
+    for (int c = 0; c < 3; c++)
+        for (int r = 0; r < 3; r++) dF.m[c][r] = I.m[c][r] + L.m[c][r] * dt; /* DP:308 */
+    double etot[6], etr[6], einc[6];
+    m3_voigt(m3_half_log_aat(p->F), etot);          /* DP:320-331, F before the step */
+    M3 FeT = m3_mul(dF, p->Fe);                     /* DP:337-339 */
+    m3_voigt(m3_half_log_aat(FeT), etr);            /* DP:342, DP:348-353 */
+    m3_voigt(m3_half_log_aat(dF), einc);            /* DP:343, DP:355-360 */
+    if (!mt) return false;
+    cr_dispatch(cr, *mt, p->R, p->sig, etr);        /* DP:379-386, from zero stress */
+    double eel[6];
+    for (int k = 0; k < 6; k++) eel[k] = etr[k] - cr.depsp[k]; /* DP:390-392 */
+    M3 Sel = m3_zero();
+    Sel.m[0][0] = eel[0];
+    Sel.m[1][1] = eel[1];
+    Sel.m[2][2] = eel[2];
+    Sel.m[0][1] = 0.5 * eel[3];
+    Sel.m[1][0] = 0.5 * eel[3];
+    Sel.m[1][2] = 0.5 * eel[4];
+    Sel.m[2][1] = 0.5 * eel[4];
+    Sel.m[2][0] = 0.5 * eel[5];
+    Sel.m[0][2] = 0.5 * eel[5];
+    p->Fe = m3_exp_series(Sel);                     /* DP:410 */
+    p->F = m3_mul(p->F, dF);                        /* DP:412 */
+    double det = m3_det(p->F);                      /* DP:414-415 */
+    p->V = det * p->V0;
+    for (int k = 0; k < 6; k++) p->eps[k] = etot[k];                     /* DP:417-418 */
+    for (int k = 0; k < 6; k++) p->sig[k] = (1.0 / det) * cr.dsig[k];   /* DP:423-424 */
+    for (int k = 0; k < 6; k++) p->epsp[k] += cr.depsp[k];              /* DP:426-427 */
+    for (int k = 0; k < 6; k++) p->Ws += p->sig[k] * einc[k] * p->V;    /* DP:429-430 */
+    for (int k = 0; k < 6; k++) p->Wp += p->sig[k] * cr.depsp[k] * p->V; /* DP:432-433 */
+    p->R += cr.dR;                                  /* DP:435 */
+    return true;
 }
 
 struct Timeline {
@@ -706,6 +813,49 @@ struct oracle_engine {
 
     /* ---- the two other step loops ------------------------------------------ */
     int variant = MPM_VARIANT_DOUBLEPASS;
+    int strain = MPM_STRAIN_SMALL;
+
+    /* runSimulation_G2P_P2_LargeStrain, DP:281-437: the gather of x and L
+     * as in the small-strain pass (DP:287-303), then large_strain_point. */
+    void g2p_strain_large(double dt, int64_t *null_count, int64_t *plastic_count, int64_t *outside_count,
+                          int64_t *nonfinite_count) {
+#pragma omp for
+        for (long i = 0; i < (long)particles.size(); i++) {
+            Particle *p = particles[i];
+            if (p->agp.empty()) {
+#pragma omp atomic
+                (*outside_count)++;
+            }
+            M3 L = m3_zero();
+            for (size_t k = 0; k < p->agp.size(); k++) {
+                const AGP &a = p->agp[k];
+                Node *n = nodes[a.index];
+                if (n->m > mass_min) {
+                    p->x.x += a.N * (n->p.x / n->m) * dt;
+                    p->x.y += a.N * (n->p.y / n->m) * dt;
+                    p->x.z += a.N * (n->p.z / n->m) * dt;
+                }
+                double vv[3] = {n->v.x, n->v.y, n->v.z};
+                double gg[3] = {a.g.x, a.g.y, a.g.z};
+                for (int c = 0; c < 3; c++)
+                    for (int r = 0; r < 3; r++) L.m[c][r] += vv[r] * gg[c];
+            }
+            const bool valid = p->material >= 0 && p->material < (int)materials.size();
+            CR cr;
+            large_strain_point(p, valid ? &materials[p->material] : nullptr, L, dt, cr);
+            if (!valid) {
+#pragma omp atomic
+                (*null_count)++;
+            } else if (cr.yielded) {
+#pragma omp atomic
+                (*plastic_count)++;
+            }
+            if (!finite_state(p)) {
+#pragma omp atomic
+                (*nonfinite_count)++;
+            }
+        }
+    }
 
     /* Grid update of the contact variant, DPC:242-277: damping along p (as
      * DP:100-101), momentum integrated first, then the fixed components
@@ -976,7 +1126,10 @@ struct oracle_engine {
 #pragma omp barrier
             reapply_bc();
 #pragma omp barrier
-            g2p_strain(dt, &nulls, &plastic, &outside, &nonfinite);
+            if (strain == MPM_STRAIN_LARGE)
+                g2p_strain_large(dt, &nulls, &plastic, &outside, &nonfinite);
+            else
+                g2p_strain(dt, &nulls, &plastic, &outside, &nonfinite);
 #pragma omp barrier
             dc_schedule();
         }
@@ -1148,7 +1301,7 @@ int oracle_set_particle_count(oracle_engine *e, int64_t n) {
 }
 
 static int field_components(int field) {
-    static const int nc[MPM_PF_COUNT] = {3, 3, 3, 1, 1, 1, 9, 6, 6, 6, 1, 1, 1, 1, 1};
+    static const int nc[MPM_PF_COUNT] = {3, 3, 3, 1, 1, 1, 9, 6, 6, 6, 1, 1, 1, 1, 1, 9};
     if (field < 0 || field >= MPM_PF_COUNT) return -1;
     return nc[field];
 }
@@ -1170,6 +1323,7 @@ static double *particle_slot(Particle *p, int field, int c) {
     case MPM_PF_ENERGY_STRAIN_MAX: return &p->Wmax;
     case MPM_PF_ENERGY_STRAIN_TI: return &p->Wti;
     case MPM_PF_ENERGY_PLASTIC: return &p->Wp;
+    case MPM_PF_DEFORMATION_GRADIENT_ELASTIC: return &p->Fe.m[c / 3][c % 3];
     }
     return nullptr;
 }
@@ -1254,8 +1408,15 @@ int oracle_set_variant(oracle_engine *e, int32_t variant) {
     return MPM_OK;
 }
 
+int oracle_set_strain_formulation(oracle_engine *e, int32_t f) {
+    if (!e || f < MPM_STRAIN_SMALL || f > MPM_STRAIN_LARGE) return MPM_ERR_ARG;
+    e->strain = f;
+    return MPM_OK;
+}
+
 int oracle_step(oracle_engine *e, double dt, int32_t nsteps, int32_t nthreads) {
     if (!e || nsteps < 0 || nthreads < 1 || e->nodes.empty()) return MPM_ERR_ARG;
+    if (e->strain == MPM_STRAIN_LARGE && e->variant != MPM_VARIANT_DOUBLEPASS) return MPM_ERR_STATE;
     for (int s = 0; s < nsteps; s++) e->step(dt, nthreads);
     return MPM_OK;
 }
@@ -1264,6 +1425,7 @@ int oracle_step(oracle_engine *e, double dt, int32_t nsteps, int32_t nthreads) {
  * (console report left to the caller). */
 int oracle_run(oracle_engine *e, double dt_total, int32_t nthreads, int32_t *finished) {
     if (!e || nthreads < 1 || e->nodes.empty()) return MPM_ERR_ARG;
+    if (e->strain == MPM_STRAIN_LARGE && e->variant != MPM_VARIANT_DOUBLEPASS) return MPM_ERR_STATE;
     double acc = 0.0;
     while (acc < dt_total) {
         double dt = e->dt_max;
@@ -1426,6 +1588,76 @@ int oracle_constitutive_batch(const mpm_material *m, const double *R, const doub
         }
         dR[i] = o.dR;
     }
+    return MPM_OK;
+}
+
+/* getLog / getExp of one glm column-major 3x3 (component 3 * col + row). */
+static M3 m3_from(const double *a) {
+    M3 m;
+    for (int c = 0; c < 3; c++)
+        for (int r = 0; r < 3; r++) m.m[c][r] = a[3 * c + r];
+    return m;
+}
+static void m3_to(const M3 &m, double *a) {
+    for (int c = 0; c < 3; c++)
+        for (int r = 0; r < 3; r++) a[3 * c + r] = m.m[c][r];
+}
+int oracle_matrix_log(const double *a, double *out) {
+    if (!a || !out) return MPM_ERR_ARG;
+    m3_to(m3_log_series(m3_from(a)), out);
+    return MPM_OK;
+}
+int oracle_matrix_exp(const double *a, double *out) {
+    if (!a || !out) return MPM_ERR_ARG;
+    m3_to(m3_exp_series(m3_from(a)), out);
+    return MPM_OK;
+}
+
+/* The large-strain update of one particle with velocity gradient L (glm
+ * column-major) and step dt.  state (in/out, 41 doubles): F 9, Fe 9,
+ * sigma 6, strain 6, plastic strain 6, R, V, V0, strain energy, plastic
+ * energy.  m = NULL: a particle with no material.  *yielded: the return
+ * map yielded. */
+int oracle_large_strain_point(const mpm_material *m, const double *L, double dt, double *state, int32_t *yielded) {
+    if (!L || !state) return MPM_ERR_ARG;
+    Material mt;
+    if (m) {
+        mt.type = m->type;
+        mt.density = m->density;
+        mt.E = m->elastic_modulus;
+        mt.nu = m->poisson_ratio;
+        mt.yield = m->yield_stress;
+        mt.c0 = m->hardening_c0;
+        mt.c1 = m->hardening_c1;
+    }
+    Particle p;
+    p.F = m3_from(state);
+    p.Fe = m3_from(state + 9);
+    for (int k = 0; k < 6; k++) {
+        p.sig[k] = state[18 + k];
+        p.eps[k] = state[24 + k];
+        p.epsp[k] = state[30 + k];
+    }
+    p.R = state[36];
+    p.V = state[37];
+    p.V0 = state[38];
+    p.Ws = state[39];
+    p.Wp = state[40];
+    CR cr;
+    large_strain_point(&p, m ? &mt : nullptr, m3_from(L), dt, cr);
+    m3_to(p.F, state);
+    m3_to(p.Fe, state + 9);
+    for (int k = 0; k < 6; k++) {
+        state[18 + k] = p.sig[k];
+        state[24 + k] = p.eps[k];
+        state[30 + k] = p.epsp[k];
+    }
+    state[36] = p.R;
+    state[37] = p.V;
+    state[38] = p.V0;
+    state[39] = p.Ws;
+    state[40] = p.Wp;
+    if (yielded) *yielded = cr.yielded ? 1 : 0;
     return MPM_OK;
 }
 

@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 
-from code_mpm_amd.capi import MON, NF, PF, Health, Material, Monitors, make_material  # noqa: E402
+from code_mpm_amd.capi import MON, NF, PF, STRAIN, Health, Material, Monitors, make_material  # noqa: E402
 
 _lib = None
 
@@ -61,6 +61,10 @@ def load():
         "oracle_bases": [P, P, P, P, P],
         "oracle_constitutive_batch": [ctypes.POINTER(Material), P, P, P, P, P, P, I64],
         "oracle_timeline_velocity": [P, P, I32, DBL, P],
+        "oracle_set_strain_formulation": [P, I32],
+        "oracle_matrix_log": [P, P],
+        "oracle_matrix_exp": [P, P],
+        "oracle_large_strain_point": [P, P, DBL, P, P],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -171,6 +175,11 @@ class OracleEngine:
         _ok(self._lib.oracle_set_variant(self._h, VARIANT[variant]), "variant")
         self.variant = variant
 
+    def set_strain_formulation(self, formulation):
+        """small | large: runSimulation_G2P_P2_SmallStrain or _LargeStrain."""
+        _ok(self._lib.oracle_set_strain_formulation(self._h, STRAIN[formulation]), "strain formulation")
+        self.strain = formulation
+
     def step(self, dt, nsteps=1):
         _ok(self._lib.oracle_step(self._h, float(dt), int(nsteps), self.threads), "step")
 
@@ -249,3 +258,48 @@ def timeline_velocity(t, v, time):
     out = np.zeros(3)
     _ok(lib.oracle_timeline_velocity(_p(T), _p(V), len(T), float(time), _p(out)), "timeline")
     return out
+
+
+def _m3(a):
+    """3x3 (row, col) array -> glm column-major 9-vector."""
+    return np.ascontiguousarray(np.asarray(a, np.float64).T.reshape(9))
+
+
+def matrix_log(a):
+    """ConstitutiveRelation::getLog of a 3x3 matrix (8-term series)."""
+    out = np.empty(9)
+    _ok(load().oracle_matrix_log(_p(_m3(a)), _p(out)), "matrix_log")
+    return out.reshape(3, 3).T.copy()
+
+
+def matrix_exp(a):
+    """ConstitutiveRelation::getExp of a 3x3 matrix (9-term series)."""
+    out = np.empty(9)
+    _ok(load().oracle_matrix_exp(_p(_m3(a)), _p(out)), "matrix_exp")
+    return out.reshape(3, 3).T.copy()
+
+
+LARGE_STATE = (("F", 9), ("Fe", 9), ("sig", 6), ("eps", 6), ("epsp", 6), ("R", 1), ("V", 1), ("V0", 1),
+               ("Ws", 1), ("Wp", 1))
+
+
+def large_strain_point(material, L, dt, state):
+    """One particle's large-strain update after the node gather.  L is the
+    3x3 velocity gradient (row, col); state is a dict with F and Fe (3x3,
+    row, col), sig / eps / epsp (6) and R, V, V0, Ws, Wp; material None =
+    no material.  Returns (new state, yielded)."""
+    flat = []
+    for k, nc in LARGE_STATE:
+        v = state[k]
+        flat.extend(_m3(v) if nc == 9 else np.atleast_1d(np.asarray(v, np.float64)).tolist())
+    buf = np.ascontiguousarray(flat, np.float64)
+    y = np.zeros(1, np.int32)
+    m = None if material is None else make_material(material)
+    mp = None if m is None else ctypes.byref(m)
+    _ok(load().oracle_large_strain_point(mp, _p(_m3(L)), float(dt), _p(buf), _p(y)), "large_strain_point")
+    out, o = {}, 0
+    for k, nc in LARGE_STATE:
+        v = buf[o:o + nc]
+        out[k] = v.reshape(3, 3).T.copy() if nc == 9 else (v.copy() if nc > 1 else float(v[0]))
+        o += nc
+    return out, bool(y[0])
