@@ -275,6 +275,37 @@ def c4_bench(steps, fd_steps, every=250, fd_every=200, train_steps=1500):
     }
 
 
+def beso_bench(n_designs):
+    """One BESO iteration (Main.cpp:204-333) on the L-problem canvas of
+    Main.cpp:127-145 (500 x 500 voxels of 0.1 mm, 57,636 particles per
+    design, 16,000 steps to the end time), n_designs designs with different
+    filter radii run as one batch of instances; then the objective update,
+    the GPU smoothing filter and the soft-kill / restore pass per design."""
+    from code_mpm_amd import beso, capi
+    designs = [beso.Design(beso.l_problem(), radius=0.0012 * (1.0 + 0.25 * k)) for k in range(n_designs)]
+    run = beso.BESO(designs, order="cell")
+    t = time.perf_counter()
+    lines = run.step()
+    el = time.perf_counter() - t
+    n = sum(sc.n for sc in run.last_scenes)
+    steps = run.timing["steps"]
+    cv = designs[0].canvas
+    capi.eso_filter(cv.active, cv.eso, cv.objective, cv.nx, cv.ny, cv.d_Offset, designs[0].radius)
+    t1 = time.perf_counter()
+    for _ in range(5):
+        capi.eso_filter(cv.active, cv.eso, cv.objective, cv.nx, cv.ny, cv.d_Offset, designs[0].radius)
+    filt = (time.perf_counter() - t1) / 5
+    out = {"designs": n_designs, "particles": n, "steps": steps, "iteration_s": round(el, 3),
+           "simulate_s": round(run.timing["simulate_s"], 3),
+           "value": round(n * steps / run.timing["simulate_s"], 1), "unit": "particle-steps/s",
+           "filter_ms": round(1e3 * filt, 3), "filter_note": "mpm_eso_filter on 500 x 500 voxels incl. host copies",
+           "removed": [d.removed for d in designs], "restored": [d.restored for d in designs],
+           "log_first_design": lines[0].strip()}
+    print(f"# beso: {n_designs} designs x {n // n_designs} particles, {steps} steps in {run.timing['simulate_s']:.2f} s, "
+          f"filter {1e3 * filt:.2f} ms", file=sys.stderr, flush=True)
+    return out, cv, designs[0].radius
+
+
 def sc_speed(scene):
     return scene.info["impact_speed"]
 
@@ -305,6 +336,8 @@ def main():
                     help="C4 half-ring impact at ~48 M particles: steps each way, exact and learned (0: skip)")
     ap.add_argument("--c4-fd-steps", type=int, default=8000,
                     help="C4 force-displacement comparison at 4 h: steps each way")
+    ap.add_argument("--beso-designs", type=int, default=4,
+                    help="BESO designs batched in one iteration of the L-problem (0: skip)")
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="first run one device copy of this many GiB (the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -431,6 +464,7 @@ def main():
     batch = batch_bench(args.batch_instances, args.batch_steps) if args.batch_instances > 0 else None
     sur = surrogate_bench(scenes.honeycomb_crush(), args.surrogate_steps) if args.surrogate_steps > 0 else None
     c4 = c4_bench(args.c4_steps, args.c4_fd_steps) if args.c4_steps > 0 else None
+    beso_out, beso_canvas, beso_radius = beso_bench(args.beso_designs) if args.beso_designs > 0 else (None, None, None)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -439,6 +473,15 @@ def main():
         cpu = {"value": round(v_cpu, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
                "sample": f"x < 64 h slice of the same honeycomb ({n_cpu} particles, 64x512x32 cells), "
                          f"{s_cpu} steps after 3 warm-up, {el_cpu:.1f} s, OpenMP lock scatter, g++ -O2"}
+        if beso_out is not None:
+            # the reference's all-pairs smoothing loop (Canvas2D_CC.cpp:302-341), one thread, same canvas
+            from oracle import oracle as orc
+            cv = beso_canvas
+            t1 = time.perf_counter()
+            orc.eso_filter(cv.active, cv.eso, cv.objective, cv.nx, cv.ny, cv.d_Offset, beso_radius)
+            beso_out["cpu_filter_s"] = round(time.perf_counter() - t1, 3)
+            print(f"# beso filter on the host (all pairs, 1 thread): {beso_out['cpu_filter_s']:.2f} s",
+                  file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
@@ -465,6 +508,7 @@ def main():
         "batch": batch,
         "surrogate": sur,
         "c4": c4,
+        "beso": beso_out,
         "cpu_baseline": cpu,
         "gpu_over_cpu": round(value / cpu["value"], 1) if cpu else None,
     }

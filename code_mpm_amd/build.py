@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmpm_amd.so")
 
-SOURCES = ["mpm_kernels.hip", "mpm_engine.hip", "mpm_mlp.hip", "mpm_surrogate.hip"]
+SOURCES = ["mpm_kernels.hip", "mpm_engine.hip", "mpm_mlp.hip", "mpm_surrogate.hip", "mpm_eso.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -116,6 +116,7 @@ SIGNATURES = {
     "mpm_set_variant": (I32, [P, I32]),
     "mpm_get_variant": (I32, [P, PI32]),
     "mpm_set_strain_formulation": (I32, [P, I32]),
+    "mpm_eso_filter": (I32, [I32, P, P, P, I32, I32, DBL, DBL, P]),
     "mpm_get_strain_formulation": (I32, [P, PI32]),
     "mpm_set_tile_shape": (I32, [P, PI32]),
     "mpm_get_tile_shape": (I32, [P, PI32]),
@@ -214,3 +215,18 @@ def constitutive_batch(material, backstress, stress, dstrain, device=0):
     rc = lib.mpm_constitutive_batch(device, ctypes.byref(mat), ptr(R), ptr(s), ptr(d), ptr(ds), ptr(dp), ptr(dR), n)
     check(rc, None, "mpm_constitutive_batch")
     return ds, dp, dR
+
+
+def eso_filter(active, eso, objective, nx, ny, offset, radius, device=0):
+    """Canvas2D_CC::filterObjective_Smooth on the GPU; arrays of nx * ny
+    voxels (index x + y * nx).  Returns the new objective array."""
+    lib = load()
+    a = np.ascontiguousarray(active, dtype=np.uint8)
+    e = np.ascontiguousarray(eso, dtype=np.uint8)
+    o = np.ascontiguousarray(objective, dtype=np.float64)
+    assert a.size == e.size == o.size == nx * ny
+    out = np.empty(nx * ny)
+    rc = lib.mpm_eso_filter(int(device), ptr(a), ptr(e), ptr(o), int(nx), int(ny), float(offset), float(radius),
+                            ptr(out))
+    check(rc, None, "mpm_eso_filter")
+    return out

@@ -268,6 +268,49 @@ class Engine:
         self._check(self._lib.mpm_run(self._h, float(dt_total), ctypes.byref(fin)), "run")
         return fin.value
 
+    def run_batch(self, dt_total, use_graph=False):
+        """runSimulation_Classic_DoublePass_MPLocks(dt_total)
+        (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:450-458,
+        802-814) for every instance of a batch whose instances share one
+        d_TimeIncrement_Maximum: steps of dt_max with the last one clamped
+        to the remainder, the request accumulated in floating point as the
+        reference does.  Runs of full steps go to the device in one call
+        (replayed from a HIP graph with use_graph); a clamped step sets every
+        instance's step size for that one step.  Returns 0 while any
+        instance's time is below its end time, else 1."""
+        if not getattr(self, "batch", None):
+            raise RuntimeError("run_batch needs a batch (load_batch)")
+        dts = {float(sc.dt) for sc in self.batch}
+        if len(dts) != 1:
+            raise ValueError("run_batch needs one dt_max for every instance")
+        dt_max = dts.pop()
+        acc, full = 0.0, 0
+
+        def flush():
+            nonlocal full
+            if full:
+                (self.step_graph if use_graph else (lambda k: self.step(0.0, k)))(full)
+            full = 0
+
+        while acc < dt_total:
+            dt = dt_max
+            if dt_max > (dt_total - acc):
+                dt = dt_total - acc
+            acc += dt
+            if dt == dt_max:
+                full += 1
+                continue
+            flush()
+            for k, sc in enumerate(self.batch):
+                self._check(self._lib.mpm_set_instance_time(self._h, k, self.instance_time(k), dt, float(sc.time_end)),
+                            "instance time")
+            self.step(0.0, 1)
+            for k, sc in enumerate(self.batch):
+                self._check(self._lib.mpm_set_instance_time(self._h, k, self.instance_time(k), dt_max,
+                                                            float(sc.time_end)), "instance time")
+        flush()
+        return 0 if any(self.instance_time(k) < sc.time_end for k, sc in enumerate(self.batch)) else 1
+
     def synchronize(self):
         self._check(self._lib.mpm_synchronize(self._h), "synchronize")
 

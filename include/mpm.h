@@ -366,6 +366,20 @@ int mpm_constitutive_batch(int32_t device, const mpm_material *material, const d
                            const double *stress, const double *dstrain, double *dstress,
                            double *dstrain_plastic, double *dbackstress, int64_t n);
 
+/* --- BESO objective filter ---------------------------------------------------- */
+
+/* Canvas2D_CC::filterObjective_Smooth (Canvas2D_CC.cpp:302-341) on device
+ * `device`, host arrays in and out, for an nx x ny voxel canvas (voxel
+ * index x + y * nx, centre (i + 0.5) * offset per axis, Canvas2D_CC.cpp:13-28):
+ * each active voxel's objective becomes the weighted mean, weights
+ * (radius - d) / radius, of the objectives of the active ESO voxels closer
+ * than radius, summed in ascending voxel index (the reference's order), so
+ * the result equals the reference's double for double.  Inactive voxels
+ * keep their objective.  A voxel with no such neighbour gets 0/0 (NaN), as
+ * in the reference. */
+int mpm_eso_filter(int32_t device, const uint8_t *active, const uint8_t *eso, const double *objective, int32_t nx,
+                   int32_t ny, double offset, double radius, double *out);
+
 /* --- learned return-map surrogate (no counterpart in the reference) ---------- */
 
 /* A 13 -> 64 -> 64 -> 64 -> 7 SiLU MLP with BF16 weights and FP32

@@ -1661,6 +1661,44 @@ int oracle_large_strain_point(const mpm_material *m, const double *L, double dt,
     return MPM_OK;
 }
 
+/* Canvas2D_CC::filterObjective_Smooth, Canvas2D_CC.cpp:302-341, as the
+ * reference runs it: the active voxels in ascending index (getVoxels_Active,
+ * :262-276), and for each one every active voxel in that order, the ESO ones
+ * closer than the radius weighted by (R - d) / R; centres per
+ * Canvas2D_CC.cpp:22, distance glm::length(a - b). */
+int oracle_eso_filter(const uint8_t *active, const uint8_t *eso, const double *obj, int32_t nx, int32_t ny,
+                      double off, double radius, double *out) {
+    if (nx < 0 || ny < 0) return MPM_ERR_ARG;
+    const int64_t nv = (int64_t)nx * ny;
+    if (nv && (!active || !eso || !obj || !out)) return MPM_ERR_ARG;
+    std::vector<int64_t> act;
+    for (int64_t v = 0; v < nv; v++) {
+        out[v] = obj[v];
+        if (active[v]) act.push_back(v);
+    }
+    std::vector<double> px(act.size()), py(act.size()), res(act.size());
+    for (size_t k = 0; k < act.size(); k++) {
+        px[k] = (double)(act[k] % nx) * off + 0.5 * off;
+        py[k] = (double)(act[k] / nx) * off + 0.5 * off;
+    }
+    for (size_t k = 0; k < act.size(); k++) {
+        double sum = 0.0, wsum = 0.0;
+        for (size_t j = 0; j < act.size(); j++) {
+            double dx = px[k] - px[j], dy = py[k] - py[j];
+            double d = std::sqrt(dx * dx + dy * dy);
+            if (!eso[act[j]]) continue;
+            if (d < radius) {
+                double w = (radius - d) / radius;
+                wsum += w;
+                sum += w * obj[act[j]];
+            }
+        }
+        res[k] = sum / wsum;
+    }
+    for (size_t k = 0; k < act.size(); k++) out[act[k]] = res[k];
+    return MPM_OK;
+}
+
 int oracle_timeline_velocity(const double *t, const double *v3p, int32_t n, double time, double out[3]) {
     Timeline tl;
     tl.t.assign(t, t + n);

@@ -65,6 +65,7 @@ def load():
         "oracle_matrix_log": [P, P],
         "oracle_matrix_exp": [P, P],
         "oracle_large_strain_point": [P, P, DBL, P, P],
+        "oracle_eso_filter": [P, P, P, I32, I32, DBL, DBL, P],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -303,3 +304,15 @@ def large_strain_point(material, L, dt, state):
         out[k] = v.reshape(3, 3).T.copy() if nc == 9 else (v.copy() if nc > 1 else float(v[0]))
         o += nc
     return out, bool(y[0])
+
+
+def eso_filter(active, eso, objective, nx, ny, offset, radius):
+    """Canvas2D_CC::filterObjective_Smooth as the reference runs it (all
+    pairs of active voxels)."""
+    a = np.ascontiguousarray(active, np.uint8)
+    e = np.ascontiguousarray(eso, np.uint8)
+    o = np.ascontiguousarray(objective, np.float64)
+    out = np.empty(int(nx) * int(ny))
+    _ok(load().oracle_eso_filter(_p(a), _p(e), _p(o), int(nx), int(ny), float(offset), float(radius), _p(out)),
+        "eso_filter")
+    return out
