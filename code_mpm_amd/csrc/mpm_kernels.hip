@@ -971,6 +971,9 @@ __device__ __forceinline__ void node_remap_dispatch(const Batch &bt, const NodeI
  * for the particle at hand instead (see the P2G loop). */
 #define PART_CELLS (BLOCK / 8)
 #define PART_STAGE 256
+/* P2G staging chunk: small enough that a block's LDS (staged fields plus
+ * the lanes' exchange area) stays under 32 KB, so five blocks share a CU */
+#define PART_STAGE_P2G 240
 
 /* Shape factors s[a][o] of a particle's base-cell corner nodes (axis a,
  * offset o), the values axis_shape gives each corner lane. */
@@ -985,16 +988,17 @@ __device__ __forceinline__ void particle_factors(const GridGeom &g, const double
     }
 }
 
-template <bool REMAP, bool MULTI>
+template <bool REMAP, bool MULTI, bool FEXT = false>
 __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NView Nd, BView B, CellParts C) {
     /* staged per particle: m v (3) and either the shape factors s (6, remap)
-     * or x (3, P2G), then for P2G m, V, sigma (6), f_ext (3); rows padded by
-     * one double so that the corner-dependent rows of one particle fall in
-     * different banks */
-    constexpr int PS = PART_STAGE, PR = REMAP ? PS + 1 : PS;
+     * or x (3, P2G), then for P2G m, V, sigma (6) and, with FEXT, f_ext (3);
+     * remap rows padded by one double so that the corner-dependent rows of
+     * one particle fall in different banks */
+    constexpr int PS = REMAP ? PART_STAGE : PART_STAGE_P2G, PR = REMAP ? PS + 1 : PS;
+    constexpr bool SF = !REMAP && FEXT;
     __shared__ double sS[REMAP ? 6 : 3][PR], spv[3][PR];
     __shared__ double sm[REMAP ? 1 : PR], sV[REMAP ? 1 : PR], ss[REMAP ? 1 : 6][REMAP ? 1 : PR];
-    __shared__ double sf[REMAP ? 1 : 3][REMAP ? 1 : PR];
+    __shared__ double sf[SF ? 3 : 1][SF ? PR : 1];
     __shared__ int8_t sdc[PS];
     /* P2G: what the 8 lanes of a cell share for the particle at hand: the
      * six axis shapes (factor, gradient-taken and sign bits) and the twelve
@@ -1015,7 +1019,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
     auto qindex = [&](int t) { return t == 0 ? 2 * oy + oz : (t == 1 ? 4 + 2 * ox + oz : 8 + 2 * ox + oy); };
     const int qk1 = qindex(qt1), qk2 = qindex(qt2);
     const bool skip_dc = REMAP ? bt.variant != VAR_DP : bt.variant == VAR_SP;
-    const bool fext = !REMAP && bt.fext;
+    constexpr bool fext = SF;
     for (uint32_t g0 = blockIdx.x * PART_CELLS; g0 < nocc; g0 += gridDim.x * PART_CELLS) {
         const uint32_t g1 = min(g0 + (uint32_t)PART_CELLS, nocc);
         const uint32_t r0 = B.start[B.occ[g0]], r1 = B.start[B.occ[g1 - 1] + 1];
@@ -2394,9 +2398,14 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
             break;
         }
         if (order == ORDER_CELL) {
-            if (P.n > 0)
-                LAUNCH_MULTI(bt, (k_cell_partials<false, M>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt,
-                             P, N, B, C);
+            if (P.n > 0) {
+                if (bt.fext)
+                    LAUNCH_MULTI(bt, (k_cell_partials<false, M, true>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s,
+                                 bt, P, N, B, C);
+                else
+                    LAUNCH_MULTI(bt, (k_cell_partials<false, M, false>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s,
+                                 bt, P, N, B, C);
+            }
             if (mid) (void)hipEventRecord(mid, s);
             LAUNCH_MULTI(bt, (k_node_cells<false, M>), dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
         } else {
