@@ -76,7 +76,10 @@ def read_traffic():
     return out
 
 
-def cpu_baseline(scene, threads, steps=20, warmup=3):
+def cpu_baseline(scene, threads, steps=300, warmup=3):
+    """The oracle (the reference's structure: AoS particles, per-node locks,
+    per-step adjacency rebuild) on the x < 64 h slice of the workload; 300
+    steps are about 10 s of work at 16 threads."""
     from code_mpm_amd import scenes
     from oracle.oracle import OracleEngine
     sample = scenes.crop_x(scene, 64)
