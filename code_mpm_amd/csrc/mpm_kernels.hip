@@ -1319,10 +1319,10 @@ struct PAccum {
 __device__ __forceinline__ void load_accum(const PView &P, size_t i, bool dp, PAccum &a) {
     const size_t n = P.n;
 #pragma unroll
-    for (int k = 0; k < 6; k++) a.eps[k] = P.eps[k * n + i];
-    a.Ws = P.Ws[i];
-    a.Wti = dp ? P.Wti[i] : 0.0;
-    a.Wmax = dp ? P.Wmax[i] : 0.0;
+    for (int k = 0; k < 6; k++) a.eps[k] = __builtin_nontemporal_load(&P.eps[k * n + i]);
+    a.Ws = __builtin_nontemporal_load(&P.Ws[i]);
+    a.Wti = dp ? __builtin_nontemporal_load(&P.Wti[i]) : 0.0;
+    a.Wmax = dp ? __builtin_nontemporal_load(&P.Wmax[i]) : 0.0;
 }
 
 /* The plastic fields, when they change: reads εp and Wp, writes εp + dεp,
@@ -1333,12 +1333,12 @@ __device__ __forceinline__ void update_plastic(const PView &P, size_t i, const C
     if (!change) return;
     double epsp[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) epsp[k] = P.epsp[k * n + i];
-    const double Wp = P.Wp[i];
+    for (int k = 0; k < 6; k++) epsp[k] = __builtin_nontemporal_load(&P.epsp[k * n + i]);
+    const double Wp = __builtin_nontemporal_load(&P.Wp[i]);
 #pragma unroll
-    for (int k = 0; k < 6; k++) P.epsp[k * n + i] = epsp[k] + cr.depsp[k];
-    P.Wp[i] = Wp + wp;
-    P.R[i] = R + cr.dR;
+    for (int k = 0; k < 6; k++) __builtin_nontemporal_store(epsp[k] + cr.depsp[k], &P.epsp[k * n + i]);
+    __builtin_nontemporal_store(Wp + wp, &P.Wp[i]);
+    __builtin_nontemporal_store(R + cr.dR, &P.R[i]);
 }
 
 /* State update after the return map (DP:242-278 minus F and V, which the
@@ -1358,12 +1358,12 @@ __device__ __forceinline__ uint32_t finish_update(const PView &P, size_t i, cons
     for (int k = 0; k < 6; k++) wp += (sig[k] - 0.5 * cr.dsig[k]) * cr.depsp[k] * V;
     update_plastic(P, i, cr, R, wp);
 #pragma unroll
-    for (int k = 0; k < 6; k++) P.eps[k * n + i] = a.eps[k] + de[k];
+    for (int k = 0; k < 6; k++) __builtin_nontemporal_store(a.eps[k] + de[k], &P.eps[k * n + i]);
 #pragma unroll
-    for (int k = 0; k < 6; k++) P.sig[k * n + i] = sig[k];
-    P.Ws[i] = Ws;
-    P.Wti[i] = a.Wti + Ws * dt;
-    if (Ws > a.Wmax) P.Wmax[i] = Ws;
+    for (int k = 0; k < 6; k++) __builtin_nontemporal_store(sig[k], &P.sig[k * n + i]);
+    __builtin_nontemporal_store(Ws, &P.Ws[i]);
+    __builtin_nontemporal_store(a.Wti + Ws * dt, &P.Wti[i]);
+    if (Ws > a.Wmax) __builtin_nontemporal_store(Ws, &P.Wmax[i]);
     uint32_t bad = 0;
     for (int k = 0; k < 6; k++) bad |= !isfinite(sig[k]);
     return bad;
@@ -1470,15 +1470,15 @@ __device__ __forceinline__ void g2p_kinematics_particle(const Batch &bt, const P
     const uint32_t cell_off = (uint32_t)I.cell_off;
     const int32_t tl_count = I.tl_count;
     const double vt[3] = {I.vt[0], I.vt[1], I.vt[2]};
-    const double x0[3] = {P.x[i], P.x[n + i], P.x[2 * n + i]};
+    const double x0[3] = {__builtin_nontemporal_load(&P.x[i]), __builtin_nontemporal_load(&P.x[n + i]), __builtin_nontemporal_load(&P.x[2 * n + i])};
     const int32_t dcr = P.dc_rank[i];
     const int mat = P.mat[i];
     double F[3][3];
 #pragma unroll
     for (int c = 0; c < 3; c++)
 #pragma unroll
-        for (int r = 0; r < 3; r++) F[c][r] = P.F[(3 * c + r) * n + i];
-    const double V0 = P.V0[i];
+        for (int r = 0; r < 3; r++) F[c][r] = __builtin_nontemporal_load(&P.F[(3 * c + r) * n + i]);
+    const double V0 = __builtin_nontemporal_load(&P.V0[i]);
     const double hx[3] = {g.h[0], g.h[1], g.h[2]};
     const int gn[3] = {g.nn[0], g.nn[1], g.nn[2]};
     double x[3] = {x0[0], x0[1], x0[2]};
@@ -1542,9 +1542,9 @@ __device__ __forceinline__ void g2p_kinematics_particle(const Batch &bt, const P
                     L[c][r] = in ? t : L[c][r];
                 }
         }
-        P.x[i] = x[0];
-        P.x[n + i] = x[1];
-        P.x[2 * n + i] = x[2];
+        __builtin_nontemporal_store(x[0], &P.x[i]);
+        __builtin_nontemporal_store(x[1], &P.x[n + i]);
+        __builtin_nontemporal_store(x[2], &P.x[2 * n + i]);
         if (LARGE) {
 #pragma unroll
             for (int c = 0; c < 3; c++)
@@ -1608,12 +1608,12 @@ __device__ __forceinline__ void g2p_kinematics_particle(const Batch &bt, const P
 #pragma unroll
             for (int c = 0; c < 3; c++)
 #pragma unroll
-                for (int r = 0; r < 3; r++) P.F[(3 * c + r) * n + i] = Fn[c][r];
+                for (int r = 0; r < 3; r++) __builtin_nontemporal_store(Fn[c][r], &P.F[(3 * c + r) * n + i]);
             double det = +Fn[0][0] * (Fn[1][1] * Fn[2][2] - Fn[2][1] * Fn[1][2]) -
                          Fn[1][0] * (Fn[0][1] * Fn[2][2] - Fn[2][1] * Fn[0][2]) +
                          Fn[2][0] * (Fn[0][1] * Fn[1][2] - Fn[1][1] * Fn[0][2]);
             ko.V = det * V0;
-            P.V[i] = ko.V;
+            __builtin_nontemporal_store(ko.V, &P.V[i]);
         }
         }
         flag = !(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2])) ? 1 : 0;
@@ -1796,8 +1796,8 @@ __device__ __forceinline__ void state_update_particle(const PView &P, const MatT
     const bool valid = mat >= 0 && mat < mt.n;
     double sig[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) sig[k] = P.sig[k * n + i];
-    const double R = P.R[i];
+    for (int k = 0; k < 6; k++) sig[k] = __builtin_nontemporal_load(&P.sig[k * n + i]);
+    const double R = __builtin_nontemporal_load(&P.R[i]);
     PAccum acc;
     load_accum(P, i, VAR == VAR_DP, acc);
     f_nonfinite = ko.flag;
