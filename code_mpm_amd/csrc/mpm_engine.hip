@@ -85,6 +85,8 @@ struct mpm_engine {
     uint32_t ntiles = 0;
     Inst *dinst = nullptr;
     uint32_t *dtile_off = nullptr, *dpinst = nullptr;
+    uint32_t *dwave = nullptr; /* batch wave table, see Batch::wave */
+    uint32_t nwaves = 0;
     double *dtl = nullptr;
     bool table_dirty = true;
     bool layout_set = false;
@@ -217,7 +219,8 @@ struct mpm_engine {
     void free_particles() {
         for (int f = 0; f < MPM_PF_COUNT; f++) dfree(pf[f]);
         dfree(pmat); dfree(pdc_rank); dfree(pdc_axes); dfree(pdc_mult); dfree(pde); dfree(pflag);
-        dfree(bkey); dfree(bsorted); dfree(dpinst);
+        dfree(bkey); dfree(bsorted); dfree(dpinst); dfree(dwave);
+        nwaves = 0;
         for (int k = 0; k < MPM_MON_COUNT; k++) mon[k].clear();
     }
     void free_queue() {
@@ -300,6 +303,8 @@ struct mpm_engine {
         b.inst = dinst;
         b.tile_off = dtile_off;
         b.pinst = inst.size() > 1 ? dpinst : nullptr;
+        b.wave = inst.size() > 1 ? dwave : nullptr;
+        b.nwaves = inst.size() > 1 ? nwaves : 0;
         b.tl = dtl;
         b.ninst = (int)inst.size();
         b.ntiles = ntiles;
@@ -810,6 +815,8 @@ int mpm_set_grid(mpm_engine *e, const double length[3], const int32_t cells[3]) 
     I.npart = e->n;
     e->inst.assign(1, I);
     dfree(e->dpinst);
+    dfree(e->dwave);
+    e->nwaves = 0;
     return e->build_layout();
 }
 
@@ -837,6 +844,20 @@ int mpm_set_instances(mpm_engine *e, int32_t count, const double *lengths, const
     if (total > 0) {
         if ((rc = e->hip(hipMalloc(&e->dpinst, 4 * (size_t)total), "alloc particle instances"))) return rc;
         if ((rc = e->hip(hipMemcpy(e->dpinst, pi.data(), 4 * (size_t)total, hipMemcpyHostToDevice), "upload instances")))
+            return rc;
+    }
+    /* the wave table: one (instance, first particle) entry per 64 consecutive
+     * particles of an instance */
+    std::vector<uint32_t> wt;
+    for (int k = 0; k < count; k++)
+        for (int64_t q = 0; q < v[k].npart; q += 64) {
+            wt.push_back((uint32_t)k);
+            wt.push_back((uint32_t)(v[k].part_off + q));
+        }
+    e->nwaves = (uint32_t)(wt.size() / 2);
+    if (!wt.empty()) {
+        if ((rc = e->hip(hipMalloc(&e->dwave, 4 * wt.size()), "alloc wave table"))) return rc;
+        if ((rc = e->hip(hipMemcpy(e->dwave, wt.data(), 4 * wt.size(), hipMemcpyHostToDevice), "upload wave table")))
             return rc;
     }
     return MPM_OK;

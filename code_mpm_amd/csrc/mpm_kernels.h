@@ -120,6 +120,9 @@ struct Batch {
     Inst *inst;               /* device table, ninst entries */
     const uint32_t *tile_off; /* ninst + 1 prefix of node tiles */
     const uint32_t *pinst;    /* instance of each particle; null when ninst == 1 */
+    const uint32_t *wave;     /* batch only: per wave of 64 consecutive particles of one
+                                 instance, (instance, first particle); null when ninst == 1 */
+    uint32_t nwaves;
     const double *tl;         /* timeline points (t, vx, vy, vz) of all instances */
     int ninst;
     uint32_t ntiles;

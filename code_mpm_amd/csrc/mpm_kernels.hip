@@ -70,6 +70,34 @@ __device__ __forceinline__ const Inst &inst_t(const Batch &bt, uint32_t t) {
     return MULTI ? bt.inst[inst_of_tile(bt, t)] : bt.inst[0];
 }
 
+/* This thread's particle and instance in the per-particle kernels that
+ * keep the instance row in scalar registers.  One instance: the particle
+ * from the (XCD-remapped) block and thread, instance 0.  A batch: each wave
+ * takes the 64 consecutive particles of one instance that its wave-table
+ * entry names, so the instance index is wave-uniform and its row is read
+ * through scalar loads, as for one instance.  Returns false for a wave
+ * beyond the table (such a wave has no particle and may leave at once:
+ * these kernels have no block-wide barrier). */
+template <bool MULTI>
+__device__ __forceinline__ bool particle_of_thread(const Batch &bt, uint32_t n, uint32_t &i, bool &live,
+                                                   uint32_t &inst) {
+    if (!MULTI) {
+        i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+        live = i < n;
+        inst = 0;
+        return true;
+    }
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane(
+        (int)(xcd_remap(blockIdx.x, gridDim.x) * (BLOCK / 64) + (threadIdx.x >> 6)));
+    if (w >= bt.nwaves) return false;
+    inst = bt.wave[2 * (size_t)w];
+    const uint32_t first = bt.wave[2 * (size_t)w + 1];
+    const Inst &I = bt.inst[inst];
+    i = first + (threadIdx.x & 63u);
+    live = (int64_t)i < I.part_off + I.npart;
+    return true;
+}
+
 /* TimeLine::getVelocity (TimeLine.h:33-46): the last interval [t_i, t_i+1)
  * containing the time wins; zero outside every interval. */
 __device__ __forceinline__ void timeline_velocity(const double *tl, int off, int count, double time, double out[3]) {
@@ -828,13 +856,14 @@ __device__ __forceinline__ void g2p_velocity_update(const Inst &I, const NView &
 
 template <bool MULTI>
 __global__ __launch_bounds__(BLOCK) void k_g2p_velocity(Batch bt, PView P, NView Nd) {
-    uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-    if (i >= P.n) return;
+    uint32_t i, k;
+    bool live;
+    if (!particle_of_thread<MULTI>(bt, (uint32_t)P.n, i, live, k) || !live) return;
     if (bt.variant != VAR_DP && P.dc_rank[i] >= 0) return; /* DPC:309-310, SP:279-280 */
     const size_t n = P.n;
     const double x[3] = {P.x[i], P.x[n + i], P.x[2 * n + i]};
     double v[3] = {P.v[i], P.v[n + i], P.v[2 * n + i]};
-    g2p_velocity_update(inst_p<MULTI>(bt, i), Nd, x, v);
+    g2p_velocity_update(bt.inst[k], Nd, x, v);
     P.v[i] = v[0];
     P.v[n + i] = v[1];
     P.v[2 * n + i] = v[2];
@@ -1837,14 +1866,15 @@ __device__ __forceinline__ void state_update_particle(const PView &P, const MatT
 template <int VAR, bool MULTI>
 __global__ __launch_bounds__(BLOCK) void k_g2p_fused(Batch bt, PView P, NView Nd, BView B, MatTable mt,
                                                     unsigned long long *health) {
-    uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-    const bool live = i < P.n;
+    uint32_t i, k;
+    bool live;
+    if (!particle_of_thread<MULTI>(bt, (uint32_t)P.n, i, live, k)) return;
     uint32_t f_outside = 0, f_null = 0, f_plastic = 0, f_nonfinite = 0;
     uint32_t next_key = MPM_NONE;
     if (live) {
         const double *g2 = Nd.g2;
         KinOut ko;
-        g2p_kinematics_particle<VAR, false>(bt, P, inst_p<MULTI>(bt, i), i,
+        g2p_kinematics_particle<VAR, false>(bt, P, bt.inst[k], i,
                                             [=](int, size_t node, double2 &r0, double2 &r1, double2 &r2, double2 &r3) {
                                                 const double2 *q = reinterpret_cast<const double2 *>(g2 + node * 8u);
                                                 r0 = q[0];
@@ -2346,6 +2376,12 @@ static unsigned partial_grid(int64_t n) {
     return (unsigned)std::min<int64_t>(b, 8192);
 }
 
+/* Blocks of the per-particle kernels that take a batch wave by wave: one
+ * wave per wave-table entry. */
+static unsigned wave_grid(const Batch &bt) {
+    return (unsigned)std::max<int64_t>(((int64_t)bt.nwaves + BLOCK / 64 - 1) / (BLOCK / 64), 1);
+}
+
 /* Launches KERNEL with M = whether the batch has more than one instance. */
 #define LAUNCH_MULTI(bt, KERNEL, ...)                                   \
     do {                                                                 \
@@ -2368,10 +2404,14 @@ static void launch_g2p_strain(int mode, dim3 grid, hipStream_t s, const Batch &b
         if (mid) (void)hipEventRecord(mid, s);
         return;
     }
-    /* one instance, exact map: the fused pass (with a batch's per-particle
-     * instance lookups it no longer fits the registers of two waves) */
-    if (mode == MODE_EXACT && bt.ninst == 1) {
-        hipLaunchKernelGGL((k_g2p_fused<VAR, false>), grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health);
+    /* exact map: the fused pass (a batch's waves each take one instance,
+     * whose row then sits in scalar registers as for a single instance) */
+    if (mode == MODE_EXACT) {
+        if (bt.ninst > 1)
+            hipLaunchKernelGGL((k_g2p_fused<VAR, true>), dim3(wave_grid(bt)), dim3(BLOCK), 0, s, bt, P, N, B, mt,
+                               health);
+        else
+            hipLaunchKernelGGL((k_g2p_fused<VAR, false>), grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health);
         if (mid) (void)hipEventRecord(mid, s);
         return;
     }
@@ -2415,7 +2455,8 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
         break;
     case PHASE_G2P_VELOCITY:
         if (P.n > 0)
-            LAUNCH_MULTI(bt, (k_g2p_velocity<M>), dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, bt, P, N);
+            LAUNCH_MULTI(bt, (k_g2p_velocity<M>), dim3(M ? wave_grid(bt) : blocks_for(P.n, BLOCK)), dim3(BLOCK), 0,
+                         s, bt, P, N);
         if (mid) (void)hipEventRecord(mid, s);
         break;
     case PHASE_REMAP_GRID:
