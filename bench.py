@@ -145,6 +145,9 @@ def surrogate_bench(scene, steps, train_steps=1500):
         eng.load_scene(scene)
         if mode == "surrogate":
             eng.set_surrogate(s, material=0, residual_tolerance=1e-4)
+        # a state to compare at equal step counts: both runs take the same two
+        # untimed warm-up steps (first-step allocations stay out of the window)
+        eng.step(scene.dt, 2)
         eng.synchronize()
         t0 = time.perf_counter()
         eng.step(scene.dt, steps)

@@ -760,12 +760,12 @@ int mpm_create(mpm_engine **out, int32_t device) {
     mpm_engine *e = new mpm_engine();
     e->device = device;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(&e->d_health, sizeof(unsigned long long) * HEALTH_SLOTS) != hipSuccess ||
+        hipMalloc(&e->d_health, sizeof(unsigned long long) * HEALTH_WORDS) != hipSuccess ||
         hipMalloc(&e->d_mon_out, sizeof(double) * 8) != hipSuccess ||
         hipMalloc(&e->qcount, sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&e->capcount, sizeof(unsigned long long)) != hipSuccess ||
         hipMemset(e->capcount, 0, sizeof(unsigned long long)) != hipSuccess ||
-        hipMemset(e->d_health, 0, sizeof(unsigned long long) * HEALTH_SLOTS) != hipSuccess) {
+        hipMemset(e->d_health, 0, sizeof(unsigned long long) * HEALTH_WORDS) != hipSuccess) {
         g_create_error = "stream or buffer creation failed";
         mpm_destroy(e);
         return MPM_ERR_HIP;
@@ -1299,10 +1299,14 @@ int mpm_get_instance_monitors(mpm_engine *e, int32_t instance, mpm_monitors *out
 int mpm_get_health(mpm_engine *e, mpm_health *out) {
     CHECK_E(e);
     if (!out) return e->fail(MPM_ERR_ARG, "null health output");
-    unsigned long long h[HEALTH_SLOTS];
+    std::vector<unsigned long long> w(HEALTH_WORDS);
     int rc = e->hip(hipStreamSynchronize(e->stream), "health sync");
-    if (!rc) rc = e->hip(hipMemcpy(h, e->d_health, sizeof(h), hipMemcpyDeviceToHost), "health copy");
+    if (!rc) rc = e->hip(hipMemcpy(w.data(), e->d_health, sizeof(unsigned long long) * HEALTH_WORDS,
+                                   hipMemcpyDeviceToHost), "health copy");
     if (rc) return rc;
+    unsigned long long h[HEALTH_SLOTS] = {};
+    for (int s = 0; s < HEALTH_STRIPES; s++)
+        for (int k = 0; k < HEALTH_SLOTS; k++) h[k] += w[(size_t)s * HEALTH_STRIDE + k];
     out->steps = e->steps;
     out->outside_grid = (int64_t)h[HEALTH_OUTSIDE];
     out->null_material = (int64_t)h[HEALTH_NULL];
@@ -1318,7 +1322,7 @@ int mpm_reset_health(mpm_engine *e) {
     e->steps = 0;
     int rc = e->hip(hipStreamSynchronize(e->stream), "health sync");
     if (rc) return rc;
-    return e->hip(hipMemset(e->d_health, 0, sizeof(unsigned long long) * HEALTH_SLOTS), "health reset");
+    return e->hip(hipMemset(e->d_health, 0, sizeof(unsigned long long) * HEALTH_WORDS), "health reset");
 }
 
 int mpm_set_surrogate(mpm_engine *e, mpm_surrogate *s, int32_t material, double residual_tolerance) {

@@ -1319,10 +1319,11 @@ __global__ __launch_bounds__(BLOCK) void k_node_cells(Batch bt, PView P, NView N
     }
 }
 
-__device__ __forceinline__ void block_count_add(unsigned long long *dst, uint32_t flag) {
-    /* wave ballot, then one atomic per wave */
+__device__ __forceinline__ void block_count_add(unsigned long long *health, int slot, uint32_t flag) {
+    /* wave ballot, then one atomic per wave, into this block's copy */
     uint64_t b = __ballot(flag != 0);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(dst, (unsigned long long)__popcll(b));
+    if ((threadIdx.x & 63) == 0 && b)
+        atomicAdd(&health[(blockIdx.x % HEALTH_STRIPES) * HEALTH_STRIDE + slot], (unsigned long long)__popcll(b));
 }
 
 /* The accumulated fields the state update reads and writes back.  They
@@ -1701,7 +1702,7 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_kinematics(Batch bt, PView P, NVi
         B.key[i] = next_key;
     }
     wave_key_add(B.count, next_key, next_key != MPM_NONE);
-    block_count_add(&health[HEALTH_OUTSIDE], f_outside);
+    block_count_add(health, HEALTH_OUTSIDE, f_outside);
 }
 
 /* The return map and state update of the same pass (DP:204-278, DPC:479-524,
@@ -1808,10 +1809,10 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_constitutive(Batch bt, PView P, M
 #pragma unroll
         for (int k = 0; k < 6; k++) P.eps[k * n + i] = acc.eps[k] + de[k];
     }
-    block_count_add(&health[HEALTH_NULL], f_null);
-    block_count_add(&health[HEALTH_PLASTIC], f_plastic);
-    block_count_add(&health[HEALTH_NONFINITE], f_nonfinite);
-    if (MODE == MODE_SURROGATE) block_count_add(&health[HEALTH_SURROGATE_CALLS], f_queued);
+    block_count_add(health, HEALTH_NULL, f_null);
+    block_count_add(health, HEALTH_PLASTIC, f_plastic);
+    block_count_add(health, HEALTH_NONFINITE, f_nonfinite);
+    if (MODE == MODE_SURROGATE) block_count_add(health, HEALTH_SURROGATE_CALLS, f_queued);
 }
 
 /* The exact state update of one particle that the pass visits, from the
@@ -1887,10 +1888,10 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_fused(Batch bt, PView P, NView Nd
         if (!ko.ctl) state_update_particle<VAR>(P, mt, i, ko, f_null, f_plastic, f_nonfinite);
     }
     wave_key_add(B.count, next_key, next_key != MPM_NONE);
-    block_count_add(&health[HEALTH_OUTSIDE], f_outside);
-    block_count_add(&health[HEALTH_NULL], f_null);
-    block_count_add(&health[HEALTH_PLASTIC], f_plastic);
-    block_count_add(&health[HEALTH_NONFINITE], f_nonfinite);
+    block_count_add(health, HEALTH_OUTSIDE, f_outside);
+    block_count_add(health, HEALTH_NULL, f_null);
+    block_count_add(health, HEALTH_PLASTIC, f_plastic);
+    block_count_add(health, HEALTH_NONFINITE, f_nonfinite);
 }
 
 /* ---- large-strain pass ---------------------------------------------------- */
@@ -2107,10 +2108,10 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_large(Batch bt, PView P, NView Nd
         large_strain_update(P, mt, i, ko, f_null, f_plastic, f_nonfinite);
     }
     wave_key_add(B.count, next_key, next_key != MPM_NONE);
-    block_count_add(&health[HEALTH_OUTSIDE], f_outside);
-    block_count_add(&health[HEALTH_NULL], f_null);
-    block_count_add(&health[HEALTH_PLASTIC], f_plastic);
-    block_count_add(&health[HEALTH_NONFINITE], f_nonfinite);
+    block_count_add(health, HEALTH_OUTSIDE, f_outside);
+    block_count_add(health, HEALTH_NULL, f_null);
+    block_count_add(health, HEALTH_PLASTIC, f_plastic);
+    block_count_add(health, HEALTH_NONFINITE, f_nonfinite);
 }
 
 /* Applies the network's output to the queued points: the stress increment
@@ -2194,8 +2195,8 @@ __global__ __launch_bounds__(BLOCK) void k_surrogate_post(Batch bt, PView P, Mat
             else
                 f_nonfinite = finish_update_new_stress(P, i, sig, cr, R, P.V[i], de, acc);
         }
-        block_count_add(&health[HEALTH_SURROGATE_FALLBACKS], f_fallback);
-        block_count_add(&health[HEALTH_NONFINITE], f_nonfinite);
+        block_count_add(health, HEALTH_SURROGATE_FALLBACKS, f_fallback);
+        block_count_add(health, HEALTH_NONFINITE, f_nonfinite);
     }
 }
 
