@@ -2086,14 +2086,15 @@ __device__ __forceinline__ void large_strain_update(const PView &P, const MatTab
 template <bool MULTI>
 __global__ __launch_bounds__(BLOCK) void k_g2p_large(Batch bt, PView P, NView Nd, BView B, MatTable mt,
                                                     unsigned long long *health) {
-    uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-    const bool live = i < P.n;
+    uint32_t i, k;
+    bool live;
+    if (!particle_of_thread<MULTI>(bt, (uint32_t)P.n, i, live, k)) return;
     uint32_t f_outside = 0, f_null = 0, f_plastic = 0, f_nonfinite = 0;
     uint32_t next_key = MPM_NONE;
     if (live) {
         const double *g2 = Nd.g2;
         KinOut ko;
-        g2p_kinematics_particle<VAR_DP, false, true>(bt, P, inst_p<MULTI>(bt, i), i,
+        g2p_kinematics_particle<VAR_DP, false, true>(bt, P, bt.inst[k], i,
                                                      [=](int, size_t node, double2 &r0, double2 &r1, double2 &r2,
                                                          double2 &r3) {
                                                          const double2 *q =
@@ -2401,7 +2402,8 @@ static void launch_g2p_strain(int mode, dim3 grid, hipStream_t s, const Batch &b
                               hipEvent_t mid) {
     if (VAR == VAR_DP && bt.strain == STRAIN_LARGE) {
         /* the engine allows the large-strain pass with the exact map only */
-        LAUNCH_MULTI(bt, (k_g2p_large<M>), grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health);
+        LAUNCH_MULTI(bt, (k_g2p_large<M>), dim3(M ? wave_grid(bt) : grid.x), dim3(BLOCK), 0, s, bt, P, N, B, mt,
+                     health);
         if (mid) (void)hipEventRecord(mid, s);
         return;
     }
