@@ -41,7 +41,7 @@ PER_PARTICLE = {"p2g_partials": 14 * 8, "p2g_nodes": 0, "g2p_velocity": 9 * 8, "
 PER_PLASTIC = {"g2p_constitutive": 15 * 8, "g2p_kinematics": 15 * 8}
 PER_NODE = {"p2g_partials": 0, "p2g_nodes": 20 * 8, "g2p_velocity": 4 * 8, "remap_partials": 0,
             "remap_nodes": 10 * 8, "g2p_kinematics": 7 * 8, "g2p_constitutive": 0, "bin": 0}
-# With the exact return map and one instance, G2P pass 2 is one kernel
+# With the exact return map, G2P pass 2 is one kernel
 # (k_g2p_fused), booked in the kinematics slot: its algorithmic bytes are
 # the kinematics' and the state update's together (the strain increment no
 # longer crosses memory), and the constitutive slot launches nothing.

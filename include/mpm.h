@@ -324,7 +324,7 @@ int mpm_step_timed(mpm_engine *e, double dt, int32_t nsteps, double phase_ms[MPM
  * summed over nsteps, in these slots.  A phase that runs one kernel in the
  * particle order (P2G, remap) books it in its first slot; with a surrogate
  * set, its network and post kernels count in the constitutive slot.  With
- * the exact return map and one instance, G2P pass 2 is one fused kernel
+ * the exact return map (one instance or a batch), G2P pass 2 is one fused kernel
  * (kinematics and state update of each particle in one thread), booked in
  * the kinematics slot; the constitutive slot then times nothing. */
 enum {
