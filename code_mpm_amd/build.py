@@ -51,5 +51,40 @@ def build(verbose=False):
     return LIB
 
 
+HOST = os.path.join(HERE, "host")
+HOST_LIB = os.path.join(HERE, "libmpm_physics.so")
+BIN = os.path.join(HERE, "bin")
+GXX_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+
+
+def build_host(verbose=False):
+    """The C++ host classes (libmpm_physics.so, over libmpm_amd.so) and the
+    reference-style bar driver (bin/mpm_bar_demo), with g++: the host side
+    holds no HIP types, only the C ABI."""
+    inc = ["-I", os.path.join(os.path.dirname(HERE), "include")]
+    rocm_lib = "/opt/rocm/lib"
+    cmds = [
+        ["g++"] + GXX_FLAGS + ["-fPIC", "-shared", os.path.join(HOST, "mpm_physics.cpp")] + inc +
+        ["-L", HERE, "-lmpm_amd", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link," + rocm_lib, "-o", HOST_LIB + ".tmp"],
+    ]
+    os.makedirs(BIN, exist_ok=True)
+    demo = os.path.join(BIN, "mpm_bar_demo")
+    cmds.append(["g++"] + GXX_FLAGS + [os.path.join(HOST, "bar_demo.cpp")] + inc +
+                ["-L", HERE, "-lmpm_physics", "-lmpm_amd", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link," + rocm_lib,
+                 "-o", demo + ".tmp"])
+    for k, cmd in enumerate(cmds):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError("g++ failed building the C++ host side")
+        if k == 0:
+            os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    os.replace(demo + ".tmp", demo)
+    return HOST_LIB, demo
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_host(verbose=True))
