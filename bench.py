@@ -479,6 +479,12 @@ def main():
         cpu = {"value": round(v_cpu, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
                "sample": f"x < 64 h slice of the same honeycomb ({n_cpu} particles, 64x512x32 cells), "
                          f"{s_cpu} steps after 3 warm-up, {el_cpu:.1f} s, OpenMP lock scatter, g++ -O2"}
+        if threads != 8:
+            # the reference's own team size (PhysicsEngine.h:27), half the steps
+            v8, n8, s8, el8 = cpu_baseline(scene, 8, steps=150)
+            cpu["at_8_threads"] = {"value": round(v8, 1), "unit": "particle-steps/s", "cores": 8,
+                                   "sample": f"same slice, {s8} steps, {el8:.1f} s"}
+            print(f"# cpu baseline at 8 threads: {v8:.3e} particle-steps/s", file=sys.stderr, flush=True)
         if beso_out is not None:
             # the reference's all-pairs smoothing loop (Canvas2D_CC.cpp:302-341), one thread, same canvas
             from oracle import oracle as orc
