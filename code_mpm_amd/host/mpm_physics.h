@@ -185,6 +185,8 @@ class PhysicsEngine {
   public:
     explicit PhysicsEngine(int device = 0);
     virtual ~PhysicsEngine(); /* deletes every point, node and material, PhysicsEngine.cpp:4-38 */
+    PhysicsEngine(const PhysicsEngine &) = delete; /* owns its objects and its device engine */
+    PhysicsEngine &operator=(const PhysicsEngine &) = delete;
 
     /* The cantilever bar of this build's C0 workload: 64 x 8 x 8 cells of
      * h = 1.5625e-3 m, 2 x 2 x 2 points per cell at (i + 0.25) h and
