@@ -2,14 +2,16 @@
 initializeWorld_Classic_Bar, then runSimulation_Classic_DoublePass_MPLocks
 over 250 steps' worth of time and getMaterialPoints) against the oracle
 running the same world over the same request, bit for bit, in both
-summation orders; the console reports fire at the reference's interval."""
+summation orders; the console reports fire at the reference's interval.
+The per-point ConstitutiveRelation calls of the C++ layer against the
+oracle's return maps."""
 import os
 import subprocess
 
 import numpy as np
 import pytest
 
-from tests.test_host_cpp import DEMO, read_dump
+from tests.test_host_cpp import CHECK, DEMO, read_dump
 
 pytestmark = pytest.mark.gpu
 
@@ -38,3 +40,32 @@ def test_cpp_driver_matches_oracle(order, tmp_path):
     assert reports.shape[0] == 2
     assert 100 * sc.dt < reports[0, 0] < 102 * sc.dt and 200 * sc.dt < reports[1, 0] < 203 * sc.dt
     assert np.all(reports[:, 5] > 0.0)      # strain energy of the bending bar
+
+
+@pytest.mark.parametrize("mtype", [0, 3, 4])
+def test_cpp_constitutive_per_point(mtype, tmp_path):
+    """ConstitutiveRelation::calculateIncrement_Elastic / _Plastic /
+    _VonMisesHardening (ConstitutiveRelation.h:16-62), one object and one
+    device call per point as the reference's loops make them, bit for bit
+    against the oracle's return maps; about half the points yield."""
+    from oracle import oracle
+    rng = np.random.default_rng(10 + mtype)
+    n = 64
+    E, nu, sy, c0, c1 = 70e9, 0.33, 250e6, 15.0, 300e6
+    sig = rng.normal(0, 80e6, (n, 6))
+    de = rng.normal(0, 1.2e-3, (n, 6))
+    R = np.abs(rng.normal(0, 50e6, n))
+    rows = np.concatenate([R[:, None], sig, de], axis=1)
+    inp = np.concatenate([[mtype, E, nu, sy, c0, c1, n], rows.ravel()])
+    src, dst = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    inp.tofile(src)
+    r = subprocess.run([CHECK, "constitutive", src, dst], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = np.fromfile(dst, dtype=np.float64).reshape(n, 13)
+    mat = dict(type=mtype, elastic_modulus=E, poisson_ratio=nu, yield_stress=sy, hardening_c0=c0, hardening_c1=c1)
+    ds, dp, dR = oracle.constitutive_batch(mat, R, sig, de)
+    assert np.array_equal(out[:, 0:6], ds)
+    assert np.array_equal(out[:, 6:12], dp)
+    assert np.array_equal(out[:, 12], dR)
+    if mtype != 0:
+        assert 8 < np.count_nonzero(np.abs(dp).sum(1)) < n

@@ -13,6 +13,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "code_mpm_amd", "libmpm_physics.so")
 DEMO = os.path.join(ROOT, "code_mpm_amd", "bin", "mpm_bar_demo")
+CHECK = os.path.join(ROOT, "code_mpm_amd", "bin", "mpm_host_check")
 
 
 def read_dump(path):
@@ -56,3 +57,64 @@ def test_bar_world_matches_python_scene(tmp_path):
 def test_driver_usage_without_arguments():
     r = subprocess.run([DEMO], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+def mediator_positions(L, cells, rng):
+    """Random interior points, points on node planes (both layers kept),
+    points just outside either face (one layer), far outside and NaN (none)."""
+    h = L / cells
+    pts = [rng.uniform(0.0, 1.0, (200, 3)) * L]
+    on = rng.uniform(0.0, 1.0, (60, 3)) * L
+    axis = rng.integers(0, 3, 60)
+    on[np.arange(60), axis] = rng.integers(0, cells[axis] + 1) * h[axis]
+    pts.append(on)
+    for ax in range(3):
+        lo = rng.uniform(0.0, 1.0, (10, 3)) * L
+        lo[:, ax] = -rng.uniform(0.0, 0.9, 10) * h[ax]
+        hi = rng.uniform(0.0, 1.0, (10, 3)) * L
+        hi[:, ax] = L[ax] + rng.uniform(0.0, 0.9, 10) * h[ax]
+        pts += [lo, hi]
+    far = rng.uniform(0.0, 1.0, (6, 3)) * L
+    far[:3, 0] = 3.0 * L[0]
+    far[3:, 1] = -2.0 * L[1]
+    nan = rng.uniform(0.0, 1.0, (2, 3)) * L
+    nan[:, 2] = np.nan
+    return np.concatenate(pts + [far, nan])
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_mediator_matches_oracle(tmp_path, seed):
+    """GridPoint_Mediator of the C++ host layer (findAdjacentGridPoints and
+    calculateBases_Classic, GridPoint_Mediator.cpp:4-144) on the nodes the
+    C++ GridPoint_Factory made: the same node list, in the same order, and
+    the same shape values and gradients, double for double, as the oracle's
+    mediator."""
+    from oracle import oracle
+    rng = np.random.default_rng(seed)
+    L = rng.uniform(0.01, 2.0, 3)
+    cells = rng.integers(1, 40, 3).astype(np.int32)
+    x = mediator_positions(L, cells, rng)
+    inp = np.concatenate([L, cells.astype(np.float64), [x.shape[0]], x.ravel()])
+    src, dst = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    inp.tofile(src)
+    subprocess.run([CHECK, "mediator", src, dst], check=True, capture_output=True, timeout=60)
+    out = np.fromfile(dst, dtype=np.float64)
+    h = L / cells
+    pos, counts = 0, []
+    for xp in x:
+        cnt = int(out[pos])
+        idx = out[pos + 1: pos + 9]
+        pos += 9
+        ref = oracle.find_adjacent(L, cells, xp)
+        assert cnt == len(ref) and np.array_equal(idx[:cnt], ref) and np.all(idx[cnt:] == -1), xp
+        nn = cells + 1
+        for k, node in enumerate(ref):
+            ix, iy, iz = node % nn[0], (node // nn[0]) % nn[1], node // (nn[0] * nn[1])
+            xn = np.array([ix * L[0] / cells[0], iy * L[1] / cells[1], iz * L[2] / cells[2]])
+            N, g = oracle.bases(h, xp, xn)
+            got = out[pos + 4 * k: pos + 4 * k + 4]
+            assert got[0] == N and np.array_equal(got[1:], g), (xp, node)
+        pos += 4 * cnt
+        counts.append(cnt)
+    assert pos == out.size
+    assert set(counts) >= {0, 4, 8}
