@@ -129,6 +129,7 @@ SIGNATURES = {
     "mpm_get_instance_monitors": (I32, [P, I32, ctypes.POINTER(Monitors)]),
     "mpm_run": (I32, [P, DBL, PI32]),
     "mpm_step_timed": (I32, [P, DBL, I32, PDBL]),
+    "mpm_step_phase": (I32, [P, I32, DBL]),
     "mpm_step_timed_kernels": (I32, [P, DBL, I32, PDBL]),
     "mpm_synchronize": (I32, [P]),
     "mpm_get_time": (I32, [P, PDBL, PI64]),

@@ -513,6 +513,56 @@ struct mpm_engine {
         return MPM_OK;
     }
 
+    /* A step taken phase by phase (mpm_step_phase): phase_step is the index,
+     * in phase_order, of the next phase, or -1 when no such step is open. */
+    int phase_step = -1;
+    bool phase_dt_given = false;
+    double phase_dt = 0.0;
+
+    int step_phase(int phase, double dt) {
+        int rc;
+        if (phase_step < 0) {
+            if (phase != PHASE_BIN) return fail(MPM_ERR_STATE, "a step taken phase by phase opens with the binning phase");
+            if ((rc = begin_steps())) return rc;
+            phase_dt_given = dt > 0.0;
+            phase_dt = dt;
+            if ((rc = hip(launch_set_dt(batch(), 0, phase_dt_given ? dt : inst[0].dt_max, stream), "set dt")) ||
+                (rc = hip(launch_prepare(batch(), stream), "step prepare")) ||
+                (rc = hip(launch_bin(pview(), bview(), bblock, stream), "cell binning")))
+                return rc;
+            phase_step = 0;
+            return MPM_OK;
+        }
+        int order_of[4];
+        phase_order(order_of);
+        if (phase != order_of[phase_step])
+            return fail(MPM_ERR_STATE, "phase %d out of order (the open step's next phase is %d)", phase,
+                        order_of[phase_step]);
+        /* a failed launch abandons the step */
+        const int at = phase_step;
+        phase_step = -1;
+        const Batch b = batch();
+        const SurrogateArgs sa = sargs();
+        if (phase == PHASE_G2P_STRAIN && sur && (rc = hip(launch_zero_u32(qcount, 1, stream), "queue reset")))
+            return rc;
+        if ((rc = hip(launch_phase(phase, b, pview(), nview(), bview(), parts, order, mt, d_health, sa, stream),
+                      "step kernels")))
+            return rc;
+        if (at + 1 < 4) {
+            phase_step = at + 1;
+            return MPM_OK;
+        }
+        if (sur) {
+            rc = hip(mlp_launch_forward(sur->params, sur->wbf16, qx, 1, n, qy, 1, n, n, stream, qcount), "surrogate forward");
+            if (!rc) rc = hip(launch_surrogate_post(b, pview(), mt, d_health, sa, stream), "surrogate post");
+            if (rc) return rc;
+        }
+        if ((rc = hip(launch_advance(b, stream), "step advance"))) return rc;
+        keys_valid = true;
+        advance_mirror(phase_dt_given ? &phase_dt : nullptr);
+        return MPM_OK;
+    }
+
     /* Mean milliseconds of `reps` launches of one phase (or of the phase
      * pair phase, phase2 when phase2 >= 0). */
     int time_phase(int phase, int reps, const SurrogateArgs &sa, hipEvent_t e0, hipEvent_t e1, double &ms,
@@ -796,6 +846,7 @@ int mpm_destroy(mpm_engine *e) {
 const char *mpm_last_error(const mpm_engine *e) { return e ? e->err.c_str() : g_create_error.c_str(); }
 
 int mpm_set_grid(mpm_engine *e, const double length[3], const int32_t cells[3]) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (!length || !cells) return e->fail(MPM_ERR_ARG, "null grid argument");
     HostInst I;
@@ -822,6 +873,7 @@ int mpm_set_grid(mpm_engine *e, const double length[3], const int32_t cells[3]) 
 
 int mpm_set_instances(mpm_engine *e, int32_t count, const double *lengths, const int32_t *cells,
                       const int64_t *particle_counts) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (count < 1 || !lengths || !cells || !particle_counts) return e->fail(MPM_ERR_ARG, "bad instance list");
     std::vector<HostInst> v(count);
@@ -874,6 +926,7 @@ int mpm_get_instance_layout(const mpm_engine *e, int32_t instance, int64_t *part
 }
 
 int mpm_set_node_boundary(mpm_engine *e, const uint8_t *fixed_mask, const double *friction, int64_t n_nodes) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (!e->layout_set) return e->fail(MPM_ERR_STATE, "set_node_boundary before set_grid");
     if (n_nodes != e->nnodes) return e->fail(MPM_ERR_ARG, "node count %lld != %lld", (long long)n_nodes, (long long)e->nnodes);
@@ -897,6 +950,7 @@ int mpm_set_node_boundary(mpm_engine *e, const uint8_t *fixed_mask, const double
 }
 
 int mpm_set_materials(mpm_engine *e, const mpm_material *materials, int32_t n) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (n < 0 || n > MPM_MAX_MATERIALS) return e->fail(MPM_ERR_RANGE, "material count %d outside [0, %d]", n, MPM_MAX_MATERIALS);
     if (n > 0 && !materials) return e->fail(MPM_ERR_ARG, "null material table");
@@ -917,6 +971,7 @@ int mpm_set_materials(mpm_engine *e, const mpm_material *materials, int32_t n) {
 }
 
 int mpm_set_particle_count(mpm_engine *e, int64_t n) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (n < 0 || n >= (int64_t)0xFFFFFFFFll) return e->fail(MPM_ERR_RANGE, "particle count %lld unsupported", (long long)n);
     if (e->inst.size() > 1) {
@@ -938,6 +993,7 @@ int mpm_get_particle_count(const mpm_engine *e, int64_t *n) {
 }
 
 int mpm_set_particle_field(mpm_engine *e, int32_t field, const double *src, int64_t n) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (field < 0 || field >= MPM_PF_COUNT) return e->fail(MPM_ERR_ARG, "unknown particle field %d", field);
     if (n != e->n) return e->fail(MPM_ERR_ARG, "field length %lld != particle count %lld", (long long)n, (long long)e->n);
@@ -971,6 +1027,7 @@ int mpm_get_particle_field(mpm_engine *e, int32_t field, double *dst, int64_t n)
 }
 
 int mpm_set_particle_material(mpm_engine *e, const int32_t *material, int64_t n) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (n != e->n) return e->fail(MPM_ERR_ARG, "material length mismatch");
     if (n == 0) return MPM_OK;
@@ -981,6 +1038,7 @@ int mpm_set_particle_material(mpm_engine *e, const int32_t *material, int64_t n)
 
 int mpm_set_displacement_control(mpm_engine *e, const int64_t *index, const uint8_t *axes, const float *multiplier,
                                  int64_t count) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (count < 0 || (count > 0 && (!index || !axes || !multiplier))) return e->fail(MPM_ERR_ARG, "bad control list");
     if (count >= (int64_t)0x7FFFFFFF) return e->fail(MPM_ERR_RANGE, "control list too long");
@@ -1005,6 +1063,7 @@ int mpm_set_displacement_control(mpm_engine *e, const int64_t *index, const uint
 }
 
 int mpm_set_monitor_list(mpm_engine *e, int32_t kind, const int64_t *index, int64_t count) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (kind < 0 || kind >= MPM_MON_COUNT) return e->fail(MPM_ERR_ARG, "unknown monitor kind %d", kind);
     if (count < 0 || (count > 0 && !index)) return e->fail(MPM_ERR_ARG, "bad monitor list");
@@ -1015,6 +1074,7 @@ int mpm_set_monitor_list(mpm_engine *e, int32_t kind, const int64_t *index, int6
 }
 
 int mpm_set_instance_timeline(mpm_engine *e, int32_t instance, const double *time, const double *v3, int32_t n) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     if (!e) return MPM_ERR_ARG;
     if (instance < 0 || instance >= (int32_t)e->inst.size()) return e->fail(MPM_ERR_ARG, "bad instance");
     if (n < 0 || (n > 0 && (!time || !v3))) return e->fail(MPM_ERR_ARG, "bad timeline");
@@ -1031,6 +1091,7 @@ int mpm_set_instance_timeline(mpm_engine *e, int32_t instance, const double *tim
 }
 
 int mpm_set_timeline(mpm_engine *e, const double *time, const double *v3, int32_t n) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     if (!e) return MPM_ERR_ARG;
     if (e->inst.empty()) return e->fail(MPM_ERR_STATE, "set_timeline before set_grid");
     for (int32_t k = 0; k < (int32_t)e->inst.size(); k++) {
@@ -1041,6 +1102,7 @@ int mpm_set_timeline(mpm_engine *e, const double *time, const double *v3, int32_
 }
 
 int mpm_set_instance_time(mpm_engine *e, int32_t instance, double time, double dt_max, double time_end) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     if (!e) return MPM_ERR_ARG;
     if (instance < 0 || instance >= (int32_t)e->inst.size()) return e->fail(MPM_ERR_ARG, "bad instance");
     e->inst[instance].time = time;
@@ -1051,6 +1113,7 @@ int mpm_set_instance_time(mpm_engine *e, int32_t instance, double time, double d
 }
 
 int mpm_set_time(mpm_engine *e, double time, double dt_max, double time_end) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     if (!e) return MPM_ERR_ARG;
     if (e->inst.empty()) return e->fail(MPM_ERR_STATE, "set_time before set_grid");
     for (auto &I : e->inst) {
@@ -1063,6 +1126,7 @@ int mpm_set_time(mpm_engine *e, double time, double dt_max, double time_end) {
 }
 
 int mpm_set_damping(mpm_engine *e, double damping, double mass_minimum) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     if (!e) return MPM_ERR_ARG;
     if (e->inst.empty()) return e->fail(MPM_ERR_STATE, "set_damping before set_grid");
     for (auto &I : e->inst) {
@@ -1074,6 +1138,7 @@ int mpm_set_damping(mpm_engine *e, double damping, double mass_minimum) {
 }
 
 int mpm_step(mpm_engine *e, double dt, int32_t nsteps) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (nsteps < 0) return e->fail(MPM_ERR_ARG, "negative step count");
     if (e->inst.size() > 1 && dt > 0.0)
@@ -1086,6 +1151,7 @@ int mpm_step(mpm_engine *e, double dt, int32_t nsteps) {
 }
 
 int mpm_set_variant(mpm_engine *e, int32_t variant) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (variant < MPM_VARIANT_DOUBLEPASS || variant > MPM_VARIANT_SINGLEPASS)
         return e->fail(MPM_ERR_ARG, "unknown variant %d", variant);
@@ -1098,6 +1164,7 @@ int mpm_set_variant(mpm_engine *e, int32_t variant) {
 }
 
 int mpm_set_strain_formulation(mpm_engine *e, int32_t formulation) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (formulation != MPM_STRAIN_SMALL && formulation != MPM_STRAIN_LARGE)
         return e->fail(MPM_ERR_ARG, "unknown strain formulation %d", formulation);
@@ -1123,6 +1190,7 @@ int mpm_get_variant(const mpm_engine *e, int32_t *variant) {
 }
 
 int mpm_set_summation_order(mpm_engine *e, int32_t order) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (order != MPM_ORDER_PARTICLE && order != MPM_ORDER_CELL) return e->fail(MPM_ERR_ARG, "unknown order %d", order);
     (void)hipStreamSynchronize(e->stream);
@@ -1132,6 +1200,7 @@ int mpm_set_summation_order(mpm_engine *e, int32_t order) {
 }
 
 int mpm_set_tile_shape(mpm_engine *e, const int32_t shape[3]) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (!shape || shape[0] < 1 || shape[1] < 1 || shape[2] < 1 || shape[0] * shape[1] * shape[2] != NODE_BLOCK)
         return e->fail(MPM_ERR_ARG, "tile shape must multiply to %d", NODE_BLOCK);
@@ -1147,6 +1216,7 @@ int mpm_get_tile_shape(const mpm_engine *e, int32_t shape[3]) {
 }
 
 int mpm_autotune(mpm_engine *e, int32_t reps, int32_t shape[3], double *node_ms) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (reps < 1 || reps > 100) return e->fail(MPM_ERR_ARG, "reps outside [1, 100]");
     int rc = e->autotune_step(reps, node_ms);
@@ -1156,12 +1226,14 @@ int mpm_autotune(mpm_engine *e, int32_t reps, int32_t shape[3], double *node_ms)
 }
 
 int mpm_step_graph(mpm_engine *e, int32_t nsteps) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (nsteps < 1 || nsteps > 100000) return e->fail(MPM_ERR_ARG, "graph step count out of range");
     return e->step_graph(nsteps);
 }
 
 int mpm_step_timed(mpm_engine *e, double dt, int32_t nsteps, double phase_ms[MPM_PHASE_COUNT]) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (nsteps < 0 || !phase_ms) return e->fail(MPM_ERR_ARG, "bad timed-step arguments");
     if (e->inst.size() > 1 && dt > 0.0) return e->fail(MPM_ERR_ARG, "multi-instance engines take dt <= 0");
@@ -1187,6 +1259,7 @@ int mpm_step_timed(mpm_engine *e, double dt, int32_t nsteps, double phase_ms[MPM
 }
 
 int mpm_step_timed_kernels(mpm_engine *e, double dt, int32_t nsteps, double kernel_ms[MPM_KSLOT_COUNT]) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (nsteps < 0 || !kernel_ms) return e->fail(MPM_ERR_ARG, "bad timed-step arguments");
     if (e->inst.size() > 1 && dt > 0.0) return e->fail(MPM_ERR_ARG, "multi-instance engines take dt <= 0");
@@ -1226,7 +1299,16 @@ int mpm_step_timed_kernels(mpm_engine *e, double dt, int32_t nsteps, double kern
     return rc;
 }
 
+int mpm_step_phase(mpm_engine *e, int32_t phase, double dt) {
+    CHECK_E(e);
+    if (phase < 0 || phase >= MPM_PHASE_COUNT) return e->fail(MPM_ERR_ARG, "unknown phase %d", phase);
+    if (e->phase_step < 0 && e->inst.size() > 1 && dt > 0.0)
+        return e->fail(MPM_ERR_ARG, "a multi-instance engine steps each instance by its own dt_max (pass dt <= 0)");
+    return e->step_phase(phase, dt);
+}
+
 int mpm_run(mpm_engine *e, double dt_total, int32_t *finished) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (e->inst.size() != 1) return e->fail(MPM_ERR_STATE, "mpm_run drives a single-instance engine");
     double acc = 0.0;
@@ -1326,6 +1408,7 @@ int mpm_reset_health(mpm_engine *e) {
 }
 
 int mpm_set_surrogate(mpm_engine *e, mpm_surrogate *s, int32_t material, double residual_tolerance) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (s && s->device != e->device) return e->fail(MPM_ERR_ARG, "surrogate lives on another device");
     if (s && (material < 0 || material >= e->mt.n)) return e->fail(MPM_ERR_ARG, "surrogate material out of range");
@@ -1341,6 +1424,7 @@ int mpm_set_surrogate(mpm_engine *e, mpm_surrogate *s, int32_t material, double 
 }
 
 int mpm_set_sample_capture(mpm_engine *e, int64_t capacity) {
+    if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
     if (capacity < 0) return e->fail(MPM_ERR_ARG, "negative capture capacity");
     (void)hipStreamSynchronize(e->stream);

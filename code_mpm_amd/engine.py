@@ -249,6 +249,17 @@ class Engine:
     def step(self, dt, nsteps=1):
         self._check(self._lib.mpm_step(self._h, float(dt), int(nsteps)), "step")
 
+    def step_phase(self, phase, dt=0.0):
+        """One phase of a step taken phase by phase ("bin" opens the step and
+        takes its dt; the others follow in the loop's order, and "g2p_strain"
+        closes it)."""
+        self._check(self._lib.mpm_step_phase(self._h, capi.PHASES.index(phase), float(dt)), f"step_phase {phase}")
+
+    def phase_order(self):
+        """The phases of one step in the current loop's order."""
+        mid = ("remap_grid", "g2p_velocity") if self.variant == "singlepass" else ("g2p_velocity", "remap_grid")
+        return ("bin", "p2g_grid") + mid + ("g2p_strain",)
+
     def step_timed(self, dt, nsteps=1):
         """Steps with HIP-event timing per phase; returns {phase: total ms}."""
         out = np.zeros(len(capi.PHASES))

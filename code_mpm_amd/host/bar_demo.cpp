@@ -5,8 +5,10 @@
  * steps' worth of time, and writes the particles' state (and the console
  * reports) to a binary file.
  *
- *   mpm_bar_demo STEPS OUTFILE [ORDER]     ORDER: 0 particle, 1 cell (default 0);
- *                                         STEPS 0: the world as built, no device use
+ *   mpm_bar_demo STEPS OUTFILE [ORDER [PHASES]]
+ *       ORDER: 0 particle, 1 cell (default 0); STEPS 0: the world as built,
+ *       no device use; PHASES 1: each step driven phase by phase
+ *       (runSimulation_Phase) instead of by the loop (no console reports)
  *
  * File: int64 n, int64 reports; then n-major doubles per field, component
  * by component (x 3, v 3, sigma 6, F 9, strain energy 1, volume 1); then 9
@@ -22,15 +24,23 @@ using namespace mpmhost;
 
 int main(int argc, char **argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s STEPS OUTFILE [ORDER]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s STEPS OUTFILE [ORDER [PHASES]]\n", argv[0]);
         return 2;
     }
     const int steps = std::atoi(argv[1]);
     PhysicsEngine pe(0);
     pe.initializeWorld_Classic_Bar();
     pe.i_SummationOrder = argc > 3 ? std::atoi(argv[3]) : MPM_ORDER_PARTICLE;
+    const bool phases = argc > 4 && std::atoi(argv[4]) != 0;
     /* STEPS = 0 writes the world as built, without touching the device */
-    const int status = steps > 0 ? pe.runSimulation_Classic_DoublePass_MPLocks(steps * pe.getTime_Increment()) : 0;
+    int status = 0;
+    if (steps > 0 && !phases) status = pe.runSimulation_Classic_DoublePass_MPLocks(steps * pe.getTime_Increment());
+    for (int s = 0; phases && s < steps && status >= 0; s++) {
+        static const int32_t order[] = {MPM_PHASE_BIN, MPM_PHASE_P2G_GRID, MPM_PHASE_G2P_VELOCITY,
+                                        MPM_PHASE_REMAP_GRID, MPM_PHASE_G2P_STRAIN};
+        for (int32_t ph : order)
+            if (status >= 0) status = pe.runSimulation_Phase(MPM_VARIANT_DOUBLEPASS, ph, pe.getTime_Increment());
+    }
     if (status < 0) {
         std::fprintf(stderr, "runSimulation failed: %s\n", pe.last_error());
         return 1;

@@ -407,6 +407,19 @@ int PhysicsEngine::run_loop(double total, int32_t variant) {
     return 1;
 }
 
+int PhysicsEngine::runSimulation_Phase(int32_t variant, int32_t phase, double dt) {
+    if (!gpu) return -1;
+    if (!b_Uploaded && upload() != MPM_OK) return -1;
+    if (phase == MPM_PHASE_BIN) {
+        if (mpm_set_variant(gpu, variant) != MPM_OK) return fail(0), -1;
+        d_PhaseIncrement = dt;
+    }
+    if (mpm_step_phase(gpu, phase, dt) != MPM_OK) return fail(0), -1;
+    b_ParticlesFresh = b_GridFresh = false;
+    if (phase == MPM_PHASE_G2P_STRAIN) d_Time += d_PhaseIncrement;
+    return 0;
+}
+
 int PhysicsEngine::runSimulation_Classic_DoublePass_MPLocks(double total) {
     return run_loop(total, MPM_VARIANT_DOUBLEPASS);
 }

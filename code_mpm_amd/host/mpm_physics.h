@@ -203,6 +203,13 @@ class PhysicsEngine {
     int runSimulation_Classic_DoublePass_MPLocks(double dTimeIncrement_Total);
     int runSimulation_Classic_DoublePass_MPLocks_Contact(double dTimeIncrement_Total);
     int runSimulation_Classic_SinglePass_MP(double dTimeIncrement_Total);
+    /* The loops' components (PhysicsEngine.h:57-63, with the inline pass 1
+     * and remap) as the device step's phases, for scene code that drives a
+     * step itself: MPM_PHASE_BIN (reset + adjacent-node search) opens a step
+     * of dTimeIncrement in the loop `variant` (MPM_VARIANT_*), the others
+     * follow in that loop's order (mpm_step_phase) and MPM_PHASE_G2P_STRAIN
+     * closes it and advances d_Time.  0, or -1 with last_error() set. */
+    int runSimulation_Phase(int32_t variant, int32_t phase, double dTimeIncrement);
 
     double getTime_Current() const { return d_Time; }
     double getTime_End() const { return d_TimeEnd; }
@@ -254,6 +261,7 @@ class PhysicsEngine {
 
     mpm_engine *gpu = nullptr;
     bool b_Uploaded = false;
+    double d_PhaseIncrement = 0.0; /* the open phase-by-phase step's dt */
     bool b_ParticlesFresh = true, b_GridFresh = true;
     std::string s_Error;
 };

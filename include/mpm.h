@@ -320,6 +320,23 @@ enum {
  * PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:487-790).
  * Synchronises after every step. */
 int mpm_step_timed(mpm_engine *e, double dt, int32_t nsteps, double phase_ms[MPM_PHASE_COUNT]);
+/* One step taken phase by phase, for callers that run the step loop's
+ * components themselves (PhysicsEngine.h:57-63: runSimulation_ResetGrid,
+ * _FindAGPs, _M2G, _DisplacementControl, _IntegrateGrid and
+ * _G2P_P2_SmallStrain / _LargeStrain, with the inline pass 1 and remap of
+ * PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:462-803).  On
+ * the device they come as these phases: MPM_PHASE_BIN (grid reset and the
+ * adjacent-node search: cell binning) opens the step and takes its dt (as
+ * mpm_step's; <= 0 for each instance's dt_max); then, in the loop's order,
+ * MPM_PHASE_P2G_GRID (M2G, displacement control and grid integration),
+ * MPM_PHASE_G2P_VELOCITY and MPM_PHASE_REMAP_GRID (DoublePass and contact
+ * loops: pass 1, then the remap; single pass: the velocity sum, then its G2P
+ * velocity), and MPM_PHASE_G2P_STRAIN, which closes the step (time += dt).
+ * dt is ignored after the first phase.  The phases of one step equal one
+ * mpm_step.  A phase out of order returns MPM_ERR_STATE; while a step is
+ * open, set-up and other stepping calls return MPM_ERR_STATE (getters
+ * work).  A failed launch abandons the step. */
+int mpm_step_phase(mpm_engine *e, int32_t phase, double dt);
 /* The same with one time per kernel (HIP events on the engine's stream),
  * summed over nsteps, in these slots.  A phase that runs one kernel in the
  * particle order (P2G, remap) books it in its first slot; with a surrogate

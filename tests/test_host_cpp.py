@@ -31,6 +31,7 @@ def test_host_library_exports_reference_members():
     for member in ("PhysicsEngine::runSimulation_Classic_DoublePass_MPLocks(double)",
                    "PhysicsEngine::runSimulation_Classic_DoublePass_MPLocks_Contact(double)",
                    "PhysicsEngine::runSimulation_Classic_SinglePass_MP(double)",
+                   "PhysicsEngine::runSimulation_Phase(int, int, double)",
                    "PhysicsEngine::getMaterialPoints()", "PhysicsEngine::getGridPoints()",
                    "PhysicsEngine::reportConsole()", "GridPoint_Factory::createGrid",
                    "MaterialPoint_Factory_Classic_CC::createDomain_Cuboid",
