@@ -731,6 +731,27 @@ __device__ __forceinline__ void node_p2g_finish(const Batch &bt, const NodeIn &i
         node_integrate(in, Nd, node, nn, m, p, f, w);
 }
 
+/* The particle fields one particle-order P2G term reads. */
+struct P2GIn {
+    double x[3], v[3], m, V, sig[6], fe[3];
+    int dcr;
+};
+
+__device__ __forceinline__ void p2g_load(P2GIn &d, const PView &P, uint32_t id, bool fext) {
+    const size_t n = P.n;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        d.x[a] = P.x[a * n + id];
+        d.v[a] = P.v[a * n + id];
+        d.fe[a] = fext ? P.fext[a * n + id] : 0.0;
+    }
+    d.m = P.m[id];
+    d.V = P.V[id];
+#pragma unroll
+    for (int k = 0; k < 6; k++) d.sig[k] = P.sig[k * n + id];
+    d.dcr = P.dc_rank[id];
+}
+
 __global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd, BView B) {
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const Inst &I = bt.inst[inst_of_tile(bt, tile)];
@@ -739,7 +760,6 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd,
     node_of_tile(bt, g, tile - I.tile_off, ix, iy, iz);
     if (ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) return;
     const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
-    const size_t n = P.n;
     const double xn[3] = {node_coord(ix, g.L[0], g.cells[0]), node_coord(iy, g.L[1], g.cells[1]),
                           node_coord(iz, g.L[2], g.cells[2])};
     const NodeIn in = node_in<false>(bt, I, Nd, node);
@@ -755,39 +775,42 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd,
     w.rank[0] = w.rank[1] = w.rank[2] = -1;
     w.v[0] = w.v[1] = w.v[2] = 0.0;
     const bool all_axes = bt.variant != VAR_DP, skip_dc = bt.variant == VAR_SP;
+    const bool fext = I.has_fext;
+    /* software-pipelined: the next particle (from the merge) is fetched
+     * while the current one is added */
+    P2GIn cur;
+    p2g_load(cur, P, id, fext);
     while (id != MPM_NONE) {
-        if (skip_dc && P.dc_rank[id] >= 0) { /* SP:116-117, 157-158 */
-            dc_consider(w, P, id, all_axes);
-            merge_advance(M, B, id);
-            id = merge_min(M);
-            continue;
-        }
-        /* runSimulation_M2G body, DP:69-84 */
-        AxisShape ax = axis_shape(P.x[id], xn[0], g.h[0]);
-        AxisShape ay = axis_shape(P.x[n + id], xn[1], g.h[1]);
-        AxisShape az = axis_shape(P.x[2 * n + id], xn[2], g.h[2]);
-        double N, gx, gy, gz;
-        shape_full(ax, ay, az, g.h, N, gx, gy, gz);
-        double mp = P.m[id];
-        m += N * mp;
-        p[0] += N * (mp * P.v[id]);
-        p[1] += N * (mp * P.v[n + id]);
-        p[2] += N * (mp * P.v[2 * n + id]);
-        double V = P.V[id];
-        const double *s = P.sig;
-        double s0 = s[id], s1 = s[n + id], s2 = s[2 * n + id], s3 = s[3 * n + id], s4 = s[4 * n + id],
-               s5 = s[5 * n + id];
-        f[0] += -V * (gx * s0 + gy * s3 + gz * s5);
-        f[1] += -V * (gy * s1 + gx * s3 + gz * s4);
-        f[2] += -V * (gz * s2 + gx * s5 + gy * s4);
-        if (I.has_fext) {
-            f[0] += N * P.fext[id];
-            f[1] += N * P.fext[n + id];
-            f[2] += N * P.fext[2 * n + id];
+        merge_advance(M, B, id);
+        const uint32_t nid = merge_min(M);
+        P2GIn nxt; /* past the last particle: the current one again, unused */
+        p2g_load(nxt, P, nid != MPM_NONE ? nid : id, fext);
+        if (!(skip_dc && cur.dcr >= 0)) { /* SP:116-117, 157-158 */
+            /* runSimulation_M2G body, DP:69-84 */
+            AxisShape ax = axis_shape(cur.x[0], xn[0], g.h[0]);
+            AxisShape ay = axis_shape(cur.x[1], xn[1], g.h[1]);
+            AxisShape az = axis_shape(cur.x[2], xn[2], g.h[2]);
+            double N, gx, gy, gz;
+            shape_full(ax, ay, az, g.h, N, gx, gy, gz);
+            const double mp = cur.m;
+            m += N * mp;
+            p[0] += N * (mp * cur.v[0]);
+            p[1] += N * (mp * cur.v[1]);
+            p[2] += N * (mp * cur.v[2]);
+            const double V = cur.V;
+            const double *sg = cur.sig;
+            f[0] += -V * (gx * sg[0] + gy * sg[3] + gz * sg[5]);
+            f[1] += -V * (gy * sg[1] + gx * sg[3] + gz * sg[4]);
+            f[2] += -V * (gz * sg[2] + gx * sg[5] + gy * sg[4]);
+            if (fext) {
+                f[0] += N * cur.fe[0];
+                f[1] += N * cur.fe[1];
+                f[2] += N * cur.fe[2];
+            }
         }
         dc_consider(w, P, id, all_axes);
-        merge_advance(M, B, id);
-        id = merge_min(M);
+        id = nid;
+        cur = nxt;
     }
     node_p2g_finish(bt, in, Nd, node, m, p, f, w);
 }
@@ -924,6 +947,12 @@ __device__ __forceinline__ void node_remap_dispatch(const Batch &bt, const NodeI
  * order, then the second displacement-control override (DP:568) and the BC
  * re-application (DP:573-599). Stores the final v, p, the zeroed force
  * components and the hoisted p/m used by G2P pass 2 (DP:198). */
+/* The particle fields one particle-order remap term reads. */
+struct RemapIn {
+    double x[3], v[3], m;
+    int dcr;
+};
+
 __global__ __launch_bounds__(BLOCK) void k_remap_grid(Batch bt, PView P, NView Nd, BView B) {
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const Inst &I = bt.inst[inst_of_tile(bt, tile)];
@@ -946,17 +975,35 @@ __global__ __launch_bounds__(BLOCK) void k_remap_grid(Batch bt, PView P, NView N
     w.rank[0] = w.rank[1] = w.rank[2] = -1;
     w.v[0] = w.v[1] = w.v[2] = 0.0;
     const bool all_axes = bt.variant != VAR_DP, skip_dc = bt.variant != VAR_DP;
-    for (uint32_t id = merge_min(M); id != MPM_NONE; id = merge_min(M)) {
-        if (massive && !(skip_dc && P.dc_rank[id] >= 0)) { /* DPC:358-359, SP:157-158 */
-            double N = axis_shape(P.x[id], xn[0], g.h[0]).s * axis_shape(P.x[n + id], xn[1], g.h[1]).s *
-                       axis_shape(P.x[2 * n + id], xn[2], g.h[2]).s;
-            double mp = P.m[id];
-            v[0] += N * (mp * P.v[id]) / m;
-            v[1] += N * (mp * P.v[n + id]) / m;
-            v[2] += N * (mp * P.v[2 * n + id]) / m;
+    /* software-pipelined like k_p2g_grid */
+    auto load = [&](RemapIn &d, uint32_t id) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            d.x[a] = P.x[a * n + id];
+            d.v[a] = P.v[a * n + id];
+        }
+        d.m = P.m[id];
+        d.dcr = P.dc_rank[id];
+    };
+    uint32_t id = merge_min(M);
+    RemapIn cur;
+    if (id != MPM_NONE) load(cur, id);
+    while (id != MPM_NONE) {
+        merge_advance(M, B, id);
+        const uint32_t nid = merge_min(M);
+        RemapIn nxt; /* past the last particle: the current one again, unused */
+        load(nxt, nid != MPM_NONE ? nid : id);
+        if (massive && !(skip_dc && cur.dcr >= 0)) { /* DPC:358-359, SP:157-158 */
+            double N = axis_shape(cur.x[0], xn[0], g.h[0]).s * axis_shape(cur.x[1], xn[1], g.h[1]).s *
+                       axis_shape(cur.x[2], xn[2], g.h[2]).s;
+            double mp = cur.m;
+            v[0] += N * (mp * cur.v[0]) / m;
+            v[1] += N * (mp * cur.v[1]) / m;
+            v[2] += N * (mp * cur.v[2]) / m;
         }
         dc_consider(w, P, id, all_axes);
-        merge_advance(M, B, id);
+        id = nid;
+        cur = nxt;
     }
     node_remap_dispatch(bt, in, Nd, node, v, w);
 }
