@@ -54,8 +54,9 @@ KERNELS = {
              "remap_nodes": "k_node_cells<true, false>", "g2p_kinematics": "k_g2p_fused<0, false>",
              "g2p_constitutive": None},
     "particle": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
-                 "p2g_partials": "k_p2g_grid", "p2g_nodes": None, "g2p_velocity": "k_g2p_velocity<false>",
-                 "remap_partials": "k_remap_grid", "remap_nodes": None,
+                 "p2g_partials": "k_cell_partials<false, false, false, true>", "p2g_nodes": "k_p2g_list<false>",
+                 "g2p_velocity": "k_g2p_velocity<false>",
+                 "remap_partials": "k_cell_partials<true, false, false, true>", "remap_nodes": "k_remap_list",
                  "g2p_kinematics": "k_g2p_fused<0, false>", "g2p_constitutive": None},
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_honeycomb8m.txt")

@@ -180,7 +180,7 @@ struct mpm_engine {
         free_parts();
     }
     void free_parts() {
-        dfree(parts.w); dfree(bocc);
+        dfree(parts.w); dfree(bocc); dfree(parts.list);
         parts = CellParts{};
         parts_cap = 0;
         occ_cap = 0;
@@ -203,7 +203,8 @@ struct mpm_engine {
         int64_t cap = occ_cap;
         if (cap == parts_cap && parts.w) return MPM_OK;
         dfree(parts.w);
-        parts = CellParts{};
+        parts.w = nullptr;
+        parts.cap = 0;
         parts_cap = 0;
         const size_t c8 = 8 * (size_t)cap;
         int rc;
@@ -213,6 +214,21 @@ struct mpm_engine {
         }
         parts.cap = cap;
         parts_cap = cap;
+        drop_graphs();
+        return MPM_OK;
+    }
+    /* Particle order: the per-(particle, corner) P2G terms, 10 doubles each
+     * (7 without f_ext; sized for the larger). */
+    int ensure_list() {
+        const int64_t cap = std::max<int64_t>(n, 1);
+        if (parts.list && parts.lcap == cap) return MPM_OK;
+        dfree(parts.list);
+        parts.list = nullptr;
+        parts.lcap = 0;
+        int rc;
+        if ((rc = hip(hipMalloc(&parts.list, (size_t)cap * 8u * 10u * sizeof(double)), "alloc particle-order terms")))
+            return rc;
+        parts.lcap = cap;
         drop_graphs();
         return MPM_OK;
     }
@@ -492,6 +508,7 @@ struct mpm_engine {
         if (sur && (rc = ensure_queue())) return rc;
         if ((rc = ensure_occ())) return rc;
         if (order == ORDER_CELL && (rc = ensure_parts())) return rc;
+        if (order == ORDER_PARTICLE && (rc = ensure_list())) return rc;
         return ensure_keys();
     }
 

@@ -103,6 +103,10 @@ enum { STRAIN_SMALL = 0, STRAIN_LARGE = 1 };
 struct CellParts {
     double *w;
     int64_t cap; /* occupied-cell capacity */
+    /* particle order: per (sorted position, corner) the P2G terms of that
+     * particle at that corner node (7 doubles, 10 with f_ext) */
+    double *list;
+    int64_t lcap; /* particle capacity */
 };
 
 /* slot[] of a cell: its rank among occupied cells, MPM_NONE when empty,

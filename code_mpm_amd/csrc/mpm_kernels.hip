@@ -731,90 +731,6 @@ __device__ __forceinline__ void node_p2g_finish(const Batch &bt, const NodeIn &i
         node_integrate(in, Nd, node, nn, m, p, f, w);
 }
 
-/* The particle fields one particle-order P2G term reads. */
-struct P2GIn {
-    double x[3], v[3], m, V, sig[6], fe[3];
-    int dcr;
-};
-
-__device__ __forceinline__ void p2g_load(P2GIn &d, const PView &P, uint32_t id, bool fext) {
-    const size_t n = P.n;
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        d.x[a] = P.x[a * n + id];
-        d.v[a] = P.v[a * n + id];
-        d.fe[a] = fext ? P.fext[a * n + id] : 0.0;
-    }
-    d.m = P.m[id];
-    d.V = P.V[id];
-#pragma unroll
-    for (int k = 0; k < 6; k++) d.sig[k] = P.sig[k * n + id];
-    d.dcr = P.dc_rank[id];
-}
-
-__global__ __launch_bounds__(BLOCK) void k_p2g_grid(Batch bt, PView P, NView Nd, BView B) {
-    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-    const Inst &I = bt.inst[inst_of_tile(bt, tile)];
-    const GridGeom &g = I.g;
-    int ix, iy, iz;
-    node_of_tile(bt, g, tile - I.tile_off, ix, iy, iz);
-    if (ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) return;
-    const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
-    const double xn[3] = {node_coord(ix, g.L[0], g.cells[0]), node_coord(iy, g.L[1], g.cells[1]),
-                          node_coord(iz, g.L[2], g.cells[2])};
-    const NodeIn in = node_in<false>(bt, I, Nd, node);
-    Merge8 M;
-    merge_init(M, B, g, I.cell_off, ix, iy, iz);
-    uint32_t id = merge_min(M);
-    if (id == MPM_NONE) {
-        Nd.active[node] = 0;
-        return;
-    }
-    double m = 0.0, p[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0};
-    DCWin w;
-    w.rank[0] = w.rank[1] = w.rank[2] = -1;
-    w.v[0] = w.v[1] = w.v[2] = 0.0;
-    const bool all_axes = bt.variant != VAR_DP, skip_dc = bt.variant == VAR_SP;
-    const bool fext = I.has_fext;
-    /* software-pipelined: the next particle (from the merge) is fetched
-     * while the current one is added */
-    P2GIn cur;
-    p2g_load(cur, P, id, fext);
-    while (id != MPM_NONE) {
-        merge_advance(M, B, id);
-        const uint32_t nid = merge_min(M);
-        P2GIn nxt; /* past the last particle: the current one again, unused */
-        p2g_load(nxt, P, nid != MPM_NONE ? nid : id, fext);
-        if (!(skip_dc && cur.dcr >= 0)) { /* SP:116-117, 157-158 */
-            /* runSimulation_M2G body, DP:69-84 */
-            AxisShape ax = axis_shape(cur.x[0], xn[0], g.h[0]);
-            AxisShape ay = axis_shape(cur.x[1], xn[1], g.h[1]);
-            AxisShape az = axis_shape(cur.x[2], xn[2], g.h[2]);
-            double N, gx, gy, gz;
-            shape_full(ax, ay, az, g.h, N, gx, gy, gz);
-            const double mp = cur.m;
-            m += N * mp;
-            p[0] += N * (mp * cur.v[0]);
-            p[1] += N * (mp * cur.v[1]);
-            p[2] += N * (mp * cur.v[2]);
-            const double V = cur.V;
-            const double *sg = cur.sig;
-            f[0] += -V * (gx * sg[0] + gy * sg[3] + gz * sg[5]);
-            f[1] += -V * (gy * sg[1] + gx * sg[3] + gz * sg[4]);
-            f[2] += -V * (gz * sg[2] + gx * sg[5] + gy * sg[4]);
-            if (fext) {
-                f[0] += N * cur.fe[0];
-                f[1] += N * cur.fe[1];
-                f[2] += N * cur.fe[2];
-            }
-        }
-        dc_consider(w, P, id, all_axes);
-        id = nid;
-        cur = nxt;
-    }
-    node_p2g_finish(bt, in, Nd, node, m, p, f, w);
-}
-
 /* G2P pass 1 for one particle: v_p += N * (f_i / m_i) * dt over the
  * adjacent nodes in GridPoint_Mediator order (x, then y, then z innermost),
  * DP:490-510. */
@@ -947,67 +863,6 @@ __device__ __forceinline__ void node_remap_dispatch(const Batch &bt, const NodeI
  * order, then the second displacement-control override (DP:568) and the BC
  * re-application (DP:573-599). Stores the final v, p, the zeroed force
  * components and the hoisted p/m used by G2P pass 2 (DP:198). */
-/* The particle fields one particle-order remap term reads. */
-struct RemapIn {
-    double x[3], v[3], m;
-    int dcr;
-};
-
-__global__ __launch_bounds__(BLOCK) void k_remap_grid(Batch bt, PView P, NView Nd, BView B) {
-    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-    const Inst &I = bt.inst[inst_of_tile(bt, tile)];
-    const GridGeom &g = I.g;
-    int ix, iy, iz;
-    node_of_tile(bt, g, tile - I.tile_off, ix, iy, iz);
-    if (ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) return;
-    const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
-    if (!Nd.active[node]) return;
-    const size_t n = P.n, nn = (size_t)bt.nnodes;
-    const double xn[3] = {node_coord(ix, g.L[0], g.cells[0]), node_coord(iy, g.L[1], g.cells[1]),
-                          node_coord(iz, g.L[2], g.cells[2])};
-    const NodeIn in = node_in<true>(bt, I, Nd, node);
-    const double m = in.m;
-    const bool massive = m > in.mass_min;
-    Merge8 M;
-    merge_init(M, B, g, I.cell_off, ix, iy, iz);
-    double v[3] = {0.0, 0.0, 0.0};
-    DCWin w;
-    w.rank[0] = w.rank[1] = w.rank[2] = -1;
-    w.v[0] = w.v[1] = w.v[2] = 0.0;
-    const bool all_axes = bt.variant != VAR_DP, skip_dc = bt.variant != VAR_DP;
-    /* software-pipelined like k_p2g_grid */
-    auto load = [&](RemapIn &d, uint32_t id) {
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            d.x[a] = P.x[a * n + id];
-            d.v[a] = P.v[a * n + id];
-        }
-        d.m = P.m[id];
-        d.dcr = P.dc_rank[id];
-    };
-    uint32_t id = merge_min(M);
-    RemapIn cur;
-    if (id != MPM_NONE) load(cur, id);
-    while (id != MPM_NONE) {
-        merge_advance(M, B, id);
-        const uint32_t nid = merge_min(M);
-        RemapIn nxt; /* past the last particle: the current one again, unused */
-        load(nxt, nid != MPM_NONE ? nid : id);
-        if (massive && !(skip_dc && cur.dcr >= 0)) { /* DPC:358-359, SP:157-158 */
-            double N = axis_shape(cur.x[0], xn[0], g.h[0]).s * axis_shape(cur.x[1], xn[1], g.h[1]).s *
-                       axis_shape(cur.x[2], xn[2], g.h[2]).s;
-            double mp = cur.m;
-            v[0] += N * (mp * cur.v[0]) / m;
-            v[1] += N * (mp * cur.v[1]) / m;
-            v[2] += N * (mp * cur.v[2]) / m;
-        }
-        dc_consider(w, P, id, all_axes);
-        id = nid;
-        cur = nxt;
-    }
-    node_remap_dispatch(bt, in, Nd, node, v, w);
-}
-
 __device__ __forceinline__ void node_remap_dispatch(const Batch &bt, const NodeIn &in, const NView &Nd, int64_t node,
                                                     double v[3], const DCWin &w) {
     const size_t nn = (size_t)bt.nnodes;
@@ -1064,7 +919,12 @@ __device__ __forceinline__ void particle_factors(const GridGeom &g, const double
     }
 }
 
-template <bool REMAP, bool MULTI, bool FEXT = false>
+/* LIST (for the particle summation order): each lane stores its cell's
+ * particles' terms at its corner node in C.list instead of summing them;
+ * k_p2g_list / k_remap_list then add them per node in ascending particle
+ * id.  The remap stores terms only at nodes above the mass minimum, the
+ * only ones that add them. */
+template <bool REMAP, bool MULTI, bool FEXT = false, bool LIST = false>
 __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NView Nd, BView B, CellParts C) {
     /* staged per particle: m v (3) and either the shape factors s (6, remap)
      * or x (3, P2G), then for P2G m, V, sigma (6) and, with FEXT, f_ext (3);
@@ -1135,7 +995,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
                 mn = Nd.m[node];
                 massive = mn > I.mass_min;
             }
-            if (!REMAP && have && bt.tflag) bt.tflag[I.tile_off + tile_of_node(bt, g, ix, iy, iz)] = 1;
+            if (!REMAP && !LIST && have && bt.tflag) bt.tflag[I.tile_off + tile_of_node(bt, g, ix, iy, iz)] = 1;
         }
         double m = 0.0, p[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0};
         for (uint32_t base = r0; base < r1; base += PS) {
@@ -1205,6 +1065,25 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
                     const double gy = (by & 1u) ? ((by & 2u) ? -qy : qy) : 0.0;
                     const double gz = (bz & 1u) ? ((bz & 2u) ? -qz : qz) : 0.0;
                     const double N = sx * sy * sz;
+                    if (LIST) {
+                        if (have) {
+                            double *r = C.list + ((size_t)q * 8u + (size_t)corner) * (FEXT ? 10u : 7u);
+                            const double V = sV[j];
+                            r[0] = N * sm[j];
+                            r[1] = N * spv[0][j];
+                            r[2] = N * spv[1][j];
+                            r[3] = N * spv[2][j];
+                            r[4] = -V * (gx * ss[0][j] + gy * ss[3][j] + gz * ss[5][j]);
+                            r[5] = -V * (gy * ss[1][j] + gx * ss[3][j] + gz * ss[4][j]);
+                            r[6] = -V * (gz * ss[2][j] + gx * ss[5][j] + gy * ss[4][j]);
+                            if (fext) {
+                                r[FEXT ? 7 : 0] = N * sf[0][j];
+                                r[FEXT ? 8 : 0] = N * sf[1][j];
+                                r[FEXT ? 9 : 0] = N * sf[2][j];
+                            }
+                        }
+                        continue;
+                    }
                     m += N * sm[j];
                     p[0] += N * spv[0][j];
                     p[1] += N * spv[1][j];
@@ -1224,13 +1103,20 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
                     const uint32_t j = q - base;
                     if (sdc[j]) continue; /* DPC:358-359, SP:157-158 */
                     const double N = sS[0 + ox][j] * sS[2 + oy][j] * sS[4 + oz][j];
+                    if (LIST) {
+                        double *r = C.list + ((size_t)q * 8u + (size_t)corner) * 3u;
+                        r[0] = N * spv[0][j] / mn;
+                        r[1] = N * spv[1][j] / mn;
+                        r[2] = N * spv[2][j] / mn;
+                        continue;
+                    }
                     f[0] += N * spv[0][j] / mn;
                     f[1] += N * spv[1][j] / mn;
                     f[2] += N * spv[2][j] / mn;
                 }
             }
         }
-        if (!have) continue;
+        if (!have || LIST) continue;
         const size_t t = (size_t)corner * (size_t)C.cap + (size_t)slot;
         if (!REMAP) {
             /* one 64-byte record per (cell, corner): m, p, f */
@@ -1252,6 +1138,105 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
  * (cells holding one are flagged in their slot by the binning). */
 __device__ __forceinline__ void dc_scan_cell(DCWin &w, const PView &P, const BView &B, uint32_t c, bool all_axes) {
     for (uint32_t q = B.start[c]; q < B.start[c + 1]; q++) dc_consider(w, P, B.sorted[q], all_axes);
+}
+
+/* Node side of the particle order from per-particle terms (k_cell_partials
+ * with LIST): the node merges its 8 cells' sorted id lists and adds each
+ * particle's stored terms at this node (its corner 7 - k in cell k) in
+ * ascending id, the order of the reference's one-thread M2G loop
+ * (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:48-89). */
+template <bool FEXT>
+__global__ __launch_bounds__(BLOCK) void k_p2g_list(Batch bt, PView P, NView Nd, BView B, CellParts C) {
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const Inst &I = bt.inst[inst_of_tile(bt, tile)];
+    const GridGeom &g = I.g;
+    int ix, iy, iz;
+    node_of_tile(bt, g, tile - I.tile_off, ix, iy, iz);
+    if (ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) return;
+    const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
+    const NodeIn in = node_in<false>(bt, I, Nd, node);
+    Merge8 M;
+    merge_init(M, B, g, I.cell_off, ix, iy, iz);
+    uint32_t id = merge_min(M);
+    if (id == MPM_NONE) {
+        Nd.active[node] = 0;
+        return;
+    }
+    double m = 0.0, p[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0};
+    DCWin w;
+    w.rank[0] = w.rank[1] = w.rank[2] = -1;
+    w.v[0] = w.v[1] = w.v[2] = 0.0;
+    const bool all_axes = bt.variant != VAR_DP, skip_dc = bt.variant == VAR_SP;
+    const bool fext = FEXT && I.has_fext;
+    constexpr size_t RS = FEXT ? 10u : 7u;
+    while (id != MPM_NONE) {
+        uint32_t pos = 0, corner = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (M.h[k] == id) {
+                pos = M.p[k];
+                corner = 7u - (uint32_t)k;
+            }
+        if (!(skip_dc && P.dc_rank[id] >= 0)) { /* SP:116-117, 157-158 */
+            const double *r = C.list + ((size_t)pos * 8u + (size_t)corner) * RS;
+            m += r[0];
+            p[0] += r[1];
+            p[1] += r[2];
+            p[2] += r[3];
+            f[0] += r[4];
+            f[1] += r[5];
+            f[2] += r[6];
+            if (fext) {
+                f[0] += r[FEXT ? 7 : 0];
+                f[1] += r[FEXT ? 8 : 0];
+                f[2] += r[FEXT ? 9 : 0];
+            }
+        }
+        dc_consider(w, P, id, all_axes);
+        merge_advance(M, B, id);
+        id = merge_min(M);
+    }
+    node_p2g_finish(bt, in, Nd, node, m, p, f, w);
+}
+
+/* The remap's node side in the particle order from the stored terms (the
+ * ascending-id sums of the reference's one-thread loop). */
+__global__ __launch_bounds__(BLOCK) void k_remap_list(Batch bt, PView P, NView Nd, BView B, CellParts C) {
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const Inst &I = bt.inst[inst_of_tile(bt, tile)];
+    const GridGeom &g = I.g;
+    int ix, iy, iz;
+    node_of_tile(bt, g, tile - I.tile_off, ix, iy, iz);
+    if (ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) return;
+    const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
+    if (!Nd.active[node]) return;
+    const NodeIn in = node_in<true>(bt, I, Nd, node);
+    const bool massive = in.m > in.mass_min;
+    Merge8 M;
+    merge_init(M, B, g, I.cell_off, ix, iy, iz);
+    double v[3] = {0.0, 0.0, 0.0};
+    DCWin w;
+    w.rank[0] = w.rank[1] = w.rank[2] = -1;
+    w.v[0] = w.v[1] = w.v[2] = 0.0;
+    const bool all_axes = bt.variant != VAR_DP, skip_dc = bt.variant != VAR_DP;
+    for (uint32_t id = merge_min(M); id != MPM_NONE; id = merge_min(M)) {
+        uint32_t pos = 0, corner = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (M.h[k] == id) {
+                pos = M.p[k];
+                corner = 7u - (uint32_t)k;
+            }
+        if (massive && !(skip_dc && P.dc_rank[id] >= 0)) { /* DPC:358-359, SP:157-158 */
+            const double *r = C.list + ((size_t)pos * 8u + (size_t)corner) * 3u;
+            v[0] += r[0];
+            v[1] += r[1];
+            v[2] += r[2];
+        }
+        dc_consider(w, P, id, all_axes);
+        merge_advance(M, B, id);
+    }
+    node_remap_dispatch(bt, in, Nd, node, v, w);
 }
 
 /* Node side of the cell order: adds the 8 partials in neighbour order
@@ -2499,8 +2484,20 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
             if (mid) (void)hipEventRecord(mid, s);
             LAUNCH_MULTI(bt, (k_node_cells<false, M>), dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
         } else {
-            hipLaunchKernelGGL(k_p2g_grid, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B);
+            if (!C.list) return hipErrorInvalidValue;
+            if (P.n > 0) {
+                if (bt.fext)
+                    LAUNCH_MULTI(bt, (k_cell_partials<false, M, true, true>), dim3(partial_grid(P.n)), dim3(BLOCK), 0,
+                                 s, bt, P, N, B, C);
+                else
+                    LAUNCH_MULTI(bt, (k_cell_partials<false, M, false, true>), dim3(partial_grid(P.n)), dim3(BLOCK),
+                                 0, s, bt, P, N, B, C);
+            }
             if (mid) (void)hipEventRecord(mid, s);
+            if (bt.fext)
+                hipLaunchKernelGGL(k_p2g_list<true>, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
+            else
+                hipLaunchKernelGGL(k_p2g_list<false>, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
         }
         break;
     case PHASE_G2P_VELOCITY:
@@ -2522,8 +2519,12 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
             if (mid) (void)hipEventRecord(mid, s);
             LAUNCH_MULTI(bt, (k_node_cells<true, M>), dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
         } else {
-            hipLaunchKernelGGL(k_remap_grid, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B);
+            if (!C.list) return hipErrorInvalidValue;
+            if (P.n > 0)
+                LAUNCH_MULTI(bt, (k_cell_partials<true, M, false, true>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s,
+                             bt, P, N, B, C);
             if (mid) (void)hipEventRecord(mid, s);
+            hipLaunchKernelGGL(k_remap_list, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
         }
         break;
     case PHASE_G2P_STRAIN:
