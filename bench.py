@@ -49,8 +49,8 @@ FUSED_G2P = {"g2p_kinematics": (26 * 8 + 5 + 30 * 8, 7 * 8)}
 # kernel of each slot, per summation order (names as rocprofv3 reports them)
 KERNELS = {
     "cell": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
-             "p2g_partials": "k_cell_partials<false, false, false>", "p2g_nodes": "k_node_cells<false, false>",
-             "g2p_velocity": "k_g2p_velocity<false>", "remap_partials": "k_cell_partials<true, false, false>",
+             "p2g_partials": "k_cell_partials<false, false, false, false>", "p2g_nodes": "k_node_cells<false, false>",
+             "g2p_velocity": "k_g2p_velocity<false>", "remap_partials": "k_cell_partials<true, false, false, false>",
              "remap_nodes": "k_node_cells<true, false>", "g2p_kinematics": "k_g2p_fused<0, false>",
              "g2p_constitutive": None},
     "particle": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
