@@ -504,9 +504,10 @@ def graded_foam(seed, h=1.0e-4, n_target=None, E=10.0e9, nu=0.3, rho=1000.0, yie
     """One graded-foam instance (the batched workload): a cuboid of 2x2x2
     particles per cell with spherical voids whose radii are lognormal
     (mu = ln 3h, sigma = 0.3) and whose number density grows linearly along
-    z; porosity ~ U[0.3, 0.7] and particle count ~ log-uniform [1e4, 2e5],
-    both drawn from ``seed``.  J2 perfectly plastic.  Fixed-y support below,
-    displacement-controlled platen on top at platen_speed."""
+    z (three times denser at the back face than at the front), across the
+    loading axis y; porosity ~ U[0.3, 0.7] and particle count ~ log-uniform
+    [1e4, 2e5], both drawn from ``seed``.  J2 perfectly plastic.  Fixed-y
+    support below, displacement-controlled platen on top at platen_speed."""
     rng = np.random.default_rng(seed)
     porosity = rng.uniform(0.3, 0.7)
     if n_target is None:
@@ -524,8 +525,9 @@ def graded_foam(seed, h=1.0e-4, n_target=None, E=10.0e9, nu=0.3, rho=1000.0, yie
     lo = np.array([margin, 1, margin]) * h
     hi = lo + body * h
     keep = np.ones(x.shape[1], bool)
-    # voids: add until the porosity is reached; centre density linear in y (1 at the bottom, 3 at the top).
-    # Particles come 8 per cell, cells x fastest: a void only tests the cells of its bounding box.
+    # voids: add until the porosity is reached; centre density linear in z (1 at the front face z = lo,
+    # 3 at the back), across the loading axis y.  Particles come 8 per cell, cells x fastest: a void
+    # only tests the cells of its bounding box.
     target_removed = porosity * x.shape[1]
     removed = 0
     sub8 = np.arange(8)
@@ -535,7 +537,7 @@ def graded_foam(seed, h=1.0e-4, n_target=None, E=10.0e9, nu=0.3, rho=1000.0, yie
         r = np.exp(rng.normal(np.log(3.0 * h), 0.3))
         while True:
             c = lo + rng.uniform(0, 1, 3) * (hi - lo)
-            if rng.uniform() < (1.0 + 2.0 * (c[1] - lo[1]) / (hi[1] - lo[1])) / 3.0:
+            if rng.uniform() < (1.0 + 2.0 * (c[2] - lo[2]) / (hi[2] - lo[2])) / 3.0:
                 break
         a0 = np.clip(np.floor((c - r - lo) / h).astype(int) - 1, 0, body - 1)
         a1 = np.clip(np.floor((c + r - lo) / h).astype(int) + 1, 0, body - 1)
