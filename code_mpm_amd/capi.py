@@ -147,6 +147,7 @@ SIGNATURES = {
     "mpm_surrogate_set_normalization": (I32, [P, DBL, DBL, P, P, P, P, P, P]),
     "mpm_surrogate_forward": (I32, [P, P, P, I64]),
     "mpm_surrogate_gradient": (I32, [P, P, P, I64, P, P]),
+    "mpm_surrogate_gradient_noalloc": (I32, [P, P, P, I64, P, P]),
     "mpm_surrogate_load_training_set": (I32, [P, P, P, I64]),
     "mpm_surrogate_train": (I32, [P, I32, I32, ctypes.c_float, ctypes.c_uint64, P]),
     "mpm_set_surrogate": (I32, [P, P, I32, DBL]),

@@ -24,6 +24,7 @@ struct mpm_surrogate {
     int adam_step = 0;
     double stress_scale = 1.0, strain_scale = 1.0;
     MLPNorm norm{};
+    float *scratch = nullptr;     /* this device's address of the library's fixed scratch array */
 
     ~mpm_surrogate();
     int ensure_partial(int64_t rows);

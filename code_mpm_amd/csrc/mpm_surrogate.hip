@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -14,11 +15,38 @@
 #include "../../include/mpm.h"
 #include "mpm_surrogate.h"
 
+/* Fixed gradient scratch of mpm_surrogate_gradient_noalloc: per 64-row chunk
+ * PARAM_COUNT gradient partials, then one squared-error sum per chunk, for
+ * up to MPM_SURROGATE_NOALLOC_MAX_ROWS rows (128 chunks, 5.35 MB).  Each GPU
+ * the library runs on holds its own copy until the process ends.  In host
+ * code the name is not a device address: kernels get the address that
+ * scratch_address() looked up for the device instead. */
+static constexpr int64_t NOALLOC_CHUNKS = (MPM_SURROGATE_NOALLOC_MAX_ROWS + 63) / 64;
+static __device__ __align__(256) float scratch[NOALLOC_CHUNKS * (PARAM_COUNT + 1)];
+
 namespace {
 template <typename T>
 void dfree(T *&p) {
     if (p) (void)hipFree(p);
     p = nullptr;
+}
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+float *scratch_ptr[MAX_DEVICES] = {};
+
+/* The device address of this device's scratch, looked up once per device
+ * with that device current (called from mpm_surrogate_create). */
+int scratch_address(int device, float **out) {
+    if (device < 0 || device >= MAX_DEVICES) return MPM_ERR_RANGE;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MPM_ERR_HIP;
+        scratch_ptr[device] = static_cast<float *>(p);
+    }
+    *out = scratch_ptr[device];
+    return MPM_OK;
 }
 }  // namespace
 
@@ -60,6 +88,11 @@ int mpm_surrogate_create(mpm_surrogate **out, int32_t device, uint64_t seed) {
     if (hipSetDevice(device) != hipSuccess) return MPM_ERR_HIP;
     mpm_surrogate *s = new mpm_surrogate();
     s->device = device;
+    int src = scratch_address(device, &s->scratch);
+    if (src) {
+        delete s;
+        return src;
+    }
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&s->params, sizeof(float) * PARAM_COUNT) != hipSuccess ||
         hipMalloc(&s->wbf16, sizeof(uint16_t) * WEIGHT_BF16_COUNT) != hipSuccess ||
@@ -173,6 +206,20 @@ int mpm_surrogate_gradient(mpm_surrogate *s, const float *x, const float *t, int
         hipMemcpy(grad, s->grad, sizeof(float) * PARAM_COUNT, hipMemcpyDeviceToHost) != hipSuccess)
         return MPM_ERR_HIP;
     if (loss && hipMemcpy(loss, s->loss, sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return MPM_ERR_HIP;
+    return MPM_OK;
+}
+
+int mpm_surrogate_gradient_noalloc(mpm_surrogate *s, const float *x, const float *t, int64_t n, float *grad,
+                                    float *loss) {
+    if (!s || n <= 0 || !x || !t || !grad) return MPM_ERR_ARG;
+    if (n > MPM_SURROGATE_NOALLOC_MAX_ROWS) return MPM_ERR_RANGE;
+    if (!s->scratch) return MPM_ERR_STATE;
+    if (hipSetDevice(s->device) != hipSuccess) return MPM_ERR_HIP;
+    /* every call writes all partials of its chunks before the reduction
+     * reads them, so nothing needs clearing between calls */
+    if (mlp_launch_gradient(s->params, s->wbf16, x, t, n, s->scratch, grad, loss, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return MPM_ERR_HIP;
     return MPM_OK;
 }
 

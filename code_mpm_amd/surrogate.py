@@ -132,6 +132,12 @@ class Surrogate:
                  "gradient")
         return g, float(loss[0])
 
+    def gradient_device(self, x_ptr, t_ptr, n, grad_ptr, loss_ptr=None):
+        """mpm_surrogate_gradient_noalloc on rows already in device memory
+        (raw device addresses); allocates nothing."""
+        self._ck(self._lib.mpm_surrogate_gradient_noalloc(self.handle, x_ptr, t_ptr, int(n), grad_ptr, loss_ptr),
+                 "gradient (no allocation)")
+
     def train(self, xs, ts, steps=2000, batch=4096, lr=1e-3, seed=0):
         """Adam on the GPU; returns the per-step batch losses."""
         rng = np.random.default_rng(seed)

@@ -428,6 +428,21 @@ int mpm_surrogate_set_normalization(mpm_surrogate *s, double stress_scale, doubl
 int mpm_surrogate_forward(mpm_surrogate *s, const float *x, float *y, int64_t n);
 /* Gradient of mean((y - t)^2) over n rows and the 7 outputs. */
 int mpm_surrogate_gradient(mpm_surrogate *s, const float *x, const float *t, int64_t n, float *grad, float *loss);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight training or serving loops,
+ * harnesses that forbid hipMalloc): mpm_surrogate_gradient on rows already
+ * in device memory, allocating nothing.  x (n x 13), t (n x 7), grad
+ * (mpm_surrogate_param_count() floats) and loss (1 float, may be NULL) are
+ * device pointers on the surrogate's device.  The per-chunk gradient
+ * partials live in a fixed library array (5.35 MB per GPU, held until the
+ * process ends), so n is limited to MPM_SURROGATE_NOALLOC_MAX_ROWS; a larger
+ * n returns MPM_ERR_RANGE.  All calls on one GPU share that array: calls
+ * that overlap on one GPU (several host threads or surrogates) race.
+ * Returns after the result is written (waits for the surrogate's stream).
+ * Equal, float for float, to mpm_surrogate_gradient on the same rows. */
+#define MPM_SURROGATE_NOALLOC_MAX_ROWS 8192
+int mpm_surrogate_gradient_noalloc(mpm_surrogate *s, const float *x, const float *t, int64_t n, float *grad,
+                                   float *loss);
 /* Copies a standardised training set to the device. */
 int mpm_surrogate_load_training_set(mpm_surrogate *s, const float *x, const float *t, int64_t n);
 /* Adam on random 64-row-aligned windows of `batch` rows; loss_history
