@@ -53,6 +53,11 @@ KERNELS = {
              "g2p_velocity": "k_g2p_velocity<false>", "remap_partials": "k_cell_partials<true, false, false, false>",
              "remap_nodes": "k_node_cells<true, false>", "g2p_kinematics": "k_g2p_fused<0, false>",
              "g2p_constitutive": None},
+    "tile": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
+             "p2g_partials": "k_tile_p2g<false, false, false>", "p2g_nodes": "k_tile_bnd<false, false>",
+             "g2p_velocity": "k_g2p_velocity<false>", "remap_partials": "k_tile_p2g<true, false, false>",
+             "remap_nodes": "k_tile_bnd<true, false>", "g2p_kinematics": "k_g2p_fused<0, false>",
+             "g2p_constitutive": None},
     "particle": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
                  "p2g_partials": "k_cell_partials<false, false, false, true>", "p2g_nodes": "k_p2g_list<false>",
                  "g2p_velocity": "k_g2p_velocity<false>",
@@ -209,7 +214,14 @@ def _run_curve(scene, surrogate, steps, every, material=0):
         m = eng.monitors()
         rows.append([m["position_y"], m["force_y"], m["strain_energy"], m["plastic_energy"], m["momentum_y"]])
     health = eng.health()
-    active = int(eng.get_node("ACTIVE").sum())
+    act_mask = eng.get_node("ACTIVE")[0] != 0
+    active = int(act_mask.sum())
+    # tile order: the share of active nodes whose 8 cells lie in one
+    # summation tile (finished by the tile kernel; the rest by the face pass)
+    nn = np.asarray(scene.cells) + 1
+    idx = np.nonzero(act_mask)[0]
+    ix, iy, iz = idx % nn[0], (idx // nn[0]) % nn[1], idx // (nn[0] * nn[1])
+    inner = float(np.mean((ix % 8 != 7) & (iy % 8 != 7) & (iz % 4 != 3))) if active else 0.0
     eng.close()
     return 1e3 * el / (every * (steps // every)), np.array(rows), health, active
 
@@ -330,8 +342,8 @@ def main():
                          "launched kernel by kernel; the other is reported beside it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--order", default="cell", choices=["cell", "particle"],
-                    help="node summation order (cell: fast, particle: the reference's one-thread order)")
+    ap.add_argument("--order", default="cell", choices=["tile", "cell", "particle"],
+                    help="node summation order (tile, cell: fast; particle: the reference's one-thread order)")
     ap.add_argument("--variant-steps", type=int, default=10,
                     help="timed steps of the contact and single-pass loops on the same state (0: skip)")
     ap.add_argument("--batch-instances", type=int, default=256,
@@ -406,7 +418,14 @@ def main():
 
     kms = eng.step_timed_kernels(scene.dt, args.profile_steps)
     avg_ms = {k: v / args.profile_steps for k, v in kms.items()}
-    active = int(eng.get_node("ACTIVE").sum())
+    act_mask = eng.get_node("ACTIVE")[0] != 0
+    active = int(act_mask.sum())
+    # tile order: the share of active nodes whose 8 cells lie in one
+    # summation tile (finished by the tile kernel; the rest by the face pass)
+    nn = np.asarray(scene.cells) + 1
+    idx = np.nonzero(act_mask)[0]
+    ix, iy, iz = idx % nn[0], (idx // nn[0]) % nn[1], idx // (nn[0] * nn[1])
+    inner = float(np.mean((ix % 8 != 7) & (iy % 8 != 7) & (iz % 4 != 3))) if active else 0.0
     health = eng.health()
     steps_total = args.warmup + args.steps + args.profile_steps
     plastic_frac = health["plastic_points"] / max(1, scene.n * steps_total)
@@ -419,7 +438,10 @@ def main():
         per_p, per_n = PER_PARTICLE[k], PER_NODE[k]
         if k in FUSED_G2P:
             per_p, per_n = FUSED_G2P[k]
-        b = per_p * scene.n + per_n * active + round(PER_PLASTIC.get(k, 0) * plastic_frac * scene.n)
+        if args.order == "tile" and k in ("p2g_partials", "remap_partials", "p2g_nodes", "remap_nodes"):
+            node_k = "p2g_nodes" if k.startswith("p2g") else "remap_nodes"
+            per_n = PER_NODE[node_k] * (inner if k.endswith("partials") else 1.0 - inner)
+        b = int(round(per_p * scene.n + per_n * active + PER_PLASTIC.get(k, 0) * plastic_frac * scene.n))
         t = [measured.get(kn) for kn in kernels[k].split("+")]
         per_kernel[k] = dict(ms=round(ms, 4), kernel=kernels[k], algorithmic_bytes=b,
                              gbs=round(b / (ms * 1e-3) / 1e9, 1) if (b and ms > 0) else None,
@@ -505,6 +527,7 @@ def main():
                                "(von Mises hardening), platen 10 m/s, seed 1",
                    "particles": scene.n, "active_nodes": active, "plastic_fraction": round(plastic_frac, 5),
                    "return_map": "exact", "summation_order": args.order,
+                   "tile_interior_node_share": round(inner, 4) if args.order == "tile" else None,
                    "node_tile": list(tile) if tile is not None else None,
                    "launch": ("one HIP graph of the K steps, replayed" if args.launch == "graph"
                               else "kernel launches on one stream")},

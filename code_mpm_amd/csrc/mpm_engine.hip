@@ -42,6 +42,7 @@ struct HostInst {
     GridGeom g{};
     int64_t node_off = 0, cell_off = 0, part_off = 0, npart = 0;
     uint32_t tile_off = 0, tiles = 0;
+    uint32_t stile_off = 0, stiles = 0; /* summation tiles of the tile order */
     double time = 0.0, dt_max = 1.0e-3, time_end = 10.0;
     double damping = 0.0, mass_min = 0.0;
     std::vector<double> tl; /* t, vx, vy, vz per point */
@@ -59,6 +60,12 @@ bool make_geom(const double L[3], const int32_t cells[3], GridGeom &g) {
     g.nnodes = (int64_t)g.nn[0] * g.nn[1] * g.nn[2];
     g.necells = (int64_t)g.ec[0] * g.ec[1] * g.ec[2];
     return true;
+}
+
+/* Summation tiles of the tile order over the instance's binning cells. */
+uint32_t stiles_of(const GridGeom &g) {
+    int64_t t = (int64_t)((g.ec[0] + ST_X - 1) / ST_X) * ((g.ec[1] + ST_Y - 1) / ST_Y) * ((g.ec[2] + ST_Z - 1) / ST_Z);
+    return (uint32_t)t;
 }
 
 uint32_t tiles_of(const GridGeom &g, const int shape[3]) {
@@ -122,6 +129,11 @@ struct mpm_engine {
     int strain = STRAIN_SMALL;
     uint8_t *dtflag = nullptr; /* node tile flags, see Batch::tflag */
     int64_t tflag_cap = 0;
+    /* tile order: summation tile offsets, occupancy flags and face records */
+    uint32_t nstiles = 0;
+    uint32_t *dstile_off = nullptr;
+    uint8_t *dsflag = nullptr;
+    double *dsrec = nullptr;
     CellParts parts{};
     int64_t parts_cap = 0;
 
@@ -177,7 +189,27 @@ struct mpm_engine {
         fixed_y.clear();
         dfree(bcount); dfree(bstart); dfree(bblock); dfree(bslot); dfree(bnocc);
         dfree(dinst); dfree(dtile_off); dfree(dtl);
+        free_stiles();
         free_parts();
+    }
+    void free_stiles() {
+        dfree(dstile_off); dfree(dsflag); dfree(dsrec);
+    }
+    /* Tile order: the face records (ST_BND per summation tile, 64 bytes
+     * each) and occupancy flags, allocated on first use. */
+    int ensure_stiles() {
+        if (dsrec) return MPM_OK;
+        int rc;
+        const size_t nt = std::max<size_t>(nstiles, 1);
+        if ((rc = hip(hipMalloc(&dsflag, nt), "alloc summation tile flags")) ||
+            (rc = hip(hipMemset(dsflag, 0, nt), "clear summation tile flags")) ||
+            (rc = hip(hipMalloc(&dsrec, nt * (size_t)ST_BND * ST_REC * sizeof(double)), "alloc tile face records"))) {
+            dfree(dsflag);
+            dfree(dsrec);
+            return rc;
+        }
+        drop_graphs();
+        return MPM_OK;
     }
     void free_parts() {
         dfree(parts.w); dfree(bocc); dfree(parts.list);
@@ -333,6 +365,10 @@ struct mpm_engine {
         b.fext = has_fext ? 1 : 0;
         b.nmat = mt.n;
         b.tflag = order == ORDER_CELL ? dtflag : nullptr;
+        b.stile_off = dstile_off;
+        b.nstiles = nstiles;
+        b.sflag = dsflag;
+        b.srec = dsrec;
         return b;
     }
 
@@ -378,12 +414,18 @@ struct mpm_engine {
         free_nodes();
         layout_set = false;
         nnodes = ncells = 0;
+        int64_t st = 0;
         for (auto &I : inst) {
             I.node_off = nnodes;
             I.cell_off = ncells;
+            I.stile_off = (uint32_t)st;
+            I.stiles = stiles_of(I.g);
             nnodes += I.g.nnodes;
             ncells += I.g.necells;
+            st += I.stiles;
         }
+        if (st >= (int64_t)0xFFFFFFFFll) return fail(MPM_ERR_RANGE, "too many summation tiles");
+        nstiles = (uint32_t)st;
         if (scan_padded_cells(ncells) >= (int64_t)0xFFFFFFFFll) return fail(MPM_ERR_RANGE, "grid too large: %lld cells", (long long)ncells);
         const size_t nn = (size_t)nnodes;
         int rc = 0;
@@ -403,13 +445,22 @@ struct mpm_engine {
             (rc = hip(hipMalloc(&bslot, 4 * (size_t)scan_padded_cells(ncells)), "alloc cell slots")) ||
             (rc = hip(hipMalloc(&bnocc, 4), "alloc occupied count")) ||
             (rc = hip(hipMalloc(&dinst, sizeof(Inst) * inst.size()), "alloc instance table")) ||
-            (rc = hip(hipMalloc(&dtile_off, 4 * (inst.size() + 1)), "alloc tile offsets"))) {
+            (rc = hip(hipMalloc(&dtile_off, 4 * (inst.size() + 1)), "alloc tile offsets")) ||
+            (rc = hip(hipMalloc(&dstile_off, 4 * (inst.size() + 1)), "alloc summation tile offsets"))) {
             free_nodes();
             return rc;
         }
         if ((rc = hip(hipMemset(nact, 0, nn), "clear active")) || (rc = hip(hipMemset(nfix, 0, nn), "clear fixed")) ||
             (rc = hip(hipMemset(nfric, 0, 8 * nn), "clear friction")) || (rc = hip(hipMemset(nm, 0, 8 * nn), "clear mass")))
             return rc;
+        {
+            std::vector<uint32_t> so(inst.size() + 1);
+            for (size_t k = 0; k < inst.size(); k++) so[k] = inst[k].stile_off;
+            so[inst.size()] = nstiles;
+            if ((rc = hip(hipMemcpy(dstile_off, so.data(), 4 * so.size(), hipMemcpyHostToDevice),
+                          "upload summation tile offsets")))
+                return rc;
+        }
         if ((rc = layout_tiles())) return rc;
         layout_set = true;
         keys_valid = false;
@@ -435,6 +486,8 @@ struct mpm_engine {
             I.npart = H.npart;
             I.tile_off = H.tile_off;
             I.tiles = H.tiles;
+            I.stile_off = H.stile_off;
+            I.stiles = H.stiles;
             I.dt = H.dt_max;
             I.damping = H.damping;
             I.mass_min = H.mass_min;
@@ -509,6 +562,7 @@ struct mpm_engine {
         if ((rc = ensure_occ())) return rc;
         if (order == ORDER_CELL && (rc = ensure_parts())) return rc;
         if (order == ORDER_PARTICLE && (rc = ensure_list())) return rc;
+        if (order == ORDER_TILE && (rc = ensure_stiles())) return rc;
         return ensure_keys();
     }
 
@@ -1209,10 +1263,11 @@ int mpm_get_variant(const mpm_engine *e, int32_t *variant) {
 int mpm_set_summation_order(mpm_engine *e, int32_t order) {
     if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
-    if (order != MPM_ORDER_PARTICLE && order != MPM_ORDER_CELL) return e->fail(MPM_ERR_ARG, "unknown order %d", order);
+    if (order != MPM_ORDER_PARTICLE && order != MPM_ORDER_CELL && order != MPM_ORDER_TILE)
+        return e->fail(MPM_ERR_ARG, "unknown order %d", order);
     (void)hipStreamSynchronize(e->stream);
     e->drop_graphs();
-    e->order = order == MPM_ORDER_CELL ? ORDER_CELL : ORDER_PARTICLE;
+    e->order = order == MPM_ORDER_CELL ? ORDER_CELL : (order == MPM_ORDER_TILE ? ORDER_TILE : ORDER_PARTICLE);
     return MPM_OK;
 }
 

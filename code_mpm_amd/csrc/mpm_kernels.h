@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/mpm.h"
 #include "mpm_device.h"
 #include "mpm_mlp.h"
 
@@ -89,7 +90,23 @@ struct BView {
 };
 
 /* Node summation order. */
-enum { ORDER_PARTICLE = 0, ORDER_CELL = 1 };
+enum { ORDER_PARTICLE = 0, ORDER_CELL = 1, ORDER_TILE = 2 };
+
+/* Tile order: binning cells (e = cell + 1 per axis) grouped in summation
+ * tiles of ST_X x ST_Y x ST_Z; a tile's node box is the (ST_X + 1) x
+ * (ST_Y + 1) x (ST_Z + 1) nodes its cells touch, of which the ST_BND on the
+ * box faces also take sums from neighbouring tiles (one record per tile and
+ * box-face node, ST_REC doubles). */
+#define ST_X MPM_SUM_TILE_X
+#define ST_Y MPM_SUM_TILE_Y
+#define ST_Z MPM_SUM_TILE_Z
+#define ST_CELLS (ST_X * ST_Y * ST_Z)
+#define ST_BX (ST_X + 1)
+#define ST_BY (ST_Y + 1)
+#define ST_BZ (ST_Z + 1)
+#define ST_BOX (ST_BX * ST_BY * ST_BZ)
+#define ST_BND (ST_BOX - (ST_X - 1) * (ST_Y - 1) * (ST_Z - 1))
+#define ST_REC 8
 /* step loops: DoublePass, DoublePass_Contact, SinglePass (same codes as MPM_VARIANT_*) */
 enum { VAR_DP = 0, VAR_DPC = 1, VAR_SP = 2 };
 /* G2P pass 2 of the DoublePass loop (same codes as MPM_STRAIN_*) */
@@ -122,6 +139,7 @@ struct Inst {
     GridGeom g;
     int64_t node_off, cell_off, part_off, npart;
     uint32_t tile_off, tiles;
+    uint32_t stile_off, stiles; /* summation tiles (tile order) */
     double dt, damping, mass_min, time;
     double vt[3];            /* TimeLine::getVelocity(time at step start) */
     int32_t tl_off, tl_count;
@@ -144,6 +162,10 @@ struct Batch {
     int strain;               /* STRAIN_SMALL / STRAIN_LARGE: G2P pass 2 of the DoublePass loop */
     int fext;                 /* any particle carries an external force */
     int nmat;                 /* materials in the table */
+    const uint32_t *stile_off; /* ninst + 1 prefix of summation tiles (tile order) */
+    uint32_t nstiles;
+    uint8_t *sflag;           /* per summation tile: 1 when it holds an occupied cell (tile order) */
+    double *srec;             /* per summation tile, ST_BND face-node records of ST_REC doubles */
     uint8_t *tflag;           /* per node tile: 1 when a corner of an occupied cell lies in it
                                  (set by the P2G partials, cleared by the remap node pass);
                                  null in the particle order */

@@ -1351,6 +1351,581 @@ __global__ __launch_bounds__(BLOCK) void k_node_cells(Batch bt, PView P, NView N
     }
 }
 
+/* ---- tile order: per summation tile, the node sums accumulated in LDS ----
+ *
+ * The binning cells are grouped in fixed summation tiles of ST_X x ST_Y x
+ * ST_Z cells.  A node adds, over the tiles that hold its 8 adjacent cells in
+ * ascending tile index, that tile's sum of those cells' partial sums (each
+ * over the cell's particles in ascending id), the cells of one tile taken in
+ * parity-class order, class = 4 (e_x & 1) + 2 (e_y & 1) + (e_z & 1).
+ *
+ * One block per tile.  Cells of one class never share a node (two cells
+ * sharing a node differ by at most one along every axis, so some axis
+ * parity differs), so the block runs the classes one after the other, each
+ * with 8 lanes per cell (one per corner) as in the cell order, and every
+ * lane adds its corner partial straight into its node's accumulator in LDS:
+ * no conflicts, and each node receives its classes in class order.  A node
+ * whose 8 cells all lie in the tile is then complete and is finished in the
+ * block (displacement control, grid update, stores); a node on the face of
+ * the tile's node box gets one record of the tile's sum, and the boundary
+ * pass (k_tile_bnd) adds the records of its 2, 4 or 8 tiles in ascending tile
+ * index and finishes it.  No partial sums leave the block except those of
+ * face nodes, and no floating-point atomics are used. */
+
+/* instance owning global summation tile t: last k with stile_off[k] <= t */
+__device__ __forceinline__ int inst_of_stile(const Batch &bt, uint32_t t) {
+    int lo = 0, hi = bt.ninst - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (bt.stile_off[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void stile_grid(const GridGeom &g, int &ntx, int &nty) {
+    ntx = (g.ec[0] + ST_X - 1) / ST_X;
+    nty = (g.ec[1] + ST_Y - 1) / ST_Y;
+}
+
+/* Index of a face node of a tile's node box (box coordinates lx < ST_BX,
+ * ly < ST_BY, lz < ST_BZ) among the ST_BND face nodes: the two z faces,
+ * then the two y faces without them, then the two x faces without both. */
+__device__ __forceinline__ int box_face_index(int lx, int ly, int lz) {
+    constexpr int ZF = ST_BX * ST_BY, YF = ST_BX * (ST_BZ - 2), XF = (ST_BY - 2) * (ST_BZ - 2);
+    if (lz == 0) return lx + ST_BX * ly;
+    if (lz == ST_BZ - 1) return ZF + lx + ST_BX * ly;
+    if (ly == 0) return 2 * ZF + lx + ST_BX * (lz - 1);
+    if (ly == ST_BY - 1) return 2 * ZF + YF + lx + ST_BX * (lz - 1);
+    if (lx == 0) return 2 * ZF + 2 * YF + (ly - 1) + (ST_BY - 2) * (lz - 1);
+    return 2 * ZF + 2 * YF + XF + (ly - 1) + (ST_BY - 2) * (lz - 1);
+}
+
+/* Particles staged per chunk, and the tile's particle ids kept on chip
+ * (tiles holding more take their ids from memory class by class). */
+#define ST_STAGE 256
+#define ST_IDS 2048
+
+/* What one staging thread carries from memory to LDS for one particle. */
+template <bool REMAP, bool FEXT>
+struct TileStage {
+    double x[3], v[3], m, V, sig[REMAP ? 1 : 6], fx[FEXT ? 3 : 1];
+    int dcr;
+};
+
+template <bool REMAP, bool MULTI, bool FEXT>
+__global__ __launch_bounds__(BLOCK) void k_tile_p2g(Batch bt, PView P, NView Nd, BView B) {
+    static_assert(ST_CELLS == BLOCK, "one thread per tile cell");
+    constexpr int NC = REMAP ? 3 : 7;               /* accumulated components */
+    constexpr int PS = ST_STAGE, PR = REMAP ? PS + 1 : PS;
+    constexpr bool SF = !REMAP && FEXT;
+    constexpr int CPC = ST_CELLS / 8;                /* cells per class */
+    static_assert(CPC * 8 == BLOCK, "8 lanes for every cell of a class");
+    static_assert(PS == BLOCK, "one staged particle per thread and chunk");
+    __shared__ double acc[NC][ST_BOX];
+    __shared__ uint32_t s_slot[ST_CELLS], s_first[ST_CELLS], s_cnt[ST_CELLS];
+    __shared__ uint8_t s_cls[8][CPC], s_rank[ST_CELLS];
+    __shared__ uint32_t s_ncls[8], s_pre[8][CPC + 1], s_base[9], s_cbase[9];
+    __shared__ uint8_t s_gcell[ST_CELLS], s_gcls[ST_CELLS];
+    __shared__ uint32_t s_gpre[ST_CELLS + 1];
+    __shared__ uint32_t s_ids[ST_IDS];
+    __shared__ double sS[REMAP ? 6 : 3][PR], spv[3][PR];
+    __shared__ double sm[REMAP ? 1 : PR], sV[REMAP ? 1 : PR], ss[REMAP ? 1 : 6][REMAP ? 1 : PR];
+    __shared__ double sf[SF ? 3 : 1][SF ? PR : 1];
+    __shared__ int8_t sdc[PS];
+    __shared__ double xs_s[REMAP ? 1 : CPC][6], xs_q[REMAP ? 1 : CPC][12];
+    __shared__ uint8_t xs_b[REMAP ? 1 : CPC][6];
+
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const Inst &I = MULTI ? bt.inst[inst_of_stile(bt, tile)] : bt.inst[0];
+    const GridGeom &g = I.g;
+    int ntx, nty;
+    stile_grid(g, ntx, nty);
+    const uint32_t lt = tile - I.stile_off;
+    const int ta = (int)(lt % (uint32_t)ntx), tb = (int)((lt / (uint32_t)ntx) % (uint32_t)nty),
+              tc = (int)(lt / ((uint32_t)ntx * (uint32_t)nty));
+    const int ex0 = ST_X * ta, ey0 = ST_Y * tb, ez0 = ST_Z * tc;
+    const int t = (int)threadIdx.x;
+    const size_t n = P.n;
+    auto cell_index = [&](int lcell) {
+        const int ex = ex0 + lcell % ST_X, ey = ey0 + (lcell / ST_X) % ST_Y, ez = ez0 + lcell / (ST_X * ST_Y);
+        return (size_t)I.cell_off + (size_t)ex + (size_t)g.ec[0] * ((size_t)ey + (size_t)g.ec[1] * ez);
+    };
+
+    /* the tile's cells, one per thread: slot, first sorted position, count */
+    {
+        const int cx = t % ST_X, cy = (t / ST_X) % ST_Y, cz = t / (ST_X * ST_Y);
+        uint32_t sl = MPM_NONE, first = 0, cnt = 0;
+        if (ex0 + cx < g.ec[0] && ey0 + cy < g.ec[1] && ez0 + cz < g.ec[2]) {
+            const size_t c = cell_index(t);
+            sl = B.slot[c];
+            first = B.start[c];
+            cnt = B.start[c + 1] - first;
+        }
+        s_slot[t] = sl;
+        s_first[t] = first;
+        s_cnt[t] = sl != MPM_NONE ? cnt : 0u;
+        for (int j = t; j < NC * ST_BOX; j += BLOCK) (&acc[0][0])[j] = 0.0;
+    }
+    __syncthreads();
+    /* class lists: wave w builds classes 2w and 2w + 1, one half-wave each;
+     * cell q of a class is (2 i + px, 2 j + py, 2 k + pz), q = i + 4 j + 16 k,
+     * kept in ascending q (any order would do: a class's cells share no node) */
+    {
+        const int lane = t & 63, half = lane >> 5, q = lane & 31;
+        const int cls = 2 * (t >> 6) + half;
+        const int px = cls >> 2, py = (cls >> 1) & 1, pz = cls & 1;
+        const int cx = 2 * (q & 3) + px, cy = 2 * ((q >> 2) & 3) + py, cz = 2 * (q >> 4) + pz;
+        const int lcell = cx + ST_X * (cy + ST_Y * cz);
+        const bool occ = s_slot[lcell] != MPM_NONE;
+        const uint32_t cnt = s_cnt[lcell];
+        const uint64_t mask = __ballot(occ);
+        const uint32_t hm = (uint32_t)(half ? (mask >> 32) : (mask & 0xFFFFFFFFull));
+        const uint32_t below = hm & ((1u << q) - 1u);
+        const int rank = __popc(below);
+        /* inclusive scan of the counts over the half-wave, in q order */
+        uint32_t x = cnt;
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 32);
+            if (q >= o) x += y;
+        }
+        if (occ) {
+            s_cls[cls][rank] = (uint8_t)lcell;
+            s_rank[lcell] = (uint8_t)rank;
+            s_pre[cls][rank + 1] = x; /* unoccupied cells add 0, so x is the prefix up to this cell */
+        }
+        if (q == 0) {
+            s_pre[cls][0] = 0;
+            s_ncls[cls] = (uint32_t)__popc(hm);
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        uint32_t b = 0;
+        for (int c = 0; c < 8; c++) {
+            s_base[c] = b;
+            b += s_pre[c][s_ncls[c]];
+        }
+        s_base[8] = b;
+    }
+    __syncthreads();
+    const uint32_t total_all = s_base[8];
+    if (!REMAP && t == 0) bt.sflag[tile] = total_all ? 1 : 0;
+    if (total_all == 0) {
+        /* uniform; the nodes whose cells all lie in this tile were touched
+         * by no particle (the face nodes are left to k_tile_bnd) */
+        if (!REMAP)
+            for (int j = t; j < ST_BOX; j += BLOCK) {
+                const int lx = j % ST_BX, ly = (j / ST_BX) % ST_BY, lz = j / (ST_BX * ST_BY);
+                const int ix = ex0 - 1 + lx, iy = ey0 - 1 + ly, iz = ez0 - 1 + lz;
+                const bool inner = lx >= 1 && lx <= ST_X - 1 && ly >= 1 && ly <= ST_Y - 1 && lz >= 1 && lz <= ST_Z - 1;
+                if (inner && ix < g.nn[0] && iy < g.nn[1] && iz < g.nn[2])
+                    Nd.active[I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz)] = 0;
+            }
+        return;
+    }
+    /* the tile's occupied cells as one list, class by class (cells of a
+     * class in list order): cell g, its first particle in the list, its
+     * class; and the tile's particle ids in the same order */
+    if (t == 0) {
+        uint32_t b = 0;
+        for (int c = 0; c < 8; c++) {
+            s_cbase[c] = b;
+            b += s_ncls[c];
+        }
+        s_cbase[8] = b;
+        s_gpre[b] = total_all;
+    }
+    __syncthreads();
+    const int ncell = (int)s_cbase[8];
+    const bool ids_on_chip = total_all <= ST_IDS;
+    if (s_slot[t] != MPM_NONE) {
+        const int cx = t % ST_X, cy = (t / ST_X) % ST_Y, cz = t / (ST_X * ST_Y);
+        const int cls = 4 * (cx & 1) + 2 * (cy & 1) + (cz & 1);
+        const int rank = s_rank[t];
+        const int gi = (int)s_cbase[cls] + rank;
+        const uint32_t dst = s_base[cls] + s_pre[cls][rank];
+        s_gcell[gi] = (uint8_t)t;
+        s_gcls[gi] = (uint8_t)cls;
+        s_gpre[gi] = dst;
+        if (ids_on_chip) {
+            const uint32_t first = s_first[t], cnt = s_cnt[t];
+            for (uint32_t i0 = 0; i0 < cnt; i0 += 8) {
+                uint32_t r[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++) r[i] = i0 + i < cnt ? B.sorted[first + i0 + i] : 0u;
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    if (i0 + i < cnt) s_ids[dst + i0 + i] = r[i];
+            }
+        }
+    }
+    __syncthreads();
+
+    const int lc = t >> 3, corner = t & 7;
+    const int ox = corner >> 2, oy = (corner >> 1) & 1, oz = corner & 1;
+    const int pa = corner < 6 ? corner >> 1 : 0, po = corner & 1;
+    const int qt1 = (int)((0xE568u >> (2 * corner)) & 3u), qt2 = (int)((0xFE01u >> (2 * corner)) & 3u);
+    auto qindex = [&](int qq) { return qq == 0 ? 2 * oy + oz : (qq == 1 ? 4 + 2 * ox + oz : 8 + 2 * ox + oy); };
+    const int qk1 = qindex(qt1), qk2 = qindex(qt2);
+    const bool skip_dc = REMAP ? bt.variant != VAR_DP : bt.variant == VAR_SP;
+    const double hx_ = g.h[0], hy_ = g.h[1], hz_ = g.h[2];
+    const double hpair = pa == 0 ? hx_ : (pa == 1 ? hy_ : hz_);
+
+    /* the id at list position q */
+    auto id_of = [&](uint32_t q) -> uint32_t {
+        if (ids_on_chip) return s_ids[q];
+        int lo = 0, hi = ncell - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_gpre[mid] <= q)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        return B.sorted[s_first[s_gcell[lo]] + (q - s_gpre[lo])];
+    };
+    /* one particle's fields, loaded into registers ahead of their chunk */
+    TileStage<REMAP, SF> pf;
+    auto load = [&](uint32_t q) {
+        const uint32_t id = id_of(q);
+        pf.x[0] = P.x[id];
+        pf.x[1] = P.x[n + id];
+        pf.x[2] = P.x[2 * n + id];
+        pf.v[0] = P.v[id];
+        pf.v[1] = P.v[n + id];
+        pf.v[2] = P.v[2 * n + id];
+        pf.m = P.m[id];
+        if (!REMAP) {
+            pf.V = P.V[id];
+#pragma unroll
+            for (int k = 0; k < (REMAP ? 1 : 6); k++) pf.sig[k] = P.sig[k * n + id];
+        }
+        if (SF) {
+            pf.fx[0] = P.fext[id];
+            pf.fx[1] = P.fext[n + id];
+            pf.fx[2] = P.fext[2 * n + id];
+        }
+        pf.dcr = skip_dc ? P.dc_rank[id] : -1;
+    };
+    /* Rounds of CPC consecutive cells of the list (8 lanes per cell, as in
+     * the cell order), their particles staged in chunks of PS; the chunk
+     * after (round r0, first particle base) */
+    auto round_end = [&](int r0) { return s_gpre[min(r0 + CPC, ncell)]; };
+    auto next = [&](int &r0, uint32_t &base) {
+        base += PS;
+        if (base >= round_end(r0)) {
+            r0 += CPC;
+            base = r0 < ncell ? s_gpre[r0] : 0u;
+        }
+    };
+    int cr0 = 0;
+    uint32_t cbase = 0;
+    if ((uint32_t)t < round_end(0)) load((uint32_t)t);
+    double m = 0.0, p[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0};
+    while (cr0 < ncell) {
+        const int r0 = cr0;
+        const uint32_t base = cbase;
+        const int g1 = min(r0 + CPC, ncell);
+        const uint32_t p0 = s_gpre[r0], top_round = s_gpre[g1];
+        const uint32_t top = min(base + (uint32_t)PS, top_round);
+        __syncthreads(); /* staging free */
+        if (base + (uint32_t)t < top) {
+            const uint32_t j = (uint32_t)t;
+            if (!REMAP) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) sS[a][j] = pf.x[a];
+                sm[j] = pf.m;
+                sV[j] = pf.V;
+#pragma unroll
+                for (int k = 0; k < (REMAP ? 1 : 6); k++) ss[k][j] = pf.sig[k];
+                if (SF) {
+                    sf[0][j] = pf.fx[0];
+                    sf[1][j] = pf.fx[1];
+                    sf[2][j] = pf.fx[2];
+                }
+            } else {
+                double sh[3][2];
+                particle_factors(g, pf.x, sh);
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int o = 0; o < 2; o++) sS[2 * a + o][j] = sh[a][o];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) spv[k][j] = pf.m * pf.v[k];
+            sdc[j] = (skip_dc && pf.dcr >= 0) ? 1 : 0;
+        }
+        __syncthreads();
+        /* the next chunk's loads, in flight while this one is summed */
+        next(cr0, cbase);
+        if (cr0 < ncell && cbase + (uint32_t)t < round_end(cr0)) load(cbase + (uint32_t)t);
+        /* this lane's (cell, corner) in this round */
+        if (base == p0) {
+            m = 0.0;
+            p[0] = p[1] = p[2] = f[0] = f[1] = f[2] = 0.0;
+        }
+        const int gi = r0 + lc;
+        const bool cell = gi < g1;
+        bool have = cell, massive = false;
+        int box = 0, mycls = -1;
+        double mn = 0.0, xpair = 0.0;
+        uint32_t v0 = 0, v1 = 0;
+        if (cell) {
+            const int lcell = s_gcell[gi];
+            mycls = s_gcls[gi];
+            const int cx = lcell % ST_X, cy = (lcell / ST_X) % ST_Y, cz = lcell / (ST_X * ST_Y);
+            const int ix = ex0 + cx - 1 + ox, iy = ey0 + cy - 1 + oy, iz = ez0 + cz - 1 + oz;
+            box = (cx + ox) + ST_BX * ((cy + oy) + ST_BY * (cz + oz));
+            v0 = s_gpre[gi];
+            v1 = s_gpre[gi + 1];
+            if (!REMAP) {
+                const int ep = pa == 0 ? ex0 + cx : (pa == 1 ? ey0 + cy : ez0 + cz);
+                const double Lp = pa == 0 ? g.L[0] : (pa == 1 ? g.L[1] : g.L[2]);
+                const int cp = pa == 0 ? g.cells[0] : (pa == 1 ? g.cells[1] : g.cells[2]);
+                xpair = node_coord(ep - 1 + po, Lp, cp);
+            }
+            if (ix < 0 || iy < 0 || iz < 0 || ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) have = false;
+            if (REMAP && have) {
+                mn = Nd.m[I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz)];
+                massive = mn > I.mass_min;
+            }
+        }
+        if (REMAP ? (have && massive) : cell) {
+            const uint32_t a = max(v0, base), b = min(v1, top);
+            if (!REMAP) {
+                for (uint32_t q = a; q < b; q++) {
+                    const uint32_t j = q - base;
+                    if (sdc[j]) continue; /* SP:116-117, 157-158 */
+                    if (corner < 6) {
+                        const AxisShape A = axis_shape(sS[pa][j], xpair, hpair);
+                        xs_s[lc][corner] = A.s;
+                        xs_b[lc][corner] = (uint8_t)((A.grad ? 1u : 0u) | (A.sgn > 0.0 ? 2u : 0u));
+                    }
+                    const double sx = xs_s[lc][0 + ox], sy = xs_s[lc][2 + oy], sz = xs_s[lc][4 + oz];
+                    const uint32_t bx = xs_b[lc][0 + ox], by = xs_b[lc][2 + oy], bz = xs_b[lc][4 + oz];
+                    {
+                        const double pyz = sy * sz, pxz = sx * sz, pxy = sx * sy;
+                        const double n1 = qt1 == 0 ? pyz : (qt1 == 1 ? pxz : pxy), d1 = qt1 == 0 ? hx_ : (qt1 == 1 ? hy_ : hz_);
+                        const double n2 = qt2 == 0 ? pyz : (qt2 == 1 ? pxz : pxy), d2 = qt2 == 0 ? hx_ : (qt2 == 1 ? hy_ : hz_);
+                        if (qt1 != 3) xs_q[lc][qk1] = n1 / d1;
+                        if (qt2 != 3) xs_q[lc][qk2] = n2 / d2;
+                    }
+                    const double qx = xs_q[lc][0 + 2 * oy + oz], qy = xs_q[lc][4 + 2 * ox + oz],
+                                 qz = xs_q[lc][8 + 2 * ox + oy];
+                    const double gx = (bx & 1u) ? ((bx & 2u) ? -qx : qx) : 0.0;
+                    const double gy = (by & 1u) ? ((by & 2u) ? -qy : qy) : 0.0;
+                    const double gz = (bz & 1u) ? ((bz & 2u) ? -qz : qz) : 0.0;
+                    const double N = sx * sy * sz;
+                    m += N * sm[j];
+                    p[0] += N * spv[0][j];
+                    p[1] += N * spv[1][j];
+                    p[2] += N * spv[2][j];
+                    const double V = sV[j];
+                    f[0] += -V * (gx * ss[0][j] + gy * ss[3][j] + gz * ss[5][j]);
+                    f[1] += -V * (gy * ss[1][j] + gx * ss[3][j] + gz * ss[4][j]);
+                    f[2] += -V * (gz * ss[2][j] + gx * ss[5][j] + gy * ss[4][j]);
+                    if (SF) {
+                        f[0] += N * sf[0][j];
+                        f[1] += N * sf[1][j];
+                        f[2] += N * sf[2][j];
+                    }
+                }
+            } else {
+                for (uint32_t q = a; q < b; q++) {
+                    const uint32_t j = q - base;
+                    if (sdc[j]) continue; /* DPC:358-359, SP:157-158 */
+                    const double N = sS[0 + ox][j] * sS[2 + oy][j] * sS[4 + oz][j];
+                    f[0] += N * spv[0][j] / mn;
+                    f[1] += N * spv[1][j] / mn;
+                    f[2] += N * spv[2][j] / mn;
+                }
+            }
+        }
+        /* the round's last chunk: the corner partials into their nodes'
+         * accumulators, class by class (cells of one class share no node;
+         * a class a round holds only in part adds the rest in the next
+         * round, before any later class) */
+        if (top == top_round) {
+            const int c0 = s_gcls[r0], c1 = s_gcls[g1 - 1];
+            const bool add = REMAP ? (have && massive) : have;
+            for (int c = c0; c <= c1; c++) {
+                if (c > c0) __syncthreads();
+                if (add && mycls == c) {
+                    if (!REMAP) {
+                        acc[0][box] += m;
+                        acc[1][box] += p[0];
+                        acc[2][box] += p[1];
+                        acc[3][box] += p[2];
+                        acc[4][box] += f[0];
+                        acc[5][box] += f[1];
+                        acc[6][box] += f[2];
+                    } else {
+                        acc[0][box] += f[0];
+                        acc[1][box] += f[1];
+                        acc[2][box] += f[2];
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    /* finish the nodes whose 8 cells lie in the tile; record the face nodes */
+    const bool all_axes = bt.variant != VAR_DP;
+    for (int j = t; j < ST_BOX; j += BLOCK) {
+        const int lx = j % ST_BX, ly = (j / ST_BX) % ST_BY, lz = j / (ST_BX * ST_BY);
+        const int ix = ex0 - 1 + lx, iy = ey0 - 1 + ly, iz = ez0 - 1 + lz;
+        if (ix < 0 || iy < 0 || iz < 0 || ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) continue;
+        const int64_t nd = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
+        const bool inner = lx >= 1 && lx <= ST_X - 1 && ly >= 1 && ly <= ST_Y - 1 && lz >= 1 && lz <= ST_Z - 1;
+        /* the node's cells that lie in this tile: local (lx - 1 + dx, ...) */
+        bool any = false, dc = false;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int cx = lx - 1 + (k >> 2), cy = ly - 1 + ((k >> 1) & 1), cz = lz - 1 + (k & 1);
+            if (cx < 0 || cy < 0 || cz < 0 || cx >= ST_X || cy >= ST_Y || cz >= ST_Z) continue;
+            const uint32_t sl = s_slot[cx + ST_X * (cy + ST_Y * cz)];
+            if (sl != MPM_NONE) {
+                any = true;
+                dc |= (sl & SLOT_DC) != 0;
+            }
+        }
+        if (!inner) {
+            double *r = bt.srec + ((size_t)tile * ST_BND + (size_t)box_face_index(lx, ly, lz)) * ST_REC;
+            double2 *o = reinterpret_cast<double2 *>(r);
+            if (!REMAP) {
+                o[0] = make_double2(acc[0][j], acc[1][j]);
+                o[1] = make_double2(acc[2][j], acc[3][j]);
+                o[2] = make_double2(acc[4][j], acc[5][j]);
+                o[3] = make_double2(acc[6][j], any ? 1.0 : 0.0);
+            } else {
+                o[0] = make_double2(acc[0][j], acc[1][j]);
+                o[1] = make_double2(acc[2][j], 0.0);
+            }
+            continue;
+        }
+        if (!REMAP) {
+            if (!any) {
+                Nd.active[nd] = 0;
+                continue;
+            }
+        } else if (!Nd.active[nd]) {
+            continue;
+        }
+        const NodeIn in = node_in<REMAP>(bt, I, Nd, nd);
+        DCWin w;
+        w.rank[0] = w.rank[1] = w.rank[2] = -1;
+        w.v[0] = w.v[1] = w.v[2] = 0.0;
+        if (dc) {
+#pragma unroll 1
+            for (int k = 0; k < 8; k++) {
+                const int cx = lx - 1 + (k >> 2), cy = ly - 1 + ((k >> 1) & 1), cz = lz - 1 + (k & 1);
+                const uint32_t sl = s_slot[cx + ST_X * (cy + ST_Y * cz)];
+                if (sl != MPM_NONE && (sl & SLOT_DC)) {
+                    const size_t c = (size_t)I.cell_off + (size_t)(ex0 + cx) +
+                                     (size_t)g.ec[0] * ((size_t)(ey0 + cy) + (size_t)g.ec[1] * (ez0 + cz));
+                    dc_scan_cell(w, P, B, (uint32_t)c, all_axes);
+                }
+            }
+        }
+        if (!REMAP) {
+            double mm = acc[0][j], pp[3] = {acc[1][j], acc[2][j], acc[3][j]}, ff[3] = {acc[4][j], acc[5][j], acc[6][j]};
+            node_p2g_finish(bt, in, Nd, nd, mm, pp, ff, w);
+        } else {
+            double vv[3] = {acc[0][j], acc[1][j], acc[2][j]};
+            node_remap_dispatch(bt, in, Nd, nd, vv, w);
+        }
+    }
+}
+
+/* Face nodes of the tile order: each tile finishes the face nodes it owns
+ * (the nodes ix in [ST_X a, ST_X a + ST_X) etc. on its upper faces, the
+ * other nodes of that range being complete in k_tile_p2g): the records of
+ * the tiles holding the node's cells, in ascending tile index, added from
+ * 0, then the same node finish as the cell order. */
+template <bool REMAP, bool MULTI>
+__global__ __launch_bounds__(BLOCK) void k_tile_bnd(Batch bt, PView P, NView Nd, BView B) {
+    static_assert(ST_CELLS == BLOCK, "one thread per owned node");
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const Inst &I = MULTI ? bt.inst[inst_of_stile(bt, tile)] : bt.inst[0];
+    const GridGeom &g = I.g;
+    int ntx, nty;
+    stile_grid(g, ntx, nty);
+    const uint32_t lt = tile - I.stile_off;
+    const int ta = (int)(lt % (uint32_t)ntx), tb = (int)((lt / (uint32_t)ntx) % (uint32_t)nty),
+              tc = (int)(lt / ((uint32_t)ntx * (uint32_t)nty));
+    const int t = (int)threadIdx.x;
+    const int ox = t % ST_X, oy = (t / ST_X) % ST_Y, oz = t / (ST_X * ST_Y);
+    const bool fx = ox == ST_X - 1, fy = oy == ST_Y - 1, fz = oz == ST_Z - 1;
+    if (!(fx || fy || fz)) return;
+    const int ix = ST_X * ta + ox, iy = ST_Y * tb + oy, iz = ST_Z * tc + oz;
+    if (ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) return;
+    const int64_t nd = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
+    /* source tiles in ascending index: d = dx + 2 dy + 4 dz */
+    double s[REMAP ? 3 : 7];
+#pragma unroll
+    for (int k = 0; k < (REMAP ? 3 : 7); k++) s[k] = 0.0;
+    bool any = false;
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+        const int dx = d & 1, dy = (d >> 1) & 1, dz = d >> 2;
+        if ((dx && !fx) || (dy && !fy) || (dz && !fz)) continue;
+        const uint32_t src = tile + (uint32_t)dx + (uint32_t)ntx * ((uint32_t)dy + (uint32_t)nty * (uint32_t)dz);
+        if (!bt.sflag[src]) continue;
+        const int lx = ix - (ST_X * (ta + dx) - 1), ly = iy - (ST_Y * (tb + dy) - 1), lz = iz - (ST_Z * (tc + dz) - 1);
+        const double2 *r = reinterpret_cast<const double2 *>(
+            bt.srec + ((size_t)src * ST_BND + (size_t)box_face_index(lx, ly, lz)) * ST_REC);
+        if (!REMAP) {
+            const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+            s[0] += r0.x;
+            s[1] += r0.y;
+            s[2] += r1.x;
+            s[3] += r1.y;
+            s[4] += r2.x;
+            s[5] += r2.y;
+            s[6] += r3.x;
+            any |= r3.y != 0.0;
+        } else {
+            const double2 r0 = r[0], r1 = r[1];
+            s[0] += r0.x;
+            s[1] += r0.y;
+            s[2] += r1.x;
+        }
+    }
+    if (!REMAP) {
+        if (!any) {
+            Nd.active[nd] = 0;
+            return;
+        }
+    } else if (!Nd.active[nd]) {
+        return;
+    }
+    const NodeIn in = node_in<REMAP>(bt, I, Nd, nd);
+    DCWin w;
+    w.rank[0] = w.rank[1] = w.rank[2] = -1;
+    w.v[0] = w.v[1] = w.v[2] = 0.0;
+    const bool all_axes = bt.variant != VAR_DP;
+    const uint32_t cell_off = (uint32_t)I.cell_off, ecx = (uint32_t)g.ec[0], ecy = (uint32_t)g.ec[1];
+#pragma unroll 1
+    for (int k = 0; k < 8; k++) {
+        const uint32_t c = cell_off + (uint32_t)(ix + (k >> 2)) +
+                           ecx * ((uint32_t)(iy + ((k >> 1) & 1)) + ecy * (uint32_t)(iz + (k & 1)));
+        const uint32_t sl = B.slot[c];
+        if (sl != MPM_NONE && (sl & SLOT_DC)) dc_scan_cell(w, P, B, c, all_axes);
+    }
+    if (!REMAP) {
+        double pp[3] = {s[1], s[2], s[3]}, ff[3] = {s[4], s[5], s[6]};
+        node_p2g_finish(bt, in, Nd, nd, s[0], pp, ff, w);
+    } else {
+        double vv[3] = {s[0], s[1], s[2]};
+        node_remap_dispatch(bt, in, Nd, nd, vv, w);
+    }
+}
+
 __device__ __forceinline__ void block_count_add(unsigned long long *health, int slot, uint32_t flag) {
     /* wave ballot, then one atomic per wave, into this block's copy */
     uint64_t b = __ballot(flag != 0);
@@ -2472,7 +3047,15 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
             if (mid) (void)hipEventRecord(mid, s);
             break;
         }
-        if (order == ORDER_CELL) {
+        if (order == ORDER_TILE) {
+            if (!bt.nstiles || !bt.srec || !bt.sflag) return hipErrorInvalidValue;
+            if (bt.fext)
+                LAUNCH_MULTI(bt, (k_tile_p2g<false, M, true>), dim3(bt.nstiles), dim3(BLOCK), 0, s, bt, P, N, B);
+            else
+                LAUNCH_MULTI(bt, (k_tile_p2g<false, M, false>), dim3(bt.nstiles), dim3(BLOCK), 0, s, bt, P, N, B);
+            if (mid) (void)hipEventRecord(mid, s);
+            LAUNCH_MULTI(bt, (k_tile_bnd<false, M>), dim3(bt.nstiles), dim3(BLOCK), 0, s, bt, P, N, B);
+        } else if (order == ORDER_CELL) {
             if (P.n > 0) {
                 if (bt.fext)
                     LAUNCH_MULTI(bt, (k_cell_partials<false, M, true>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s,
@@ -2511,7 +3094,12 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
             if (mid) (void)hipEventRecord(mid, s);
             break;
         }
-        if (order == ORDER_CELL) {
+        if (order == ORDER_TILE) {
+            if (!bt.nstiles || !bt.srec || !bt.sflag) return hipErrorInvalidValue;
+            LAUNCH_MULTI(bt, (k_tile_p2g<true, M, false>), dim3(bt.nstiles), dim3(BLOCK), 0, s, bt, P, N, B);
+            if (mid) (void)hipEventRecord(mid, s);
+            LAUNCH_MULTI(bt, (k_tile_bnd<true, M>), dim3(bt.nstiles), dim3(BLOCK), 0, s, bt, P, N, B);
+        } else if (order == ORDER_CELL) {
             if (P.n > 0) {
                 LAUNCH_MULTI(bt, (k_cell_partials<true, M>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s, bt, P, N,
                              B, C);

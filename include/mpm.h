@@ -213,7 +213,7 @@ int mpm_get_instance_monitors(mpm_engine *e, int32_t instance, mpm_monitors *out
 
 /* --- node summation order -------------------------------------------------- */
 
-/* Both orders are deterministic and use no floating-point atomics.
+/* All three orders are deterministic and use no floating-point atomics.
  * MPM_ORDER_PARTICLE: each node adds its particles' contributions in
  *   ascending particle index, the order of the reference's scatters run by
  *   one thread (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:48-89,
@@ -224,8 +224,21 @@ int mpm_get_instance_monitors(mpm_engine *e, int32_t instance, mpm_monitors *out
  *   fixed neighbour order, each partial taken over the cell's particles in
  *   ascending index; the same terms in another association, so results
  *   differ from the particle order only by rounding, and much faster on the
- *   GPU (neighbouring lanes read the same particles). */
-enum { MPM_ORDER_PARTICLE = 0, MPM_ORDER_CELL = 1 };
+ *   GPU (neighbouring lanes read the same particles).
+ * MPM_ORDER_TILE: the binning cells (cell index + 1 per axis, cells -1 .. n)
+ *   are grouped in fixed summation tiles of MPM_SUM_TILE_X x _Y x _Z; each
+ *   node adds, over the tiles that hold its 8 neighbour cells in ascending
+ *   tile index (x fastest), the tile's sum of those cells' per-cell partial
+ *   sums taken in parity-class order, class 4 (e_x & 1) + 2 (e_y & 1) +
+ *   (e_z & 1), each partial over the cell's particles in ascending index.
+ *   The same terms in another association again; on the GPU a tile's node
+ *   sums are accumulated in on-chip memory and only the sums of nodes on a
+ *   tile face pass through device memory.  Steps in this order hold a
+ *   device buffer of 16.1 KB per summation tile. */
+enum { MPM_ORDER_PARTICLE = 0, MPM_ORDER_CELL = 1, MPM_ORDER_TILE = 2 };
+#define MPM_SUM_TILE_X 8
+#define MPM_SUM_TILE_Y 8
+#define MPM_SUM_TILE_Z 4
 int mpm_set_summation_order(mpm_engine *e, int32_t order);
 
 /* --- timestep variant ------------------------------------------------------- */

@@ -32,8 +32,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
-#include <vector>
 #include <string>
+#include <utility>
+#include <vector>
 #include <omp.h>
 
 #include "../include/mpm.h"
@@ -536,38 +537,85 @@ struct oracle_engine {
         return nullptr;
     }
 
+    /* The node's neighbour cells in the order the node adds their partial
+     * sums, and where that order closes a group (the node adds each group's
+     * sum, itself summed from 0 in order).  Cell order: k = 0..7 (x outer,
+     * z inner; binning cell = node + k per axis), each cell its own group.
+     * Tile order: the cells grouped by summation tile (MPM_SUM_TILE_X x Y x Z
+     * binning cells) in ascending tile index, and within a tile in parity
+     * class order, class = 4 (e_x & 1) + 2 (e_y & 1) + (e_z & 1). */
+    void neighbour_sequence(long ni, long cell[8], bool last_of_group[8]) const {
+        int ix = (int)(ni % nn[0]), iy = (int)((ni / nn[0]) % nn[1]), iz = (int)(ni / ((long)nn[0] * nn[1]));
+        long key[8];
+        for (int k = 0; k < 8; k++) {
+            int e[3] = {ix + (k >> 2), iy + ((k >> 1) & 1), iz + (k & 1)};
+            cell[k] = e[0] + ec[0] * (e[1] + (long)ec[1] * e[2]);
+            if (order == MPM_ORDER_TILE) {
+                int ntx = (ec[0] + MPM_SUM_TILE_X - 1) / MPM_SUM_TILE_X;
+                int nty = (ec[1] + MPM_SUM_TILE_Y - 1) / MPM_SUM_TILE_Y;
+                long tile = e[0] / MPM_SUM_TILE_X +
+                            ntx * ((long)(e[1] / MPM_SUM_TILE_Y) + (long)nty * (e[2] / MPM_SUM_TILE_Z));
+                int cls = 4 * (e[0] & 1) + 2 * (e[1] & 1) + (e[2] & 1);
+                key[k] = tile * 8 + cls;
+            } else {
+                key[k] = k;
+            }
+        }
+        /* the 8 classes are distinct, so the keys are too */
+        for (int a = 1; a < 8; a++)
+            for (int b = a; b > 0 && key[b - 1] > key[b]; b--) {
+                std::swap(key[b - 1], key[b]);
+                std::swap(cell[b - 1], cell[b]);
+            }
+        for (int k = 0; k < 8; k++)
+            last_of_group[k] = order != MPM_ORDER_TILE || k == 7 || key[k] / 8 != key[k + 1] / 8;
+    }
+
     void m2g_cells(bool skip_dc = false) {
 #pragma omp for
         for (long ni = 0; ni < (long)nodes.size(); ni++) {
             Node *n = nodes[ni];
             if (!n->active) continue;
-            int ix = (int)(ni % nn[0]), iy = (int)((ni / nn[0]) % nn[1]), iz = (int)(ni / ((long)nn[0] * nn[1]));
+            long seq[8];
+            bool close[8];
+            neighbour_sequence(ni, seq, close);
+            double gm = 0.0;
+            V3 gp, gf;
             for (int k = 0; k < 8; k++) {
-                long c = (ix + (k >> 2)) + ec[0] * ((iy + ((k >> 1) & 1)) + (long)ec[1] * (iz + (k & 1)));
-                if (bin_start[c] == bin_start[c + 1]) continue;
-                double sm = 0.0;
-                V3 sp, sf;
-                for (uint32_t q = bin_start[c]; q < bin_start[c + 1]; q++) {
-                    const Particle *p = particles[bin_ids[q]];
-                    if (skip_dc && p->dc) continue;
-                    const AGP *a = agp_for(p, (unsigned)ni);
-                    double N = a->N;
-                    V3 g = a->g;
-                    sm += N * p->m;
-                    sp.x += N * (p->m * p->v.x);
-                    sp.y += N * (p->m * p->v.y);
-                    sp.z += N * (p->m * p->v.z);
-                    double V = p->V;
-                    sf.x += -V * (g.x * p->sig[0] + g.y * p->sig[3] + g.z * p->sig[5]);
-                    sf.y += -V * (g.y * p->sig[1] + g.x * p->sig[3] + g.z * p->sig[4]);
-                    sf.z += -V * (g.z * p->sig[2] + g.x * p->sig[5] + g.y * p->sig[4]);
-                    sf.x += N * p->fext.x;
-                    sf.y += N * p->fext.y;
-                    sf.z += N * p->fext.z;
+                long c = seq[k];
+                if (bin_start[c] != bin_start[c + 1]) {
+                    double sm = 0.0;
+                    V3 sp, sf;
+                    for (uint32_t q = bin_start[c]; q < bin_start[c + 1]; q++) {
+                        const Particle *p = particles[bin_ids[q]];
+                        if (skip_dc && p->dc) continue;
+                        const AGP *a = agp_for(p, (unsigned)ni);
+                        double N = a->N;
+                        V3 g = a->g;
+                        sm += N * p->m;
+                        sp.x += N * (p->m * p->v.x);
+                        sp.y += N * (p->m * p->v.y);
+                        sp.z += N * (p->m * p->v.z);
+                        double V = p->V;
+                        sf.x += -V * (g.x * p->sig[0] + g.y * p->sig[3] + g.z * p->sig[5]);
+                        sf.y += -V * (g.y * p->sig[1] + g.x * p->sig[3] + g.z * p->sig[4]);
+                        sf.z += -V * (g.z * p->sig[2] + g.x * p->sig[5] + g.y * p->sig[4]);
+                        sf.x += N * p->fext.x;
+                        sf.y += N * p->fext.y;
+                        sf.z += N * p->fext.z;
+                    }
+                    gm += sm;
+                    gp.x += sp.x; gp.y += sp.y; gp.z += sp.z;
+                    gf.x += sf.x; gf.y += sf.y; gf.z += sf.z;
                 }
-                n->m += sm;
-                n->p.x += sp.x; n->p.y += sp.y; n->p.z += sp.z;
-                n->f.x += sf.x; n->f.y += sf.y; n->f.z += sf.z;
+                if (close[k]) {
+                    n->m += gm;
+                    n->p.x += gp.x; n->p.y += gp.y; n->p.z += gp.z;
+                    n->f.x += gf.x; n->f.y += gf.y; n->f.z += gf.z;
+                    gm = 0.0;
+                    gp = V3();
+                    gf = V3();
+                }
             }
         }
     }
@@ -577,20 +625,28 @@ struct oracle_engine {
         for (long ni = 0; ni < (long)nodes.size(); ni++) {
             Node *n = nodes[ni];
             if (!n->active || !(n->m > mass_min)) continue;
-            int ix = (int)(ni % nn[0]), iy = (int)((ni / nn[0]) % nn[1]), iz = (int)(ni / ((long)nn[0] * nn[1]));
+            long seq[8];
+            bool close[8];
+            neighbour_sequence(ni, seq, close);
+            V3 gv;
             for (int k = 0; k < 8; k++) {
-                long c = (ix + (k >> 2)) + ec[0] * ((iy + ((k >> 1) & 1)) + (long)ec[1] * (iz + (k & 1)));
-                if (bin_start[c] == bin_start[c + 1]) continue;
-                V3 sv;
-                for (uint32_t q = bin_start[c]; q < bin_start[c + 1]; q++) {
-                    const Particle *p = particles[bin_ids[q]];
-                    if (skip_dc && p->dc) continue;
-                    double N = agp_for(p, (unsigned)ni)->N;
-                    sv.x += N * (p->m * p->v.x) / n->m;
-                    sv.y += N * (p->m * p->v.y) / n->m;
-                    sv.z += N * (p->m * p->v.z) / n->m;
+                long c = seq[k];
+                if (bin_start[c] != bin_start[c + 1]) {
+                    V3 sv;
+                    for (uint32_t q = bin_start[c]; q < bin_start[c + 1]; q++) {
+                        const Particle *p = particles[bin_ids[q]];
+                        if (skip_dc && p->dc) continue;
+                        double N = agp_for(p, (unsigned)ni)->N;
+                        sv.x += N * (p->m * p->v.x) / n->m;
+                        sv.y += N * (p->m * p->v.y) / n->m;
+                        sv.z += N * (p->m * p->v.z) / n->m;
+                    }
+                    gv.x += sv.x; gv.y += sv.y; gv.z += sv.z;
                 }
-                n->v.x += sv.x; n->v.y += sv.y; n->v.z += sv.z;
+                if (close[k]) {
+                    n->v.x += gv.x; n->v.y += gv.y; n->v.z += gv.z;
+                    gv = V3();
+                }
             }
         }
     }
@@ -1103,7 +1159,7 @@ struct oracle_engine {
 #pragma omp barrier
             find_agps();
 #pragma omp barrier
-            if (order == MPM_ORDER_CELL) {
+            if (order != MPM_ORDER_PARTICLE) {
 #pragma omp single
                 build_bins();
                 m2g_cells();
@@ -1117,7 +1173,7 @@ struct oracle_engine {
 #pragma omp barrier
             g2p_velocity(dt);
 #pragma omp barrier
-            if (order == MPM_ORDER_CELL)
+            if (order != MPM_ORDER_PARTICLE)
                 remap_cells();
             else
                 remap(nt);
@@ -1147,7 +1203,7 @@ struct oracle_engine {
 #pragma omp barrier
             find_agps();
 #pragma omp barrier
-            if (order == MPM_ORDER_CELL) {
+            if (order != MPM_ORDER_PARTICLE) {
 #pragma omp single
                 build_bins();
                 m2g_cells();
@@ -1161,7 +1217,7 @@ struct oracle_engine {
 #pragma omp barrier
             g2p_velocity(dt, true);
 #pragma omp barrier
-            if (order == MPM_ORDER_CELL)
+            if (order != MPM_ORDER_PARTICLE)
                 remap_cells(true);
             else
                 remap(nt, true);
@@ -1187,7 +1243,7 @@ struct oracle_engine {
 #pragma omp barrier
             find_agps();
 #pragma omp barrier
-            if (order == MPM_ORDER_CELL) {
+            if (order != MPM_ORDER_PARTICLE) {
 #pragma omp single
                 build_bins();
                 p2g_singlepass_cells();
@@ -1397,7 +1453,8 @@ int oracle_set_damping(oracle_engine *e, double damping, double mass_min) {
 }
 
 int oracle_set_order(oracle_engine *e, int32_t order) {
-    if (!e || (order != MPM_ORDER_PARTICLE && order != MPM_ORDER_CELL)) return MPM_ERR_ARG;
+    if (!e || (order != MPM_ORDER_PARTICLE && order != MPM_ORDER_CELL && order != MPM_ORDER_TILE))
+        return MPM_ERR_ARG;
     e->order = order;
     return MPM_OK;
 }

@@ -47,6 +47,17 @@ CASES = {
     "halfring_doublepass_particle_30": ("half_ring_impact", {"h": 4.0e-4, "depth_cells": 4, "foam_cells": 8,
                                                              "impact_speed": 60.0, "steps": 90},
                                         "doublepass", "particle", "small", 30),
+    "block5_doublepass_tile_12": ("random_block", {"seed": 5, "cells": [20, 18, 10], "n": 3000}, "doublepass",
+                                  "tile", "small", 12),
+    "block6_contact_tile_12": ("random_block", {"seed": 6, "cells": [18, 12, 9], "n": 2500}, "doublepass_contact",
+                               "tile", "small", 12),
+    "block7_singlepass_tile_12": ("random_block", {"seed": 7, "cells": [12, 20, 9], "n": 2500}, "singlepass",
+                                  "tile", "small", 12),
+    "honeycomb64_doublepass_tile_60": ("honeycomb_crush", {"cells": [64, 64, 8], "steps": 300}, "doublepass", "tile",
+                                       "small", 60),
+    "halfring_doublepass_tile_30": ("half_ring_impact", {"h": 4.0e-4, "depth_cells": 4, "foam_cells": 8,
+                                                         "impact_speed": 60.0, "steps": 90},
+                                    "doublepass", "tile", "small", 30),
 }
 
 

@@ -17,17 +17,18 @@ def _scenes():
     return [a, b, c]
 
 
+@pytest.mark.parametrize("order", ["particle", "tile"])
 @pytest.mark.parametrize("variant", ["doublepass", "doublepass_contact", "singlepass"])
-def test_batch_instances_match_separate_runs_and_oracle(variant):
+def test_batch_instances_match_separate_runs_and_oracle(variant, order):
     from code_mpm_amd import Engine
     from oracle.oracle import OracleEngine
     sc = _scenes()
-    batch = Engine(order="particle", variant=variant)
+    batch = Engine(order=order, variant=variant)
     offs = batch.load_batch(sc)
     steps = 12
     batch.step(0.0, steps)  # each instance by its own dt_max
     for k, s in enumerate(sc):
-        alone = OracleEngine(threads=1, variant=variant)
+        alone = OracleEngine(threads=1, order=order, variant=variant)
         alone.load_scene(s)
         alone.step(s.dt, steps)
         sl = slice(int(offs[k]), int(offs[k]) + s.n)
@@ -41,11 +42,12 @@ def test_batch_instances_match_separate_runs_and_oracle(variant):
         assert batch.instance_time(k) == alone.getTime_Current()
 
 
+@pytest.mark.parametrize("order", ["cell", "tile"])
 @pytest.mark.parametrize("variant", ["doublepass", "singlepass"])
-def test_graph_replay_matches_eager(variant):
+def test_graph_replay_matches_eager(variant, order):
     from code_mpm_amd import Engine, scenes
     sc = scenes.honeycomb_crush(cells=(48, 48, 8), steps=100)
-    eager, graph = Engine(variant=variant), Engine(variant=variant)
+    eager, graph = Engine(variant=variant, order=order), Engine(variant=variant, order=order)
     eager.load_scene(sc)
     graph.load_scene(sc)
     for rep in range(4):
@@ -57,10 +59,11 @@ def test_graph_replay_matches_eager(variant):
         assert eager.monitors() == graph.monitors()
 
 
-def test_batch_graph_replay_matches_eager():
+@pytest.mark.parametrize("order", ["cell", "tile"])
+def test_batch_graph_replay_matches_eager(order):
     from code_mpm_amd import Engine
     sc = _scenes()
-    eager, graph = Engine(), Engine()
+    eager, graph = Engine(order=order), Engine(order=order)
     eager.load_batch(sc)
     graph.load_batch(sc)
     for rep in range(3):
@@ -72,7 +75,7 @@ def test_batch_graph_replay_matches_eager():
         graph.health()
 
 
-@pytest.mark.parametrize("order,variant", [("particle", "doublepass"), ("cell", "doublepass"),
+@pytest.mark.parametrize("order,variant", [("particle", "doublepass"), ("cell", "doublepass"), ("tile", "doublepass"),
                                            ("cell", "doublepass_contact"), ("cell", "singlepass"),
                                            ("particle", "singlepass")])
 def test_autotuned_step_equals_ordinary_step(order, variant):
@@ -105,7 +108,7 @@ def _foams():
             for i, n in enumerate((10000, 25000, 14000, 18000))]
 
 
-@pytest.mark.parametrize("order", ["particle", "cell"])
+@pytest.mark.parametrize("order", ["particle", "cell", "tile"])
 def test_graded_foam_batch_matches_oracle(order):
     """Each graded-foam instance of a batch equals its own oracle run, bit
     for bit, in the loop the batched workload runs (DoublePass, perfectly

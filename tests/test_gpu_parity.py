@@ -15,7 +15,7 @@ PARTICLE_FIELDS = ["POSITION", "VELOCITY", "MASS", "VOLUME", "DEFORMATION_GRADIE
 NODE_FIELDS = ["MASS", "MOMENTUM", "VELOCITY", "FORCE", "FORCE_TEMP", "ACTIVE"]
 
 
-ORDERS = ["particle", "cell"]
+ORDERS = ["particle", "cell", "tile"]
 
 
 def _pair(scene, order="particle", variant="doublepass"):
@@ -223,8 +223,9 @@ def _abs_scales(c, scene):
 
 
 @pytest.mark.parametrize("workload", ["elastic_bar", "honeycomb_48x48x8", "half_ring_h0.6mm"])
-def test_cell_order_tracks_reference_order(workload):
-    """The headline (cell) summation order against the reference's
+@pytest.mark.parametrize("order", ["cell", "tile"])
+def test_grouped_order_tracks_reference_order(order, workload):
+    """The grouped summation orders (cell, and tile, the default) against the reference's
     one-thread particle order (the oracle) on the elastic bar, the J2
     honeycomb crush and the half ring: within 1e-12 relative per step on
     grid and particle fields over the first steps, and within 1e-9 over the
@@ -236,8 +237,8 @@ def test_cell_order_tracks_reference_order(workload):
     from code_mpm_amd import Engine
     from oracle.oracle import OracleEngine
     scene, steps = _tracking_scene(workload)
-    print(f"{workload}: {scene.n} particles, {steps} steps, cell order on the GPU vs the oracle", flush=True)
-    g = Engine(order="cell")
+    print(f"{workload}: {scene.n} particles, {steps} steps, {order} order on the GPU vs the oracle", flush=True)
+    g = Engine(order=order)
     g.load_scene(scene)
     c = OracleEngine(threads=1, order="particle")
     c.load_scene(scene)
@@ -265,7 +266,7 @@ def test_cell_order_tracks_reference_order(workload):
         b = np.array([h[key] for h in hist_c])
         scale = max(np.abs(b).max(), floor.get(key, 0.0))
         err = np.abs(a - b).max()
-        print(f"  {key}: max |cell - reference| = {err:.3e} against {scale:.3e} (peak {np.abs(b).max():.3e})",
+        print(f"  {key}: max |{order} - reference| = {err:.3e} against {scale:.3e} (peak {np.abs(b).max():.3e})",
               flush=True)
         assert err <= 1e-9 * scale, (key, err, scale)   # a column that stays 0 must be 0 in both
 

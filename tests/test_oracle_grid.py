@@ -256,9 +256,9 @@ def test_pass1_body_force_gives_g_dt():
     assert np.allclose(dx, (g * sc.dt * sc.dt)[:, None], rtol=1e-12, atol=8 * 8 * 1.8e-18)
 
 
-@pytest.mark.parametrize("order", ["particle", "cell"])
-def test_grid_pins_hold_in_both_orders(order):
-    """The cell summation order adds the same terms in another association."""
+@pytest.mark.parametrize("order", ["particle", "cell", "tile"])
+def test_grid_pins_hold_in_every_order(order):
+    """The cell and tile summation orders add the same terms in other associations."""
     sc = _block(7, shear_only=True)
     sc.damping = 0.2
     o = OracleEngine(threads=1, order=order)
