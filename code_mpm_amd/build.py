@@ -58,10 +58,9 @@ GXX_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math", "-Wall"
 
 
 def build_host(verbose=False):
-    """The C++ host classes (libmpm_physics.so, over libmpm_amd.so), the
-    reference-style bar driver (bin/mpm_bar_demo) and the host-class checker
-    used by the tests (bin/mpm_host_check), with g++: the host side holds no
-    HIP types, only the C ABI."""
+    """The C++ host classes (libmpm_physics.so, over libmpm_amd.so) and the
+    reference-style bar driver (bin/mpm_bar_demo), with g++: the host side
+    holds no HIP types, only the C ABI."""
     inc = ["-I", os.path.join(os.path.dirname(HERE), "include")]
     rocm_lib = "/opt/rocm/lib"
     cmds = [
@@ -73,10 +72,6 @@ def build_host(verbose=False):
     cmds.append(["g++"] + GXX_FLAGS + [os.path.join(HOST, "bar_demo.cpp")] + inc +
                 ["-L", HERE, "-lmpm_physics", "-lmpm_amd", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link," + rocm_lib,
                  "-o", demo + ".tmp"])
-    check = os.path.join(BIN, "mpm_host_check")
-    cmds.append(["g++"] + GXX_FLAGS + [os.path.join(HOST, "host_check.cpp")] + inc +
-                ["-L", HERE, "-lmpm_physics", "-lmpm_amd", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link," + rocm_lib,
-                 "-o", check + ".tmp"])
     for k, cmd in enumerate(cmds):
         if verbose:
             print(" ".join(cmd), flush=True)
@@ -87,8 +82,7 @@ def build_host(verbose=False):
         if k == 0:
             os.replace(HOST_LIB + ".tmp", HOST_LIB)
     os.replace(demo + ".tmp", demo)
-    os.replace(check + ".tmp", check)
-    return HOST_LIB, demo, check
+    return HOST_LIB, demo
 
 
 if __name__ == "__main__":

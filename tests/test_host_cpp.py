@@ -13,7 +13,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "code_mpm_amd", "libmpm_physics.so")
 DEMO = os.path.join(ROOT, "code_mpm_amd", "bin", "mpm_bar_demo")
-CHECK = os.path.join(ROOT, "code_mpm_amd", "bin", "mpm_host_check")
+CHECK = os.path.join(ROOT, "tests", "host", "_build", "mpm_host_check")
 
 
 def read_dump(path):
