@@ -15,11 +15,38 @@ Scenes (``code_mpm_amd.scenes.Scene``) play the part of the reference's
 ``initializeWorld_*`` functions.
 """
 import ctypes
+import time
 
 import numpy as np
 
 from . import capi
 from .capi import MON, NF, PF, check, ptr
+
+
+def script(x, precision):
+    """Script(double, precision) of Definitions.h:46-60: scientific notation,
+    that many digits after the point, explicit sign."""
+    return f"{x:+.{precision}e}"
+
+
+def console_line(m, runtime, description=""):
+    """The line PhysicsEngine::reportConsole writes (PhysicsEngine.cpp:42-151)
+    for the monitors m: the description, then tab-separated columns for time
+    and runtime, Position_y (displacement monitor set), Force_y, momentum_y,
+    Strain_y / Stress_y and the three energies, and a newline.  The
+    principal-value columns are not produced."""
+    s = description + "\ttime: " + script(m["time"], 4) + "\tRuntime: " + script(runtime, 3)
+    if m["has_position"]:
+        s += "\tPosition_y: " + script(m["position_y"], 14)
+    s += "\tForce_y: " + script(m["force_y"], 4)
+    if m["has_momentum"]:
+        s += "\t momentum_y: " + script(m["momentum_y"], 4)
+    if m["has_stress"]:
+        s += "\tStrain_y: " + script(m["strain_y"], 4) + "\tStress_y: " + script(m["stress_y"], 4)
+    if m["has_energy"]:
+        s += ("\tKinetic Energy: " + script(m["kinetic_energy"], 4) + "\tStrain Energy: " +
+              script(m["strain_energy"], 4) + "\tPlastic Energy: " + script(m["plastic_energy"], 4))
+    return s + "\n"
 
 
 class Engine:
@@ -39,6 +66,9 @@ class Engine:
         self.console_interval = None
         self.console_last = -1.0e12
         self.console_log = []
+        self.runtime_total = 0.0   # d_Runtime_Total: wall seconds of the loops' steps
+        self.console_print = False
+        self.log_file = None
         self.scene = None
 
     def set_order(self, order):
@@ -348,6 +378,7 @@ class Engine:
         acc = 0.0
         dt_max = self.scene.dt if self.scene is not None else None
         time_end = self.scene.time_end if self.scene is not None else float("inf")
+        mark = time.perf_counter()
         while acc < dt_total:
             dt = dt_max
             if dt_max > (dt_total - acc):
@@ -357,19 +388,34 @@ class Engine:
             # the console check of the reference sees d_Time before the increment
             self.step(dt, 1)
             if self.console_interval is not None and t_before - self.console_last > self.console_interval:
+                self.synchronize()
+                self.runtime_total += time.perf_counter() - mark
                 self.console_last = t_before
                 self.reportConsole(time_override=t_before)
+                mark = time.perf_counter()
+        self.synchronize()
+        self.runtime_total += time.perf_counter() - mark
         if self.getTime_Current() < time_end:
             return 0
         self.console_last = self.getTime_Current()
         self.reportConsole()
         return 1
 
-    def reportConsole(self, time_override=None):
+    def reportConsole(self, time_override=None, description=""):
+        """PhysicsEngine::reportConsole (PhysicsEngine.cpp:40-183): the
+        monitors, kept in console_log with the line the reference writes
+        (console_line); the line is printed when console_print is set and
+        appended to log_file when that is set."""
         m = self.monitors()
         if time_override is not None:
             m["time"] = time_override
+        m["line"] = console_line(m, self.runtime_total, description)
         self.console_log.append(m)
+        if getattr(self, "console_print", False):
+            print(m["line"], end="", flush=True)
+        if getattr(self, "log_file", None):
+            with open(self.log_file, "a") as f:
+                f.write(m["line"])
         return m
 
     # -- getters -------------------------------------------------------------

@@ -4,8 +4,12 @@
  */
 #include "mpm_physics.h"
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <ctime>
+#include <fstream>
+#include <iostream>
 #include <unordered_map>
 
 namespace mpmhost {
@@ -245,7 +249,62 @@ void PhysicsEngine::initializeWorld_Classic_Bar() {
     d_TimeEnd = 1000.0 * d_TimeIncrement_Maximum;
     d_TimeConsole_Interval = 100.0 * d_TimeIncrement_Maximum;
     d_Time = 0.0;
+    d_Runtime_Total = 0.0;
     b_Uploaded = false;
+    /* the description and the log file, as the reference's scene functions
+     * write them (e.g. PhysicsEngine_initializeWorld_Classic_Bar.cpp:200-255) */
+    double mass = 0.0;
+    for (const MaterialPoint_BC *p : allMaterialPoint) mass += p->d_Mass;
+    char stamp[80], started[80];
+    const std::time_t now = std::time(nullptr);
+    std::strftime(started, sizeof(started), "%d-%m-%Y %H:%M:%S", std::localtime(&now));
+    std::strftime(stamp, sizeof(stamp), "%Y%m%d%H%M%S", std::localtime(&now));
+    std::string d;
+    d += "-------------------------------------------------------------\n";
+    d += "Classic formulation, cantilever bar ---------------------------\n";
+    d += "Process started on: " + std::string(started) + "\n";
+    d += "-------------------------------------------------------------\n";
+    d += "Device: MI355X, one GPU\n";
+    d += "Time increment: " + Script(d_TimeIncrement_Maximum, 6) + "\n";
+    d += "Material Point count: " + Script((long long)allMaterialPoint.size()) + "\n";
+    d += "Mass: " + Script(mass, 6) + "\n";
+    d += "-------------------------------------------------------------\n";
+    d += "Grid Resolution: (" + Script((long long)cells.x) + "," + Script((long long)cells.y) + "," +
+         Script((long long)cells.z) + ")(" + Script(h, 3) + ")\n";
+    d += "dOffset: " + Script(off, 4) + "\n";
+    d += "Modulus: " + Script(E, 3) + " N/m^2" + "\n";
+    d += "Global Damping: " + Script(d_DampingCoefficient, 3) + "\n";
+    d_TimeConsole_Last = 0.0;
+    str_Log_FileName = std::string("./bmp/log_") + stamp + ".txt";
+    reportConsole(d);
+}
+
+std::string Script(double r, int precision) {
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%+.*e", precision, r);
+    return buf;
+}
+
+std::string Script(long long i) { return std::to_string(i); }
+
+std::string consoleLine(const ConsoleReport &r, const std::string &sDescription) {
+    std::string s = sDescription;
+    s += "\ttime: " + Script(r.time, 4);
+    s += "\tRuntime: " + Script(r.runtime, 3);
+    if (r.has_position) s += "\tPosition_y: " + Script(r.position_y, 14);
+    s += "\tForce_y: " + Script(r.force_y, 4);
+    if (r.has_momentum) s += "\t momentum_y: " + Script(r.momentum_y, 4);
+    if (r.has_stress) {
+        s += "\tStrain_y: " + Script(r.strain_y, 4);
+        s += "\tStress_y: " + Script(r.stress_y, 4);
+    }
+    if (r.has_energy) {
+        s += "\tKinetic Energy: " + Script(r.kinetic_energy, 4);
+        s += "\tStrain Energy: " + Script(r.strain_energy, 4);
+        s += "\tPlastic Energy: " + Script(r.plastic_energy, 4);
+    }
+    s += "\n";
+    return s;
 }
 
 int PhysicsEngine::upload() {
@@ -369,8 +428,19 @@ int PhysicsEngine::upload() {
 int PhysicsEngine::run_loop(double total, int32_t variant) {
     if (!gpu) return -1;
     if (!b_Uploaded && upload() != MPM_OK) return -1;
-    if (mpm_set_variant(gpu, variant) != MPM_OK) return fail(0), -1;
+    if (mpm_set_variant(gpu, variant) != MPM_OK || mpm_set_summation_order(gpu, i_SummationOrder) != MPM_OK)
+        return fail(0), -1;
     const double dt_max = d_TimeIncrement_Maximum;
+    /* d_Runtime_Total (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:461, 794):
+     * wall time of the steps, the console reports left out */
+    using clock = std::chrono::steady_clock;
+    clock::time_point mark = clock::now();
+    auto settle = [&]() {
+        mpm_synchronize(gpu);
+        const clock::time_point t = clock::now();
+        d_Runtime_Total += std::chrono::duration<double>(t - mark).count();
+        mark = t;
+    };
     int32_t pending = 0; /* full steps not yet sent to the device */
     auto flush = [&]() -> bool {
         if (!pending) return true;
@@ -394,13 +464,17 @@ int PhysicsEngine::run_loop(double total, int32_t variant) {
         d_Time += dt;
         if (d_TimeConsole_Interval > 0.0 && t_before - d_TimeConsole_Last > d_TimeConsole_Interval) {
             if (!flush()) return fail(0), -1;
+            settle();
             d_TimeConsole_Last = t_before;
-            ConsoleReport r = reportConsole();
-            r.time = t_before;
-            v_ConsoleLog.back() = r;
+            /* the reference reports before d_Time takes this step's increment */
+            d_Time = t_before;
+            reportConsole();
+            d_Time = t_before + dt;
+            mark = clock::now();
         }
     }
     if (!flush()) return fail(0), -1;
+    settle();
     if (d_Time < d_TimeEnd) return 0;
     d_TimeConsole_Last = d_Time;
     reportConsole();
@@ -411,7 +485,8 @@ int PhysicsEngine::runSimulation_Phase(int32_t variant, int32_t phase, double dt
     if (!gpu) return -1;
     if (!b_Uploaded && upload() != MPM_OK) return -1;
     if (phase == MPM_PHASE_BIN) {
-        if (mpm_set_variant(gpu, variant) != MPM_OK) return fail(0), -1;
+        if (mpm_set_variant(gpu, variant) != MPM_OK || mpm_set_summation_order(gpu, i_SummationOrder) != MPM_OK)
+            return fail(0), -1;
         d_PhaseIncrement = dt;
     }
     if (mpm_step_phase(gpu, phase, dt) != MPM_OK) return fail(0), -1;
@@ -430,11 +505,14 @@ int PhysicsEngine::runSimulation_Classic_SinglePass_MP(double total) {
     return run_loop(total, MPM_VARIANT_SINGLEPASS);
 }
 
-ConsoleReport PhysicsEngine::reportConsole() {
+ConsoleReport PhysicsEngine::reportConsole(const std::string &sDescription) {
     ConsoleReport r{};
+    r.time = d_Time;
+    r.runtime = d_Runtime_Total;
     mpm_monitors m{};
-    if (gpu && b_Uploaded && mpm_get_monitors(gpu, &m) == MPM_OK) {
-        r.time = d_Time;
+    /* the monitors of the world as it stands (uploaded first if need be:
+     * the scene functions report before the first step) */
+    if (gpu && (b_Uploaded || upload() == MPM_OK) && mpm_get_monitors(gpu, &m) == MPM_OK) {
         r.position_y = m.position_y;
         r.force_y = m.force_y;
         r.momentum_y = m.momentum_y;
@@ -443,11 +521,20 @@ ConsoleReport PhysicsEngine::reportConsole() {
         r.plastic_energy = m.plastic_energy;
         r.strain_y = m.strain_y;
         r.stress_y = m.stress_y;
+        r.has_position = m.has_position != 0;
+        r.has_momentum = m.has_momentum != 0;
+        r.has_stress = m.has_stress != 0;
+        r.has_energy = m.has_energy != 0;
+    }
+    r.line = consoleLine(r, sDescription);
+    if (b_Console_Print) std::cout << r.line << std::flush;
+    if (!str_Log_FileName.empty()) {
+        /* appended as the reference does; a log directory that does not
+         * exist leaves the file unwritten, as there */
+        std::ofstream log(str_Log_FileName.c_str(), std::ios_base::app);
+        log << r.line;
     }
     v_ConsoleLog.push_back(r);
-    if (b_Console_Print)
-        std::printf("t %.6e  Position_y %+.6e  Force_y %+.6e  Momentum_y %+.6e  KE %+.6e  SE %+.6e  PE %+.6e\n", r.time,
-                    r.position_y, r.force_y, r.momentum_y, r.kinetic_energy, r.strain_energy, r.plastic_energy);
     return r;
 }
 

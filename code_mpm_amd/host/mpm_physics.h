@@ -175,10 +175,27 @@ class ConstitutiveRelation {
     void run(Material_BC *material, double backstress, const double *stress, const double *strain_increment);
 };
 
-/* One console report (PhysicsEngine::reportConsole, PhysicsEngine.cpp:40-183). */
+/* One console report (PhysicsEngine::reportConsole, PhysicsEngine.cpp:40-183):
+ * the values, which monitor lists were non-empty, and the line as written. */
 struct ConsoleReport {
     double time, position_y, force_y, momentum_y, kinetic_energy, strain_energy, plastic_energy, strain_y, stress_y;
+    double runtime; /* d_Runtime_Total: wall seconds of the steps so far */
+    bool has_position, has_momentum, has_stress, has_energy;
+    std::string line;
 };
+
+/* Script(double, precision) of Definitions.h:46-60: scientific notation with
+ * that many digits after the point and an explicit sign. */
+std::string Script(double r, int precision);
+/* Script(int) of Definitions.h:25-41: decimal, no padding. */
+std::string Script(long long i);
+/* The line reportConsole writes (PhysicsEngine.cpp:42-151): the
+ * description, then tab-separated "name: value" columns for time and
+ * runtime, Position_y when the displacement monitor is set, Force_y,
+ * momentum_y, Strain_y / Stress_y, the three energies, and a newline.  The
+ * principal-value columns are not produced (their monitor list is not part
+ * of this build). */
+std::string consoleLine(const ConsoleReport &r, const std::string &sDescription);
 
 /* PhysicsEngine.h:30-122, the timestep part. */
 class PhysicsEngine {
@@ -192,7 +209,9 @@ class PhysicsEngine {
      * h = 1.5625e-3 m, 2 x 2 x 2 points per cell at (i + 0.25) h and
      * (i + 0.75) h, steel (E 200 GPa, nu 0.3, rho 7850, elastic), the x = 0
      * node plane fixed, the last cell column moving at (0, 0, -1) m/s,
-     * dt = 0.1 h / sqrt(E / rho). */
+     * dt = 0.1 h / sqrt(E / rho).  Ends as the reference's scene functions
+     * do: str_Log_FileName = ./bmp/log_<YYYYmmddHHMMSS>.txt and a first
+     * console report headed by the scene's description. */
     void initializeWorld_Classic_Bar();
 
     /* The three step loops (PhysicsEngine_runSimulation_Classic_*.cpp): steps
@@ -221,9 +240,12 @@ class PhysicsEngine {
      * refreshed from the device first. */
     std::vector<MaterialPoint_BC *> getMaterialPoints();
     std::vector<GridPoint *> getGridPoints();
-    /* The reportConsole values now (PhysicsEngine.cpp:40-183); appended to
-     * v_ConsoleLog and printed when b_Console_Print is set. */
-    ConsoleReport reportConsole();
+    /* PhysicsEngine::reportConsole (PhysicsEngine.cpp:40-183): the
+     * monitors now, as one line (consoleLine) printed to stdout when
+     * b_Console_Print is set (the reference always prints) and appended to
+     * str_Log_FileName when that is set; the report is also kept in
+     * v_ConsoleLog. */
+    ConsoleReport reportConsole(const std::string &sDescription = "");
 
     void world_changed() { b_Uploaded = false; }
     const char *last_error() const;
@@ -249,7 +271,9 @@ class PhysicsEngine {
     double d_DampingCoefficient = 0.0;
     double d_Mass_Minimum = 0.0;
     int32_t i_SummationOrder = MPM_ORDER_CELL;
-    bool b_Console_Print = false;
+    double d_Runtime_Total = 0.0;     /* PhysicsEngine.h:116, wall seconds of the steps */
+    std::string str_Log_FileName;     /* PhysicsEngine.h:82; empty: no log file */
+    bool b_Console_Print = true;
     std::vector<ConsoleReport> v_ConsoleLog;
 
   private:

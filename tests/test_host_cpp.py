@@ -5,6 +5,7 @@ members, and the bar world the C++ PhysicsEngine builds
 MaterialPoint_Factory_Classic_CC) is the same, double for double, as the
 bar scene the Python side builds (no device is touched)."""
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -33,7 +34,7 @@ def test_host_library_exports_reference_members():
                    "PhysicsEngine::runSimulation_Classic_SinglePass_MP(double)",
                    "PhysicsEngine::runSimulation_Phase(int, int, double)",
                    "PhysicsEngine::getMaterialPoints()", "PhysicsEngine::getGridPoints()",
-                   "PhysicsEngine::reportConsole()", "GridPoint_Factory::createGrid",
+                   "PhysicsEngine::reportConsole(", "GridPoint_Factory::createGrid",
                    "MaterialPoint_Factory_Classic_CC::createDomain_Cuboid",
                    "GridPoint_Mediator::findAdjacentGridPoints", "GridPoint_Mediator::calculateBases_Classic",
                    "ConstitutiveRelation::calculateIncrement_VonMisesHardening"):
@@ -43,10 +44,11 @@ def test_host_library_exports_reference_members():
 def test_bar_world_matches_python_scene(tmp_path):
     from code_mpm_amd import scenes
     path = str(tmp_path / "bar0.bin")
-    subprocess.run([DEMO, "0", path], check=True, capture_output=True, timeout=60)
+    subprocess.run([DEMO, "0", path], check=True, capture_output=True, timeout=60, cwd=str(tmp_path))
     got, reports = read_dump(path)
     sc = scenes.elastic_bar()
-    assert got["x"].shape[1] == sc.n and reports.shape[0] == 0
+    # the scene's own console report (PhysicsEngine_initializeWorld_Classic_Bar.cpp:240-255)
+    assert got["x"].shape[1] == sc.n and reports.shape[0] == 1 and reports[0, 0] == 0.0
     assert np.array_equal(got["x"], sc.fields["POSITION"])
     assert np.array_equal(got["v"], sc.fields["VELOCITY"])
     assert np.array_equal(got["V"], sc.fields["VOLUME"])
@@ -119,3 +121,53 @@ def test_mediator_matches_oracle(tmp_path, seed):
         counts.append(cnt)
     assert pos == out.size
     assert set(counts) >= {0, 4, 8}
+
+
+def test_bar_world_writes_the_reference_log_header(tmp_path):
+    """initializeWorld_Classic_Bar ends as the reference's scene functions
+    do (PhysicsEngine_initializeWorld_Classic_Bar.cpp:200-255): a
+    description block, then the first console line, printed and appended to
+    ./bmp/log_<YYYYmmddHHMMSS>.txt (Definitions.h:25) when ./bmp exists.
+    Without a device the monitor columns are not available; the GPU test
+    checks them."""
+    from code_mpm_amd import scenes
+    os.makedirs(tmp_path / "bmp")
+    r = subprocess.run([DEMO, "0", str(tmp_path / "bar0.bin")], capture_output=True, text=True, timeout=60,
+                       cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    logs = sorted(os.listdir(tmp_path / "bmp"))
+    assert len(logs) == 1 and re.fullmatch(r"log_\d{14}\.txt", logs[0]), logs
+    text = open(tmp_path / "bmp" / logs[0]).read()
+    assert text in r.stdout                     # the same text printed and logged
+    sc = scenes.elastic_bar()
+    mass = 0.0
+    for m in sc.fields["MASS"]:
+        mass += m                               # in particle order, as the scene code sums it
+    lines = text.split("\n")
+    assert lines[0].startswith("-----") and lines[1].startswith("Classic formulation")
+    assert re.fullmatch(r"Process started on: \d\d-\d\d-\d{4} \d\d:\d\d:\d\d", lines[2])
+    assert f"Time increment: {sc.dt:+.6e}" in lines
+    assert f"Material Point count: {sc.n}" in lines
+    assert f"Mass: {mass:+.6e}" in lines
+    assert f"Grid Resolution: (64,8,8)({sc.h[0]:+.3e})" in lines
+    assert f"dOffset: {sc.h[0] / 2:+.4e}" in lines
+    # the report line follows the description on the same line, tab-separated
+    assert re.search(r"\n\ttime: \+0\.0000e\+00\tRuntime: \+0\.000e\+00\t", text)
+    assert text.endswith("\n")
+
+
+def test_python_console_line_format():
+    """The Python mirror's console line, column by column as
+    PhysicsEngine.cpp:42-151 writes it with Script (Definitions.h:46-60):
+    scientific, explicit sign, 4 digits (14 for Position_y, 3 for Runtime),
+    columns of empty monitor lists left out, Force_y always present."""
+    from code_mpm_amd.engine import console_line
+    m = dict(time=1.25e-6, position_y=0.012345678901234567, force_y=-241.97, momentum_y=3.0e-9,
+             kinetic_energy=1.0, strain_energy=0.5, plastic_energy=0.0, strain_y=-1.5e-3, stress_y=2.5e8,
+             has_position=1, has_momentum=1, has_energy=1, has_stress=1)
+    assert console_line(m, 12.3456, "desc\n") == (
+        "desc\n\ttime: +1.2500e-06\tRuntime: +1.235e+01\tPosition_y: +1.23456789012346e-02"
+        "\tForce_y: -2.4197e+02\t momentum_y: +3.0000e-09\tStrain_y: -1.5000e-03\tStress_y: +2.5000e+08"
+        "\tKinetic Energy: +1.0000e+00\tStrain Energy: +5.0000e-01\tPlastic Energy: +0.0000e+00\n")
+    m.update(has_position=0, has_momentum=0, has_energy=0, has_stress=0)
+    assert console_line(m, 0.0) == "\ttime: +1.2500e-06\tRuntime: +0.000e+00\tForce_y: -2.4197e+02\n"
