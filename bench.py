@@ -82,13 +82,31 @@ def read_traffic():
     return out
 
 
-def cpu_baseline(scene, threads, steps=300, warmup=3):
+def cpu_baseline(threads, steps=300, warmup=3):
     """The oracle (the reference's structure: AoS particles, per-node locks,
     per-step adjacency rebuild) on the x < 64 h slice of the workload; 300
-    steps are about 10 s of work at 16 threads."""
+    steps are about 10 s of work at 16 threads.  It runs in a child process
+    that touches no GPU, with its OpenMP team pinned one thread per core
+    (OMP_PROC_BIND=close, OMP_PLACES=cores); returns (particle-steps/s,
+    particles, steps, seconds, the child's view of the host)."""
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child",
+                        f"{threads},{steps},{warmup}"], env=env, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline child failed ({r.returncode}): {r.stderr[-2000:]}")
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    return d["value"], d["particles"], d["steps"], d["seconds"], d["host"]
+
+
+def _cpu_baseline_child(spec):
+    threads, steps, warmup = (int(v) for v in spec.split(","))
+    # read before the OpenMP team starts: binding pins this thread to one core
+    host = {"logical_cpus": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}}
     from code_mpm_amd import scenes
     from oracle.oracle import OracleEngine
-    sample = scenes.crop_x(scene, 64)
+    sample = scenes.crop_x(scenes.honeycomb_crush(), 64)
     o = OracleEngine(threads=threads)
     o.load_scene(sample)
     o.step(sample.dt, warmup)
@@ -96,7 +114,8 @@ def cpu_baseline(scene, threads, steps=300, warmup=3):
     o.step(sample.dt, steps)
     el = time.perf_counter() - t0
     o.close()
-    return sample.n * steps / el, sample.n, steps, el
+    print(json.dumps({"value": sample.n * steps / el, "particles": sample.n, "steps": steps, "seconds": el,
+                      "host": host}), flush=True)
 
 
 def batch_bench(n_inst, steps):
@@ -124,58 +143,106 @@ def batch_bench(n_inst, steps):
             "ms_per_step": round(1e3 * el / steps, 4)}
 
 
-def surrogate_bench(scene, steps, train_steps=1500):
-    """The learned return map against the exact one on the honeycomb with
-    every particle's stress put on or near the yield surface (so that every
-    particle takes the plastic branch): step time both ways, the share of
-    points the network answers within the 1e-4 sigma_y residual, and the
-    stress error against the exact FP64 run after `steps` steps."""
+def surrogate_bench(steps, seeds, capture_steps, rows_total, train_steps, stress_steps=10):
+    """C3: the learned return map on the J2 honeycomb crush.
+
+    Training data: return-map samples the exact FP64 map itself produces,
+    captured (mpm_set_sample_capture) from full-size runs of the same
+    generator with other seeds, up to rows_total / len(seeds) rows each.
+    Evaluation: the seed-1 crush, `steps` steps with the exact map and
+    `steps` with the learned one (residual tolerance 1e-4 sigma_y), both
+    from the same state; step time, the share of network answers accepted
+    and the stress error at the end.  A second, stress-test comparison puts
+    every particle's stress on or near the yield surface, so that every
+    point takes the plastic branch, and times `stress_steps` steps."""
     import numpy as np
-    from code_mpm_amd import Engine
+    from code_mpm_amd import Engine, scenes
     from code_mpm_amd import surrogate as su
+    log = lambda m: print(f"# c3 {m}", file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    rows, info = su.capture_from_runs(seeds, capture_steps, rows_total // len(seeds), log=log)
+    t_capture = time.perf_counter() - t
+    scene = scenes.honeycomb_crush(seed=1)
     mat = scene.materials[0]
-    rows = su.synthetic_states(mat, scene.n, seed=7)
-    scene.fields["STRESS"] = np.ascontiguousarray(rows[:, :6].T)
-    scene.fields["BACKSTRESS"] = np.ascontiguousarray(rows[:, 12])
+    rows = rows[rows[:, 13] == 0]
+    t = time.perf_counter()
+    data = su.make_training_set(mat, rows[:, :13])
+    nrows = int(rows.shape[0])
+    del rows
     s = su.Surrogate(seed=11)
-    data = su.make_training_set(mat, su.synthetic_states(mat, 200000, seed=3))
     s.fit_normalization(data)
     xs, ts = s.standardise(data)
+    ntrain = int(xs.shape[0])
+    del data
+    t_prep = time.perf_counter() - t
     t = time.perf_counter()
     hist = s.train(xs, ts, steps=train_steps, batch=4096, lr=2e-3, seed=12)
     t_train = time.perf_counter() - t
-    out = {}
-    stress = {}
-    for mode in ("exact", "surrogate"):
-        eng = Engine(device=0, order="cell")
-        eng.load_scene(scene)
-        if mode == "surrogate":
-            eng.set_surrogate(s, material=0, residual_tolerance=1e-4)
-        # a state to compare at equal step counts: both runs take the same two
-        # untimed warm-up steps (first-step allocations stay out of the window)
-        eng.step(scene.dt, 2)
-        eng.synchronize()
-        t0 = time.perf_counter()
-        eng.step(scene.dt, steps)
-        eng.synchronize()
-        el = time.perf_counter() - t0
-        out[mode] = {"ms_per_step": round(1e3 * el / steps, 4), "health": eng.health()}
-        stress[mode] = eng.get_particle("STRESS")
-        eng.close()
-    a, b = stress["exact"], stress["surrogate"]
-    sy = mat["yield_stress"]
-    rel = np.linalg.norm(a - b, axis=0) / np.maximum(np.linalg.norm(a, axis=0), sy)
-    h = out["surrogate"]["health"]
-    calls, falls = h["surrogate_calls"], h["surrogate_fallbacks"]
-    return {"workload": "honeycomb 8.0M with every stress on or near the yield surface (von Mises hardening)",
-            "steps": steps, "residual_tolerance_sigma_y": 1e-4,
-            "exact_ms_per_step": out["exact"]["ms_per_step"], "surrogate_ms_per_step": out["surrogate"]["ms_per_step"],
-            "speedup": round(out["exact"]["ms_per_step"] / out["surrogate"]["ms_per_step"], 3),
-            "network_calls": calls, "accepted_fraction": round((calls - falls) / max(calls, 1), 4),
-            "stress_rel_error_max": float(rel.max()), "stress_rel_error_mean": float(rel.mean()),
-            "stress_rel_error_l2": float(np.linalg.norm(a - b) / np.linalg.norm(a)),
-            "train_steps": train_steps, "train_seconds": round(t_train, 2),
-            "train_loss_first_last": [float(hist[:10].mean()), float(hist[-10:].mean())]}
+    del xs, ts
+    log(f"{nrows} captured rows, {ntrain} yielding, capture {t_capture:.1f} s, prep {t_prep:.1f} s, "
+        f"train {t_train:.1f} s")
+
+    def compare(sc, nsteps, control=False):
+        out, stress = {}, {}
+        for mode in ("exact", "surrogate") + (("control",) if control else ()):
+            # control: the exact map with the tile summation order, a change of
+            # round-off size only, to show how far the crush's own dynamics
+            # carry a perturbation over the same steps
+            eng = Engine(device=0, order="tile" if mode == "control" else "cell")
+            eng.load_scene(sc)
+            if mode == "surrogate":
+                eng.set_surrogate(s, material=0, residual_tolerance=1e-4)
+            # both runs take the same two untimed steps (first-step allocations
+            # stay out of the window) and then the timed ones
+            eng.step(sc.dt, 2)
+            eng.synchronize()
+            t0 = time.perf_counter()
+            eng.step(sc.dt, nsteps)
+            eng.synchronize()
+            el = time.perf_counter() - t0
+            out[mode] = {"ms_per_step": round(1e3 * el / nsteps, 4), "health": eng.health()}
+            stress[mode] = eng.get_particle("STRESS")
+            eng.close()
+            log(f"{sc.name} {mode}: {out[mode]['ms_per_step']:.3f} ms/step")
+        a, b = stress["exact"], stress["surrogate"]
+        rel = np.linalg.norm(a - b, axis=0) / np.maximum(np.linalg.norm(a, axis=0), mat["yield_stress"])
+        h = out["surrogate"]["health"]
+        calls, falls = h["surrogate_calls"], h["surrogate_fallbacks"]
+        extra = {}
+        if control:
+            c = stress["control"]
+            rc = np.linalg.norm(a - c, axis=0) / np.maximum(np.linalg.norm(a, axis=0), mat["yield_stress"])
+            extra["order_control"] = {"what": "exact map, tile summation order, against the exact cell-order run",
+                                      "stress_rel_error_max": float(rc.max()),
+                                      "stress_rel_error_mean": float(rc.mean()),
+                                      "stress_rel_error_l2": float(np.linalg.norm(a - c) / np.linalg.norm(a))}
+        return {**extra, "steps": nsteps, "exact_ms_per_step": out["exact"]["ms_per_step"],
+                "surrogate_ms_per_step": out["surrogate"]["ms_per_step"],
+                "speedup": round(out["exact"]["ms_per_step"] / out["surrogate"]["ms_per_step"], 3),
+                "plastic_points_exact": out["exact"]["health"]["plastic_points"],
+                "network_calls": calls, "accepted_fraction": round((calls - falls) / max(calls, 1), 4),
+                "stress_rel_error_max": float(rel.max()), "stress_rel_error_mean": float(rel.mean()),
+                "stress_rel_error_l2": float(np.linalg.norm(a - b) / np.linalg.norm(a))}
+
+    crush = compare(scene, steps, control=True)
+    rows = su.synthetic_states(mat, scene.n, seed=7)
+    scene.fields["STRESS"] = np.ascontiguousarray(rows[:, :6].T)
+    scene.fields["BACKSTRESS"] = np.ascontiguousarray(rows[:, 12])
+    del rows
+    stress_test = compare(scene, stress_steps)
+    return {"workload": f"J2 honeycomb crush, 512x512x32 cells, {scene.n} particles, seed 1: exact FP64 return map "
+                        "against the learned one (residual tolerance 1e-4 sigma_y, stress error relative to "
+                        "max(|sigma|, sigma_y))",
+            "training": {"source": "mpm_set_sample_capture on full-size crushes of the same generator",
+                         "seeds": [i[0] for i in info], "steps_run": [i[1] for i in info],
+                         "rows_captured": nrows, "rows_yielding": ntrain, "train_steps": train_steps,
+                         "batch": 4096, "capture_s": round(t_capture, 1), "prep_s": round(t_prep, 1),
+                         "train_s": round(t_train, 2),
+                         "loss_first_last": [float(hist[:10].mean()), float(hist[-10:].mean())]},
+            "residual_tolerance_sigma_y": 1e-4,
+            "crush": crush,
+            "all_yielding": dict(stress_test, workload="the same particles with every stress put on or near the "
+                                                       "yield surface (every point takes the plastic branch)")}
 
 
 def _train_surrogate(mat, train_steps):
@@ -340,6 +407,8 @@ def main():
     ap.add_argument("--launch", default="eager", choices=["graph", "eager"],
                     help="time the K steps replayed from one HIP graph (recorded and run once first) or "
                          "launched kernel by kernel; the other is reported beside it")
+    ap.add_argument("--no-late-window", dest="late_window", action="store_false",
+                    help="skip the second timed window at steps 1951-2000 of the crush")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--order", default="cell", choices=["tile", "cell", "particle"],
@@ -349,8 +418,14 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=256,
                     help="graded-foam batch size for the batched measurement (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=50)
-    ap.add_argument("--surrogate-steps", type=int, default=10,
-                    help="steps of the learned-return-map comparison (0: skip)")
+    ap.add_argument("--surrogate-steps", type=int, default=2000,
+                    help="steps of the seed-1 crush each way, exact and learned return map (0: skip)")
+    ap.add_argument("--surrogate-seeds", default="2-9",
+                    help="seeds of the crushes whose return-map samples train the network (a-b or a,b,...)")
+    ap.add_argument("--surrogate-capture-steps", type=int, default=2000)
+    ap.add_argument("--surrogate-rows", type=int, default=50_000_000,
+                    help="captured rows over all training seeds")
+    ap.add_argument("--surrogate-train-steps", type=int, default=20000)
     ap.add_argument("--c4-steps", type=int, default=1000,
                     help="C4 half-ring impact at ~48 M particles: steps each way, exact and learned (0: skip)")
     ap.add_argument("--c4-fd-steps", type=int, default=8000,
@@ -360,7 +435,11 @@ def main():
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="first run one device copy of this many GiB (the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes)")
+    ap.add_argument("--cpu-baseline-child", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_baseline_child:
+        _cpu_baseline_child(args.cpu_baseline_child)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding of this "
               "path is a design note, not a runnable configuration", file=sys.stderr)
@@ -427,8 +506,29 @@ def main():
     ix, iy, iz = idx % nn[0], (idx // nn[0]) % nn[1], idx // (nn[0] * nn[1])
     inner = float(np.mean((ix % 8 != 7) & (iy % 8 != 7) & (iz % 4 != 3))) if active else 0.0
     health = eng.health()
-    steps_total = args.warmup + args.steps + args.profile_steps
+    # every step the engine has run so far (the autotune step, warm-up, both
+    # launch modes' recordings and timed windows, the profiled steps): the
+    # plastic-point counter covers all of them
+    steps_total = eng.getTime_Cycle()
     plastic_frac = health["plastic_points"] / max(1, scene.n * steps_total)
+    # a second timed window late in the 2000-step crush, when the walls have
+    # started to yield: run on to step 1950, then time 50 steps
+    late = None
+    if args.late_window and steps_total < 1950:
+        eng.step(scene.dt, 1950 - steps_total)
+        eng.synchronize()
+        p0, c0 = eng.health()["plastic_points"], eng.getTime_Cycle()
+        t1 = time.perf_counter()
+        eng.step(scene.dt, 50)
+        eng.synchronize()
+        el = time.perf_counter() - t1
+        p1 = eng.health()["plastic_points"]
+        late = {"first_step": c0 + 1, "steps": 50, "launch": "eager",
+                "value": round(scene.n * 50 / el, 1), "unit": "particle-steps/s",
+                "ms_per_step": round(1e3 * el / 50, 4),
+                "plastic_fraction": round((p1 - p0) / (scene.n * 50), 5)}
+        print(f"# late window (steps {c0 + 1}-{c0 + 50}): {late['ms_per_step']:.3f} ms/step, plastic fraction "
+              f"{late['plastic_fraction']}", file=sys.stderr, flush=True)
     kernels = KERNELS[args.order]
     measured = read_traffic() if args.order == "cell" else {}
     per_kernel = {}
@@ -491,20 +591,32 @@ def main():
     eng.close()
 
     batch = batch_bench(args.batch_instances, args.batch_steps) if args.batch_instances > 0 else None
-    sur = surrogate_bench(scenes.honeycomb_crush(), args.surrogate_steps) if args.surrogate_steps > 0 else None
+    sur = None
+    if args.surrogate_steps > 0:
+        sd = args.surrogate_seeds
+        seeds = list(range(int(sd.split("-")[0]), int(sd.split("-")[1]) + 1)) if "-" in sd else \
+            [int(v) for v in sd.split(",")]
+        sur = surrogate_bench(args.surrogate_steps, seeds, args.surrogate_capture_steps, args.surrogate_rows,
+                              args.surrogate_train_steps)
     c4 = c4_bench(args.c4_steps, args.c4_fd_steps) if args.c4_steps > 0 else None
     beso_out, beso_canvas, beso_radius = beso_bench(args.beso_designs) if args.beso_designs > 0 else (None, None, None)
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        v_cpu, n_cpu, s_cpu, el_cpu = cpu_baseline(scene, threads)
+        # every core this process may run on, up to the 16-core host share
+        # of a one-GPU box
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        v_cpu, n_cpu, s_cpu, el_cpu, host = cpu_baseline(threads)
         cpu = {"value": round(v_cpu, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
                "sample": f"x < 64 h slice of the same honeycomb ({n_cpu} particles, 64x512x32 cells), "
-                         f"{s_cpu} steps after 3 warm-up, {el_cpu:.1f} s, OpenMP lock scatter, g++ -O2"}
+                         f"{s_cpu} steps after 3 warm-up, {el_cpu:.1f} s, OpenMP lock scatter, g++ -O2, "
+                         f"one thread per core (OMP_PROC_BIND=close, OMP_PLACES=cores)",
+               "host": host}
+        print(f"# cpu baseline: {threads} threads on {host['affinity_cpus']} usable of {host['logical_cpus']} "
+              f"logical CPUs: {v_cpu:.3e} particle-steps/s", file=sys.stderr, flush=True)
         if threads != 8:
             # the reference's own team size (PhysicsEngine.h:27), half the steps
-            v8, n8, s8, el8 = cpu_baseline(scene, 8, steps=150)
+            v8, n8, s8, el8, _ = cpu_baseline(8, steps=150)
             cpu["at_8_threads"] = {"value": round(v8, 1), "unit": "particle-steps/s", "cores": 8,
                                    "sample": f"same slice, {s8} steps, {el8:.1f} s"}
             print(f"# cpu baseline at 8 threads: {v8:.3e} particle-steps/s", file=sys.stderr, flush=True)
@@ -524,8 +636,11 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "J2 honeycomb crush, 512x512x32 cells, 8.0M particles, exact FP64 return map "
-                               "(von Mises hardening), platen 10 m/s, seed 1",
-                   "particles": scene.n, "active_nodes": active, "plastic_fraction": round(plastic_frac, 5),
+                               "(von Mises hardening), platen 10 m/s, seed 1; walls 5.75 h thick, not the 4 h "
+                               "named (4 h gives 5.7M particles on this grid)",
+                   "wall_thickness": "5.75 h (4 h gives 5.7 M particles on this grid; 5.75 h gives the named 8 M)",
+                   "particles": scene.n, "active_nodes": active, "steps_in_plastic_fraction": steps_total,
+                   "plastic_fraction": round(plastic_frac, 5),
                    "return_map": "exact", "summation_order": args.order,
                    "tile_interior_node_share": round(inner, 4) if args.order == "tile" else None,
                    "node_tile": list(tile) if tile is not None else None,
@@ -535,8 +650,13 @@ def main():
                          "ms_per_step": round(1e3 * elapsed_other / args.steps, 4)},
         "roofline": {"bound": "hbm", "kernel": kernels[dominant], "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": traffic, "algorithmic_bytes": dom_bytes,
+                     "traffic": traffic,
+                     "traffic_source": (f"from profiles/{os.path.basename(TRAFFIC_FILE)} (rocprofv3 --pmc "
+                                        "FETCH_SIZE and WRITE_SIZE, scaled by a copy of known size)")
+                     if traffic is not None else None,
+                     "algorithmic_bytes": dom_bytes,
                      "avg_launch_ms": round(avg_ms[dominant], 4)},
+        "late_window": late,
         "step_effective_gbs": round(step_bytes / (elapsed / args.steps) / 1e9, 1),
         "kernels": per_kernel,
         "variants": variants,

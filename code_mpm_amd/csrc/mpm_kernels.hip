@@ -2070,6 +2070,35 @@ __device__ __forceinline__ void trial_state(const MatTable &mt, int m, double R,
     if (mt.type[m] == 4) K = K + R / sqrt(2.0);
 }
 
+/* The trial deviator s_tr = dev(sigma + C : de) and the network's inputs:
+ * the direction s_tr / sqrt(J2) (6), sqrt(J2) / K, the overstress
+ * (sqrt(J2) - K) / sigma_y, K / sigma_y, the trial pressure / sigma_y,
+ * R / sigma_y, |de| / (sigma_y / E) and sqrt(J2) / sigma_y.  All are formed
+ * in FP64 from the trial state, so the quantities the radial return depends
+ * on reach the network at FP32 resolution before standardisation, instead of
+ * as small differences of raw stresses. */
+__device__ __forceinline__ void surrogate_features(const MatTable &mt, int m, double R, const double sig[6],
+                                                   const double de[6], double stress_scale, double strain_scale,
+                                                   double dtr[6], double dev[6], double &K, double feat[MLP_IN]) {
+    double J2;
+    trial_state(mt, m, R, sig, de, dtr, J2, K);
+    double str[6];
+    for (int k = 0; k < 6; k++) str[k] = sig[k] + dtr[k];
+    const double p = (str[0] + str[1] + str[2]) / 3.0;
+    for (int k = 0; k < 6; k++) dev[k] = str[k] - (k < 3 ? p : 0.0);
+    const double q = sqrt(J2);
+    for (int k = 0; k < 6; k++) feat[k] = dev[k] / q;
+    feat[6] = q / K;
+    feat[7] = (q - K) / stress_scale;
+    feat[8] = K / stress_scale;
+    feat[9] = p / stress_scale;
+    feat[10] = R / stress_scale;
+    double e2 = 0.0;
+    for (int k = 0; k < 6; k++) e2 += de[k] * de[k];
+    feat[11] = sqrt(e2) / strain_scale;
+    feat[12] = q / stress_scale;
+}
+
 /* runSimulation_G2P_P2_SmallStrain (DP:175-280), in two kernels so that
  * each keeps enough loads in flight: the kinematics (node gather, x, L, the
  * strain increment, F and V, the displacement-control schedule of
@@ -2357,13 +2386,11 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_constitutive(Batch bt, PView P, M
         if (queue) {
             sg.qid[slot] = i;
             const size_t cap = (size_t)P.n;
-            float feat[MLP_IN];
-            for (int k = 0; k < 6; k++) feat[k] = (float)(sig[k] / sg.stress_scale);
-            for (int k = 0; k < 6; k++) feat[6 + k] = (float)(de[k] / sg.strain_scale);
-            feat[12] = (float)(R / sg.stress_scale);
+            double dtr[6], dev[6], K, feat[MLP_IN];
+            surrogate_features(mt, mat, R, sig, de, sg.stress_scale, sg.strain_scale, dtr, dev, K, feat);
             uint8_t inside = 1;
             for (int k = 0; k < MLP_IN; k++) {
-                float z = (feat[k] - sg.norm.in_mean[k]) / sg.norm.in_std[k];
+                float z = ((float)feat[k] - sg.norm.in_mean[k]) / sg.norm.in_std[k];
                 inside &= (z >= sg.norm.in_lo[k] && z <= sg.norm.in_hi[k]) ? 1 : 0;
                 sg.qx[(size_t)k * cap + slot] = z; /* feature-major: coalesced */
             }
@@ -2499,6 +2526,73 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_fused(Batch bt, PView P, NView Nd
     block_count_add(health, HEALTH_NULL, f_null);
     block_count_add(health, HEALTH_PLASTIC, f_plastic);
     block_count_add(health, HEALTH_NONFINITE, f_nonfinite);
+}
+
+/* G2P pass 2 of the DoublePass loop with the learned map, in one kernel for
+ * a single instance: the kinematics as in k_g2p_fused, then, for a point of
+ * the surrogate's material whose trial state yields, its inputs go to the
+ * network's queue (and its strain increment to P.de, which the post kernel
+ * reads); every other point takes the exact state update in registers.
+ * Only the queued points' increments cross memory, so with few yielding
+ * points the pass moves what the exact fused pass moves. */
+__global__ __launch_bounds__(BLOCK) void k_g2p_fused_queue(Batch bt, PView P, NView Nd, BView B, MatTable mt,
+                                                          unsigned long long *health, SurrogateArgs sg) {
+    uint32_t i, k;
+    bool live;
+    particle_of_thread<false>(bt, (uint32_t)P.n, i, live, k);
+    uint32_t f_outside = 0, f_null = 0, f_plastic = 0, f_nonfinite = 0, f_queued = 0;
+    uint32_t next_key = MPM_NONE;
+    bool queue = false;
+    KinOut ko;
+    double sig[6], R = 0.0;
+    if (live) {
+        const double *g2 = Nd.g2;
+        g2p_kinematics_particle<VAR_DP, false>(bt, P, bt.inst[0], i,
+                                               [=](int, size_t node, double2 &r0, double2 &r1, double2 &r2,
+                                                   double2 &r3) {
+                                                   const double2 *q = reinterpret_cast<const double2 *>(g2 + node * 8u);
+                                                   r0 = q[0];
+                                                   r1 = q[1];
+                                                   r2 = q[2];
+                                                   r3 = q[3];
+                                               },
+                                               next_key, f_outside, ko);
+        B.key[i] = next_key;
+        if (!ko.ctl && ko.mat == sg.material && ko.mat >= 0 && ko.mat < mt.n) {
+            const size_t n = P.n;
+            for (int c = 0; c < 6; c++) sig[c] = P.sig[c * n + i];
+            R = P.R[i];
+            double dtr[6], J2, K;
+            trial_state(mt, ko.mat, R, sig, ko.de, dtr, J2, K);
+            queue = !(J2 <= K * K);
+        }
+    }
+    const uint32_t slot = block_append<uint32_t>(sg.qcount, queue);
+    if (queue) {
+        const size_t n = P.n, cap = (size_t)P.n;
+        sg.qid[slot] = i;
+        for (int c = 0; c < 6; c++) P.de[c * n + i] = ko.de[c];
+        double dtr[6], dev[6], K, feat[MLP_IN];
+        surrogate_features(mt, ko.mat, R, sig, ko.de, sg.stress_scale, sg.strain_scale, dtr, dev, K, feat);
+        uint8_t inside = 1;
+        for (int c = 0; c < MLP_IN; c++) {
+            float z = ((float)feat[c] - sg.norm.in_mean[c]) / sg.norm.in_std[c];
+            inside &= (z >= sg.norm.in_lo[c] && z <= sg.norm.in_hi[c]) ? 1 : 0;
+            sg.qx[(size_t)c * cap + slot] = z;
+        }
+        sg.qenv[slot] = inside;
+        f_queued = 1;
+        f_plastic = 1;
+        f_nonfinite = ko.flag;
+    } else if (live && !ko.ctl) {
+        state_update_particle<VAR_DP>(P, mt, i, ko, f_null, f_plastic, f_nonfinite);
+    }
+    wave_key_add(B.count, next_key, next_key != MPM_NONE);
+    block_count_add(health, HEALTH_OUTSIDE, f_outside);
+    block_count_add(health, HEALTH_NULL, f_null);
+    block_count_add(health, HEALTH_PLASTIC, f_plastic);
+    block_count_add(health, HEALTH_NONFINITE, f_nonfinite);
+    block_count_add(health, HEALTH_SURROGATE_CALLS, f_queued);
 }
 
 /* ---- large-strain pass ---------------------------------------------------- */
@@ -2722,12 +2816,18 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_large(Batch bt, PView P, NView Nd
     block_count_add(health, HEALTH_NONFINITE, f_nonfinite);
 }
 
-/* Applies the network's output to the queued points: the stress increment
- * is the exact elastic trial increment minus the deviatoric part of the
- * predicted plastic correction; it is accepted when the inputs were inside the training
- * envelope and the new stress lies on the yield surface within
- * tol * sigma_y; otherwise the exact radial return runs.  The plastic strain
- * and hardening increments follow from the accepted stress increment as in
+/* Applies the network's output to the queued points.  The von Mises return
+ * scales the trial deviator
+ * (ConstitutiveRelation_calculateIncrement_VonMisesHardening.cpp:59-67):
+ * the stress increment is C : de - alpha s_tr with the trial pressure kept,
+ * alpha = 1 - K / sqrt(J2).  The network predicts the scalar alpha (its
+ * output 0); s_tr is formed here in FP64, so a direction error is not
+ * possible and the yield-surface check below bounds the error of alpha,
+ * and with it the stress error, by the tolerance.  The answer is accepted
+ * when the inputs were inside the training envelope and the new stress lies
+ * on the yield surface within tol * sigma_y; otherwise the exact radial
+ * return runs.  The plastic strain and hardening increments follow from the
+ * accepted stress increment as in
  * ConstitutiveRelation_calculateIncrement_VonMisesHardening.cpp:69-90. */
 __global__ __launch_bounds__(BLOCK) void k_surrogate_post(Batch bt, PView P, MatTable mt, unsigned long long *health,
                                                          SurrogateArgs sg) {
@@ -2738,29 +2838,19 @@ __global__ __launch_bounds__(BLOCK) void k_surrogate_post(Batch bt, PView P, Mat
         f_fallback = 0;
         f_nonfinite = 0;
         if (slot < count) {
-            const size_t n = P.n, cap = (size_t)P.n;
+            const size_t n = P.n;
             const uint32_t i = sg.qid[slot];
             const int m = P.mat[i];
             double de[6], sig[6];
             for (int k = 0; k < 6; k++) de[k] = P.de[k * n + i];
             for (int k = 0; k < 6; k++) sig[k] = P.sig[k * n + i];
             const double R = P.R[i];
-            double dtr[6], J2, K;
-            trial_state(mt, m, R, sig, de, dtr, J2, K);
+            double dtr[6], dev[6], K, feat[MLP_IN];
+            surrogate_features(mt, m, R, sig, de, sg.stress_scale, sg.strain_scale, dtr, dev, K, feat);
             CRResult cr;
             bool accept = sg.qenv[slot] != 0;
-            double corr[6];
-            for (int k = 0; k < 6; k++)
-                corr[k] = ((double)sg.qy[(size_t)k * cap + slot] * (double)sg.norm.out_std[k] +
-                           (double)sg.norm.out_mean[k]) * sg.stress_scale;
-            /* the radial return keeps the trial pressure (plastic flow is
-             * deviatoric): drop the hydrostatic part of the prediction, which
-             * the yield-surface check below cannot see */
-            const double hyd = (corr[0] + corr[1] + corr[2]) / 3.0;
-            corr[0] -= hyd;
-            corr[1] -= hyd;
-            corr[2] -= hyd;
-            for (int k = 0; k < 6; k++) cr.dsig[k] = dtr[k] - corr[k];
+            const double alpha = (double)sg.qy[slot] * (double)sg.norm.out_std[0] + (double)sg.norm.out_mean[0];
+            for (int k = 0; k < 6; k++) cr.dsig[k] = dtr[k] - alpha * dev[k];
             if (accept) {
                 double s2[6];
                 for (int k = 0; k < 6; k++) s2[k] = sig[k] + cr.dsig[k];
@@ -3025,6 +3115,11 @@ static void launch_g2p_strain(int mode, dim3 grid, hipStream_t s, const Batch &b
         if (mid) (void)hipEventRecord(mid, s);
         return;
     }
+    if (mode == MODE_SURROGATE && VAR == VAR_DP && bt.ninst == 1) {
+        hipLaunchKernelGGL(k_g2p_fused_queue, grid, dim3(BLOCK), 0, s, bt, P, N, B, mt, health, sg);
+        if (mid) (void)hipEventRecord(mid, s);
+        return;
+    }
     if (bt.ninst > 1)
         hipLaunchKernelGGL((k_g2p_kinematics<VAR, true>), grid, dim3(BLOCK), 0, s, bt, P, N, B, health);
     else
@@ -3141,9 +3236,10 @@ hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t s) {
 hipError_t launch_surrogate_post(const Batch &bt, const PView &P, const MatTable &mt, unsigned long long *health,
                                  const SurrogateArgs &sg, hipStream_t s) {
     if (P.n == 0) return hipSuccess;
-    /* the queue length is only known on the device: enough blocks for a
-     * full queue, the surplus exits at once */
-    unsigned grid = (unsigned)std::min<int64_t>(blocks_for(P.n, BLOCK), 32768);
+    /* the queue length is only known on the device: a grid that fills the
+     * chip strides over a full queue; with a short queue the surplus exits
+     * at once */
+    unsigned grid = (unsigned)std::min<int64_t>(blocks_for(P.n, BLOCK), 2048);
     hipLaunchKernelGGL(k_surrogate_post, dim3(grid), dim3(BLOCK), 0, s, bt, P, mt, health, sg);
     return hipGetLastError();
 }

@@ -2,11 +2,15 @@
  * mpm_mlp.hip — BF16 MFMA kernels of the learned return-map surrogate:
  * inference, forward+backward (mean squared error) and Adam.
  *
- * Matrix products use v_mfma_f32_16x16x16_bf16 (wave64).  Operand layout,
- * lane l = 0..63, g = l / 16, r = l % 16:
- *   A (16 x 16, row-major M x K): lane holds A[r][4g + j], j = 0..3
- *   B (16 x 16, K x N):           lane holds B[4g + j][r]
- *   D (16 x 16 FP32):             lane holds D[4g + j][r]
+ * Matrix products use gfx950's v_mfma_f32_16x16x32_bf16 (wave64) wherever
+ * the reduction length is a multiple of 32, and v_mfma_f32_16x16x16_bf16
+ * where it is 16 (the 16 padded inputs, the 16 padded outputs).  Operand
+ * layout, lane l = 0..63, g = l / 16, r = l % 16:
+ *   x32: A (16 x 32, M x K): lane holds A[r][8g + j], j = 0..7
+ *        B (32 x 16, K x N): lane holds B[8g + j][r]
+ *   x16: A (16 x 16):        lane holds A[r][4g + j], j = 0..3
+ *        B (16 x 16):        lane holds B[4g + j][r]
+ *   D (16 x 16 FP32), both:  lane holds D[4g + j][r]
  * Training: weights live in LDS as row-major BF16 with padded row strides;
  * each wave owns 64 sample rows and keeps its activations in LDS.
  * Inference: see k_mlp_infer.
@@ -14,7 +18,15 @@
 #include "mpm_mlp.h"
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+/* D += A (16 x 32) B (32 x 16), operands as BF16 bit patterns */
+__device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+}
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
     uint32_t u = __float_as_uint(f);
@@ -65,31 +77,49 @@ __device__ __forceinline__ void load_weights_lds(const uint16_t *wbf16, const fl
     for (int i = threadIdx.x; i < MLP_OUT_PAD; i += blockDim.x) bias[192 + i] = params[OFF_B4 + i];
 }
 
-/* acc[MT][NT] = A(64 x 16KT) * B(16KT x 16NT) over one wave; fa(row, k) and
- * fb(k, col) return BF16 bits. */
+/* acc[MT][NT] = A(16MT x 16KT) * B(16KT x 16NT) over one wave; fa(row, k)
+ * and fb(k, col) return BF16 bits.  KT even: 16x16x32 steps; KT == 1: one
+ * 16x16x16 step. */
 template <int MT, int NT, int KT, class FA, class FB>
 __device__ __forceinline__ void wave_gemm(FA fa, FB fb, f32x4 (&acc)[MT][NT]) {
+    static_assert(KT == 1 || KT % 2 == 0, "reduction of 16 or a multiple of 32");
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
 #pragma unroll
     for (int m = 0; m < MT; m++)
 #pragma unroll
         for (int n = 0; n < NT; n++) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int k = 0; k < KT; k++) {
+    if (KT == 1) {
         bf16x4 a[MT], b[NT];
 #pragma unroll
         for (int m = 0; m < MT; m++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) a[m][j] = (short)fa(m * 16 + r, k * 16 + 4 * g + j);
+            for (int j = 0; j < 4; j++) a[m][j] = (short)fa(m * 16 + r, 4 * g + j);
 #pragma unroll
         for (int n = 0; n < NT; n++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) b[n][j] = (short)fb(k * 16 + 4 * g + j, n * 16 + r);
+            for (int j = 0; j < 4; j++) b[n][j] = (short)fb(4 * g + j, n * 16 + r);
 #pragma unroll
         for (int m = 0; m < MT; m++)
 #pragma unroll
             for (int n = 0; n < NT; n++)
                 acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b[n], acc[m][n], 0, 0, 0);
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < KT / 2; k++) {
+        s16x8 a[MT], b[NT];
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) a[m][j] = (short)fa(m * 16 + r, k * 32 + 8 * g + j);
+#pragma unroll
+        for (int n = 0; n < NT; n++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) b[n][j] = (short)fb(k * 32 + 8 * g + j, n * 16 + r);
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int n = 0; n < NT; n++) acc[m][n] = mfma32(a[m], b[n], acc[m][n]);
     }
 }
 
@@ -163,12 +193,17 @@ __device__ __forceinline__ void wave_forward(const uint16_t *W, const float *bia
 
 /* Inference in the transposed form H^T = W^T X^T.  The weights are the
  * MFMA A operand and stay in registers for the whole kernel; 16 samples form
- * the N dimension.  A layer's FP32 output tile (D layout: lane holds
- * H^T[4g + j][r]) is, after bias, SiLU and BF16 rounding, exactly the B
- * operand of the next layer (lane holds B[4g + j][r]), so activations never
- * leave registers.  Sample s, feature k of x is x[s * xs + k * xk]; of y,
+ * the N dimension.  A layer's FP32 output tiles (D layout: in tile mt, lane
+ * holds feature 16 mt + 4g + j of sample r) feed the next layer's x32 steps
+ * straight from registers: step t takes tiles 2t and 2t + 1, so lane (g, r)
+ * holds, at reduction position 8g + j, feature 32t + 4g + j (j < 4) or
+ * 32t + 16 + 4g + (j - 4) (j >= 4); the weight fragments are loaded in the
+ * same feature order.  Sample s, feature k of x is x[s * xs + k * xk]; of y,
  * y[s * ys + k * yk]. */
 #define INFER_WAVES 4
+__device__ __forceinline__ int x32_feature(int t, int g, int j) {
+    return 32 * t + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
+}
 __global__ __launch_bounds__(64 * INFER_WAVES) void k_mlp_infer(const float *params, const uint16_t *wbf16,
                                                                 const float *x, int64_t xs, int64_t xk, float *y,
                                                                 int64_t ys, int64_t yk, int64_t n,
@@ -184,26 +219,35 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_mlp_infer(const float *par
     __syncthreads();
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     const int wave = threadIdx.x >> 6;
-    /* A fragments of W^T: tile (mt, kt) lane holds W[kt*16 + 4g + j][mt*16 + r] */
-    bf16x4 w1[4], w2[4][4], w3[4][4], w4[4];
+    /* A fragments of W^T: layer 1 (x16) lane holds W1[4g + j][16 mt + r];
+     * the x32 layers lane holds W[x32_feature(t, g, j)][16 mt + r] */
+    bf16x4 w1[4];
+    s16x8 w2[4][2], w3[4][2], w4[2];
 #pragma unroll
     for (int mt = 0; mt < 4; mt++)
 #pragma unroll
         for (int j = 0; j < 4; j++) w1[mt][j] = (short)wbf16[G_W1 + (4 * g + j) * MLP_HID + mt * 16 + r];
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++)
+    for (int t = 0; t < 2; t++)
 #pragma unroll
-        for (int kt = 0; kt < 4; kt++)
+        for (int j = 0; j < 8; j++) {
+            const int f = x32_feature(t, g, j);
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                w2[mt][kt][j] = (short)wbf16[G_W2 + (kt * 16 + 4 * g + j) * MLP_HID + mt * 16 + r];
-                w3[mt][kt][j] = (short)wbf16[G_W3 + (kt * 16 + 4 * g + j) * MLP_HID + mt * 16 + r];
+            for (int mt = 0; mt < 4; mt++) {
+                w2[mt][t][j] = (short)wbf16[G_W2 + f * MLP_HID + mt * 16 + r];
+                w3[mt][t][j] = (short)wbf16[G_W3 + f * MLP_HID + mt * 16 + r];
             }
-#pragma unroll
-    for (int kt = 0; kt < 4; kt++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) w4[kt][j] = (short)wbf16[G_W4 + (kt * 16 + 4 * g + j) * MLP_OUT_PAD + r];
+            w4[t][j] = (short)wbf16[G_W4 + f * MLP_OUT_PAD + r];
+        }
     const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    /* a layer's SiLU outputs as the next layer's two x32 B operands */
+    auto activate = [&](const f32x4 (&acc)[4], int boff, s16x8 (&bo)[2]) {
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                bo[mt >> 1][(mt & 1) * 4 + j] = (short)f2bf(silu(acc[mt][j] + bias[boff + mt * 16 + 4 * g + j]));
+    };
     for (int64_t s0 = ((int64_t)blockIdx.x * INFER_WAVES + wave) * 16; s0 < n;
          s0 += (int64_t)gridDim.x * INFER_WAVES * 16) {
         const int64_t smp = s0 + r;
@@ -215,42 +259,21 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_mlp_infer(const float *par
             const int k = 4 * g + j;
             xb[j] = (short)f2bf((live && k < MLP_IN) ? x[smp * xs + k * xk] : 0.f);
         }
-        bf16x4 a[4], b[4];
         f32x4 acc[4];
+        s16x8 hb[2];
 #pragma unroll
         for (int mt = 0; mt < 4; mt++) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[mt], xb, zero, 0, 0, 0);
-#pragma unroll
-        for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) a[mt][j] = (short)f2bf(silu(acc[mt][j] + bias[mt * 16 + 4 * g + j]));
+        activate(acc, 0, hb);
         /* layer 2 */
 #pragma unroll
-        for (int mt = 0; mt < 4; mt++) acc[mt] = zero;
-#pragma unroll
-        for (int kt = 0; kt < 4; kt++)
-#pragma unroll
-            for (int mt = 0; mt < 4; mt++)
-                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2[mt][kt], a[kt], acc[mt], 0, 0, 0);
-#pragma unroll
-        for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) b[mt][j] = (short)f2bf(silu(acc[mt][j] + bias[64 + mt * 16 + 4 * g + j]));
+        for (int mt = 0; mt < 4; mt++) acc[mt] = mfma32(w2[mt][1], hb[1], mfma32(w2[mt][0], hb[0], zero));
+        activate(acc, 64, hb);
         /* layer 3 */
 #pragma unroll
-        for (int mt = 0; mt < 4; mt++) acc[mt] = zero;
-#pragma unroll
-        for (int kt = 0; kt < 4; kt++)
-#pragma unroll
-            for (int mt = 0; mt < 4; mt++)
-                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3[mt][kt], b[kt], acc[mt], 0, 0, 0);
-#pragma unroll
-        for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) a[mt][j] = (short)f2bf(silu(acc[mt][j] + bias[128 + mt * 16 + 4 * g + j]));
+        for (int mt = 0; mt < 4; mt++) acc[mt] = mfma32(w3[mt][1], hb[1], mfma32(w3[mt][0], hb[0], zero));
+        activate(acc, 128, hb);
         /* layer 4, linear: lane holds Y^T[4g + j][r] */
-        f32x4 o = zero;
-#pragma unroll
-        for (int kt = 0; kt < 4; kt++) o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w4[kt], a[kt], o, 0, 0, 0);
+        const f32x4 o = mfma32(w4[1], hb[1], mfma32(w4[0], hb[0], zero));
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int f = 4 * g + j;

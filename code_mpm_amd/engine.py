@@ -474,6 +474,12 @@ class Engine:
         self._check(self._lib.mpm_set_sample_capture(self._h, int(capacity)), "set_sample_capture")
         self._capture_capacity = int(capacity)
 
+    def captured_samples_count(self):
+        """Rows the capture has produced so far (may exceed its capacity)."""
+        cnt = ctypes.c_int64()
+        self._check(self._lib.mpm_get_captured_samples(self._h, None, 0, ctypes.byref(cnt)), "captured count")
+        return cnt.value
+
     def captured_samples(self):
         """(rows, produced): rows is (k, 14) = sigma6, dstrain6, R, material."""
         cnt = ctypes.c_int64()

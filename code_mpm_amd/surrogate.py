@@ -3,15 +3,21 @@
 ``Surrogate`` wraps the C-ABI object: a 13 -> 64 -> 64 -> 64 -> 7 SiLU MLP
 whose forward, backward and Adam steps run in hand-written BF16 MFMA kernels
 on the GPU.  ``make_training_set`` turns return-map samples (captured from
-exact runs) into standardised (input, target) rows:
+exact runs) into (input, target) rows.  The von Mises return scales the trial
+deviator s_tr = dev(sigma + C : de)
+(ConstitutiveRelation_calculateIncrement_VonMisesHardening.cpp:59-67): the
+returned increment is C : de - alpha s_tr with alpha = 1 - K / sqrt(J2), so
+the network predicts the scalar alpha from invariants of the trial state
+formed in FP64:
 
-  inputs   sigma / s_sigma (6), dstrain / s_eps (6), R / s_sigma (1)
-  targets  (dsigma_trial - dsigma_exact) / s_sigma (6), dep_bar / s_eps (1)
+  inputs   s_tr / sqrt(J2) (6), sqrt(J2) / K, (sqrt(J2) - K) / s_sigma,
+           K / s_sigma, p_tr / s_sigma, R / s_sigma, |de| / s_eps,
+           sqrt(J2) / s_sigma
+  targets  alpha, dep_bar / s_eps, and five outputs held at zero
 
-with s_sigma = sigma_y and s_eps = sigma_y / E of the material; the targets'
-stress part is the plastic correction the radial return applies to the
-exact elastic trial increment
-(ConstitutiveRelation_calculateIncrement_VonMisesHardening.cpp:59-67).
+with K = K0 + R / sqrt(2), s_sigma = sigma_y and s_eps = sigma_y / E of the
+material.  The device kernels form the same inputs (surrogate_features in
+mpm_kernels.hip).
 """
 import ctypes
 
@@ -32,11 +38,39 @@ def hooke_increment(E, nu, de):
     return out
 
 
+def trial_invariants(material, sig, de, R):
+    """Trial deviator s_tr (k, 6), sqrt(J2), K and the trial pressure, as the
+    reference's return map forms them (VonMisesHardening.cpp:15-46)."""
+    E, nu, sy = material["elastic_modulus"], material["poisson_ratio"], material["yield_stress"]
+    st = sig + hooke_increment(E, nu, de)
+    p = (st[:, 0] + st[:, 1] + st[:, 2]) / 3.0
+    dev = st.copy()
+    dev[:, :3] -= p[:, None]
+    J2 = ((st[:, 0] - st[:, 1]) ** 2 + (st[:, 1] - st[:, 2]) ** 2 + (st[:, 2] - st[:, 0]) ** 2) / 6.0 \
+        + st[:, 3] ** 2 + st[:, 4] ** 2 + st[:, 5] ** 2
+    K = np.full(len(R), sy / np.sqrt(3.0))
+    if material.get("type", 4) == 4:
+        K = K + R / np.sqrt(2.0)
+    return dev, np.sqrt(J2), K, p
+
+
+def features(material, sig, de, R):
+    """The network's 13 inputs (before standardisation), rows of 13."""
+    sy, E = material["yield_stress"], material["elastic_modulus"]
+    s_sig, s_eps = sy, sy / E
+    dev, q, K, p = trial_invariants(material, sig, de, R)
+    return np.concatenate([dev / q[:, None], (q / K)[:, None], ((q - K) / s_sig)[:, None], (K / s_sig)[:, None],
+                           (p / s_sig)[:, None], (R / s_sig)[:, None],
+                           (np.sqrt(np.sum(de * de, axis=1)) / s_eps)[:, None], (q / s_sig)[:, None]], axis=1)
+
+
 def make_training_set(material, rows, device=0, yielding_only=True):
     """rows (k, >=13): sigma6, de6, R.  Targets come from the device's exact
-    return map.  Only rows whose trial state yields are kept by default: the
-    engine sends only those to the network.  Returns dict with raw features,
-    raw targets, scales."""
+    return map: alpha is the fraction of the trial deviator it removed,
+    (C : de - dsigma) : s_tr / (s_tr : s_tr), shears counted twice.  Only
+    rows whose trial state yields are kept by default: the engine sends only
+    those to the network.  Returns dict with raw features, raw targets,
+    scales."""
     sig = np.ascontiguousarray(rows[:, 0:6])
     de = np.ascontiguousarray(rows[:, 6:12])
     R = np.ascontiguousarray(rows[:, 12])
@@ -46,11 +80,15 @@ def make_training_set(material, rows, device=0, yielding_only=True):
         sig, de, R, ds, dp, dR = sig[keep], de[keep], R[keep], ds[keep], dp[keep], dR[keep]
     E, nu, sy = material["elastic_modulus"], material["poisson_ratio"], material["yield_stress"]
     s_sig, s_eps = sy, sy / E
-    trial = hooke_increment(E, nu, de)
-    corr = trial - ds
+    corr = hooke_increment(E, nu, de) - ds
+    dev, _, _, _ = trial_invariants(material, sig, de, R)
+    w = np.array([1.0, 1.0, 1.0, 2.0, 2.0, 2.0])
+    alpha = np.sum(w * corr * dev, axis=1) / np.sum(w * dev * dev, axis=1)
     epbar = np.sqrt(2.0 / 3.0 * np.sum(dp * dp, axis=1))
-    x = np.concatenate([sig / s_sig, de / s_eps, (R / s_sig)[:, None]], axis=1)
-    t = np.concatenate([corr / s_sig, (epbar / s_eps)[:, None]], axis=1)
+    x = features(material, sig, de, R)
+    t = np.zeros((len(R), 7))
+    t[:, 0] = alpha
+    t[:, 1] = epbar / s_eps
     return dict(x=x, t=t, stress_scale=s_sig, strain_scale=s_eps, dstress=ds, dplastic=dp, dR=dR)
 
 
@@ -108,7 +146,7 @@ class Surrogate:
         om, osd = t.mean(0), t.std(0) + 1e-30
         z = (x - im) / isd
         lo, hi = z.min(0), z.max(0)
-        span = hi - lo
+        span = np.maximum(hi - lo, 1e-3)   # a constant feature keeps a band that FP32 rounding stays inside
         self.set_normalization(data["stress_scale"], data["strain_scale"], im, isd, lo - margin * span,
                                hi + margin * span, om, osd)
 
@@ -149,6 +187,37 @@ class Surrogate:
         self._ck(self._lib.mpm_surrogate_train(self.handle, int(steps), int(batch), float(lr), int(seed), ptr(hist)),
                  "train")
         return hist
+
+
+def capture_from_runs(seeds, steps, capacity_per_run, scene_kw=None, device=0, log=None):
+    """Return-map samples the exact map itself produces on seeded honeycomb
+    crushes (scenes.honeycomb_crush(seed=s, **scene_kw)): each run steps
+    with the exact FP64 map and records every yielding point's inputs
+    (sigma6, dstrain6, R, material index) through mpm_set_sample_capture,
+    until `capacity_per_run` rows are held or `steps` steps are done.
+    Returns (rows (k, 14), per-run [(seed, steps run, rows kept)])."""
+    from . import scenes
+    from .engine import Engine
+    out, info = [], []
+    for seed in seeds:
+        sc = scenes.honeycomb_crush(seed=seed, steps=steps, **(scene_kw or {}))
+        eng = Engine(device=device, order="cell")
+        eng.load_scene(sc)
+        eng.set_sample_capture(capacity_per_run)
+        done, chunk = 0, max(1, steps // 20)
+        while done < steps:
+            k = min(chunk, steps - done)
+            eng.step(sc.dt, k)
+            done += k
+            if eng.captured_samples_count() >= capacity_per_run:
+                break
+        rows, produced = eng.captured_samples()
+        eng.close()
+        out.append(rows)
+        info.append((int(seed), int(done), int(rows.shape[0])))
+        if log:
+            log(f"capture seed {seed}: {sc.n} particles, {done} steps, {rows.shape[0]} rows (of {produced})")
+    return np.concatenate(out), info
 
 
 def synthetic_states(material, n, seed=0, strain_scale=None, r_max=None, on_surface=0.8):

@@ -418,12 +418,17 @@ int mpm_eso_filter(int32_t device, const uint8_t *active, const uint8_t *eso, co
  * accumulation on MFMA, standing in for
  * ConstitutiveRelation::calculateIncrement_VonMisesHardening /
  * calculateIncrement_Plastic (ConstitutiveRelation_calculateIncrement_*.cpp)
- * at trial states outside the yield surface.  Inputs: sigma (6) and R scaled
- * by a stress scale, the strain increment (6) by a strain scale, then
- * standardised; outputs: the plastic correction of the stress increment
- * (elastic trial increment minus the returned increment, 6, over the stress
- * scale) and the equivalent plastic strain increment (over the strain
- * scale), standardised.  Parameters: mpm_surrogate_param_count() FP32 values,
+ * at trial states outside the yield surface.  Inputs (13, formed in FP64
+ * from the trial state sigma + C : de, then standardised in FP32): the trial
+ * deviator's direction s_tr / sqrt(J2) (6), sqrt(J2) / K, the overstress
+ * (sqrt(J2) - K) / s_sigma, K / s_sigma, the trial pressure / s_sigma,
+ * R / s_sigma, |de| / s_eps and sqrt(J2) / s_sigma, with K = K0 + R/sqrt(2),
+ * s_sigma the stress scale and s_eps the strain scale.  Outputs (7,
+ * standardised): the radial-scale correction alpha, the fraction of the
+ * trial deviator the return removes (the returned increment is
+ * C : de - alpha s_tr), the equivalent plastic strain increment / s_eps, and
+ * five unused outputs trained to zero.  Parameters:
+ * mpm_surrogate_param_count() FP32 values,
  * W1[16][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64][16] b4[16]. */
 typedef struct mpm_surrogate mpm_surrogate;
 int64_t mpm_surrogate_param_count(void);

@@ -100,14 +100,11 @@ def test_zero_tolerance_falls_back_to_exact_bitwise():
 
 
 def test_trained_surrogate_in_step():
-    """Trained on the exact map's samples, the network predicts the plastic
-    correction far better than the elastic predictor (no return).  Inside
-    the step, at the strict tolerance the exact fallback keeps stresses
+    """Trained on the exact map's samples, the network predicts the radial
+    scale alpha far better than the elastic predictor (alpha = 0, no
+    return).  Inside the step, at the strict tolerance the stresses stay
     within 1e-3 of the exact run; at a loose tolerance predictions are
-    accepted and the stress error stays within what that tolerance allows.
-    The inputs are raw (sigma, de, R) scaled by sigma_y, so BF16 rounding of
-    the stress features (about 4e-3 sigma_y) bounds the accuracy of the
-    predicted correction, which is itself a few 1e-3 sigma_y."""
+    accepted and the stress error stays within what that tolerance allows."""
     from code_mpm_amd import Engine
     from code_mpm_amd import surrogate as su
     sc = _plastic_scene()
@@ -126,19 +123,20 @@ def test_trained_surrogate_in_step():
     # held-out accuracy of the correction vs the elastic predictor
     test = su.make_training_set(MAT, su.synthetic_states(MAT, 20000, seed=99))
     y = s.forward(((test["x"] - s.in_mean) / s.in_std).astype(np.float32))
-    pred = y[:, :6] * s.out_std[:6] + s.out_mean[:6]
-    err_net = np.linalg.norm(pred - test["t"][:, :6])
-    err_none = np.linalg.norm(test["t"][:, :6])
-    assert err_net < 0.35 * err_none, err_net / err_none
+    pred = y[:, 0] * s.out_std[0] + s.out_mean[0]
+    err_net = np.linalg.norm(pred - test["t"][:, 0])
+    err_none = np.linalg.norm(test["t"][:, 0])
+    assert err_net < 0.05 * err_none, err_net / err_none
     steps = 5
     exact = Engine()
     exact.load_scene(sc)
     exact.step(sc.dt, steps)
     a = exact.get_particle("STRESS")
-    # accepted points sit on the yield surface within tol * sigma_y; what the
-    # check cannot see is the direction error of the predicted deviatoric
-    # correction, which accumulates over the accepted steps
-    for tol, bound in ((1e-4, 5e-3), (5e-2, 0.5)):
+    # accepted points sit on the yield surface within tol * sigma_y, and the
+    # returned deviator is a scaling of the exact trial deviator, so an
+    # accepted point's stress is within about tol * sqrt(2) of the exact one
+    # per step
+    for tol, bound in ((1e-4, 1e-3), (5e-2, 0.5)):
         sur = Engine()
         sur.load_scene(sc)
         sur.set_surrogate(s, material=0, residual_tolerance=tol)
@@ -151,3 +149,51 @@ def test_trained_surrogate_in_step():
         assert np.isfinite(b).all()
         rel = np.linalg.norm(a - b, axis=0) / np.maximum(np.linalg.norm(a, axis=0), MAT["yield_stress"])
         assert rel.max() < bound, (tol, rel.max())
+
+
+def test_surrogate_on_the_honeycomb_crush():
+    """The learned map on the J2 honeycomb crush (64 x 64 x 8 cells, the
+    workload's generator at a test size: 2 x 2 hexagonal cells, platen at
+    50 m/s so that the walls yield within the run), trained on return-map samples
+    captured from seeded runs of the same generator (seeds 2 and 3), tried
+    on seed 1: at tolerance 0 every answer falls back and the run equals the
+    exact one bit for bit; at 1e-4 sigma_y the stresses stay within 1e-3 of
+    the exact FP64 run (relative, against max(|sigma|, sigma_y))."""
+    from code_mpm_amd import Engine, scenes
+    from code_mpm_amd import surrogate as su
+    kw = dict(cells=(64, 64, 8), hex_cells=(2, 2), platen_speed=50.0)
+    rows, info = su.capture_from_runs([2, 3], 300, 400000, scene_kw=kw)
+    print("captured", info, flush=True)
+    assert rows.shape[0] > 50000 and np.all(rows[:, 13] == 0)
+    sc = scenes.honeycomb_crush(seed=1, steps=300, **kw)
+    mat = sc.materials[0]
+    data = su.make_training_set(mat, rows[:, :13])
+    s = su.Surrogate(seed=5)
+    s.fit_normalization(data)
+    xs, ts = s.standardise(data)
+    hist = s.train(xs, ts, steps=3000, batch=4096, lr=2e-3, seed=6)
+    assert hist[-50:].mean() < 0.2 * hist[:10].mean()
+    steps = 200
+    exact = Engine()
+    exact.load_scene(sc)
+    exact.step(sc.dt, steps)
+    a = exact.get_particle("STRESS")
+    sy = mat["yield_stress"]
+    for tol in (0.0, 1e-4):
+        sur = Engine()
+        sur.load_scene(sc)
+        sur.set_surrogate(s, material=0, residual_tolerance=tol)
+        sur.step(sc.dt, steps)
+        h = sur.health()
+        b = sur.get_particle("STRESS")
+        print(f"tol {tol}: {h['surrogate_calls']} network calls, {h['surrogate_fallbacks']} fallbacks", flush=True)
+        assert h["surrogate_calls"] > 1000
+        if tol == 0.0:
+            assert h["surrogate_fallbacks"] == h["surrogate_calls"]
+            for name in ("STRESS", "STRAIN_PLASTIC", "BACKSTRESS", "ENERGY_PLASTIC", "POSITION"):
+                assert np.array_equal(exact.get_particle(name), sur.get_particle(name)), name
+        else:
+            rel = np.linalg.norm(a - b, axis=0) / np.maximum(np.linalg.norm(a, axis=0), sy)
+            print(f"tol {tol}: max stress error {rel.max():.3e}, accepted "
+                  f"{1 - h['surrogate_fallbacks'] / h['surrogate_calls']:.4f}", flush=True)
+            assert rel.max() <= 1e-3, rel.max()
