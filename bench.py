@@ -67,13 +67,14 @@ KERNELS = {
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_honeycomb8m.txt")
 
 
-def read_traffic():
-    """HBM bytes per launch per kernel, from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE scaled by a copy of known size)."""
+def read_traffic(path=TRAFFIC_FILE):
+    """HBM bytes per launch per kernel, from a rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a copy of known size; by default the
+    committed one)."""
     out = {}
-    if not os.path.exists(TRAFFIC_FILE):
+    if not os.path.exists(path):
         return out
-    for line in open(TRAFFIC_FILE):
+    for line in open(path):
         if line.startswith("#") or "hbm_bytes_per_launch[" not in line:
             continue
         tail = line.split("hbm_bytes_per_launch[", 1)[1]
@@ -301,7 +302,7 @@ def _curve_error(a, b):
             "l2": float(np.linalg.norm(a - b) / norm) if norm > 0 else None}
 
 
-def c4_bench(steps, fd_steps, every=250, fd_every=200, train_steps=1500):
+def c4_bench(steps, fd_steps, every=100, fd_every=200, train_steps=1500):
     """C4: the half ring crushed by a foam impactor (scenes.half_ring_impact),
     exact FP64 return map against the learned one on the ring's steel.
     At the full ~48 M-particle size, `steps` steps each way: step time, the
@@ -309,7 +310,8 @@ def c4_bench(steps, fd_steps, every=250, fd_every=200, train_steps=1500):
     the ring's energy curves.  The stress wave needs thousands of steps to
     reach the support, so the force-displacement comparison runs the same
     set-up at 4 h (about 0.8 M particles) for fd_steps steps: support reaction
-    against platen displacement, exact vs learned."""
+    against platen displacement, exact vs learned.  The full-size runs' own
+    support-reaction curve is reported beside it (force_displacement_full)."""
     import numpy as np
     from code_mpm_amd import scenes
     t = time.perf_counter()
@@ -327,6 +329,7 @@ def c4_bench(steps, fd_steps, every=250, fd_every=200, train_steps=1500):
     h = full["surrogate"]["health"]
     calls, falls = h["surrogate_calls"], h["surrogate_fallbacks"]
     e_rows, s_rows = full["exact"]["rows"], full["surrogate"]["rows"]
+    y0_full = sc.fields["POSITION"][1][sc.dc_index].mean()
     del sc
     small = scenes.half_ring_impact(h=2.0e-4, depth_cells=23, foam_cells=20)
     fd = {}
@@ -349,6 +352,12 @@ def c4_bench(steps, fd_steps, every=250, fd_every=200, train_steps=1500):
         "plastic_points": full["exact"]["health"]["plastic_points"],
         "ring_strain_energy_error": _curve_error(e_rows[:, 2], s_rows[:, 2]),
         "ring_plastic_energy_error": _curve_error(e_rows[:, 3], s_rows[:, 3]),
+        "force_displacement_full": {
+            "what": f"support reaction against impactor displacement on the full-size runs, sampled every {every} "
+                    "steps",
+            "displacement_end_m": float(y0_full - e_rows[-1, 0]),
+            "force_peak_exact_N": float(np.abs(e_rows[:, 1]).max()),
+            "force_error": _curve_error(e_rows[:, 1], s_rows[:, 1])},
         "force_displacement": {
             "workload": f"the same set-up at h = 2e-4 m, {small.n} particles, {fd_steps} steps, "
                         f"support reaction sampled every {fd_every} steps",
@@ -417,7 +426,8 @@ def main():
                     help="timed steps of the contact and single-pass loops on the same state (0: skip)")
     ap.add_argument("--batch-instances", type=int, default=256,
                     help="graded-foam batch size for the batched measurement (0: skip)")
-    ap.add_argument("--batch-steps", type=int, default=50)
+    ap.add_argument("--batch-steps", type=int, default=500,
+                    help="graph-replayed steps of the batch (C2 names 500)")
     ap.add_argument("--surrogate-steps", type=int, default=2000,
                     help="steps of the seed-1 crush each way, exact and learned return map (0: skip)")
     ap.add_argument("--surrogate-seeds", default="2-9",
@@ -432,6 +442,8 @@ def main():
                     help="C4 force-displacement comparison at 4 h: steps each way")
     ap.add_argument("--beso-designs", type=int, default=4,
                     help="BESO designs batched in one iteration of the L-problem (0: skip)")
+    ap.add_argument("--traffic-file", default=TRAFFIC_FILE,
+                    help="rocprofv3 --pmc summary the roofline's measured traffic is read from")
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="first run one device copy of this many GiB (the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -530,7 +542,7 @@ def main():
         print(f"# late window (steps {c0 + 1}-{c0 + 50}): {late['ms_per_step']:.3f} ms/step, plastic fraction "
               f"{late['plastic_fraction']}", file=sys.stderr, flush=True)
     kernels = KERNELS[args.order]
-    measured = read_traffic() if args.order == "cell" else {}
+    measured = read_traffic(args.traffic_file) if args.order == "cell" else {}
     per_kernel = {}
     for k, ms in avg_ms.items():
         if kernels[k] is None:
@@ -651,7 +663,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kernels[dominant], "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": traffic,
-                     "traffic_source": (f"from profiles/{os.path.basename(TRAFFIC_FILE)} (rocprofv3 --pmc "
+                     "traffic_source": (f"from {os.path.relpath(args.traffic_file, ROOT)} (rocprofv3 --pmc "
                                         "FETCH_SIZE and WRITE_SIZE, scaled by a copy of known size)")
                      if traffic is not None else None,
                      "algorithmic_bytes": dom_bytes,
