@@ -8,7 +8,8 @@
  *   mpm_bar_demo STEPS OUTFILE [ORDER [PHASES]]
  *       ORDER: 0 particle, 1 cell (default 0); STEPS 0: the world as built,
  *       no device use; PHASES 1: each step driven phase by phase
- *       (runSimulation_Phase) instead of by the loop (no console reports)
+ *       (runSimulation_Phase) instead of by the loop (no console reports
+ *       besides the scene's own at t = 0)
  *
  * File: int64 n, int64 reports; then n-major doubles per field, component
  * by component (x 3, v 3, sigma 6, F 9, strain energy 1, volume 1); then 9
