@@ -49,9 +49,16 @@ FUSED_G2P = {"g2p_kinematics": (26 * 8 + 5 + 30 * 8, 7 * 8)}
 # kernel of each slot, per summation order (names as rocprofv3 reports them)
 KERNELS = {
     "cell": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
-             "p2g_partials": "k_cell_partials<false, false, false, false>", "p2g_nodes": "k_node_cells<false, false>",
-             "g2p_velocity": "k_g2p_velocity<false>", "remap_partials": "k_cell_partials<true, false, false, false>",
-             "remap_nodes": "k_node_cells<true, false>", "g2p_kinematics": "k_g2p_fused<0, false>",
+             "p2g_partials": "k_cell_partials<false, false, false, false, false>",
+             "p2g_nodes": "k_node_cells<false, false, false>", "g2p_velocity": "k_g2p_velocity<false>",
+             "remap_partials": "k_cell_partials<true, false, false, false, false>",
+             "remap_nodes": "k_node_cells<true, false, false>", "g2p_kinematics": "k_g2p_fused<0, false>",
+             "g2p_constitutive": None},
+    "pair": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
+             "p2g_partials": "k_cell_partials<false, false, false, false, true>",
+             "p2g_nodes": "k_node_cells<false, false, true>", "g2p_velocity": "k_g2p_velocity<false>",
+             "remap_partials": "k_cell_partials<true, false, false, false, true>",
+             "remap_nodes": "k_node_cells<true, false, true>", "g2p_kinematics": "k_g2p_fused<0, false>",
              "g2p_constitutive": None},
     "tile": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
              "p2g_partials": "k_tile_p2g<false, false, false>", "p2g_nodes": "k_tile_bnd<false, false>",
@@ -59,9 +66,9 @@ KERNELS = {
              "remap_nodes": "k_tile_bnd<true, false>", "g2p_kinematics": "k_g2p_fused<0, false>",
              "g2p_constitutive": None},
     "particle": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
-                 "p2g_partials": "k_cell_partials<false, false, false, true>", "p2g_nodes": "k_p2g_list<false>",
-                 "g2p_velocity": "k_g2p_velocity<false>",
-                 "remap_partials": "k_cell_partials<true, false, false, true>", "remap_nodes": "k_remap_list",
+                 "p2g_partials": "k_cell_partials<false, false, false, true, false>",
+                 "p2g_nodes": "k_p2g_list<false>", "g2p_velocity": "k_g2p_velocity<false>",
+                 "remap_partials": "k_cell_partials<true, false, false, true, false>", "remap_nodes": "k_remap_list",
                  "g2p_kinematics": "k_g2p_fused<0, false>", "g2p_constitutive": None},
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_honeycomb8m.txt")
@@ -125,7 +132,7 @@ def batch_bench(n_inst, steps):
     from code_mpm_amd import Engine, scenes
     t = time.perf_counter()
     scs = [scenes.graded_foam(1000 + i) for i in range(n_inst)]
-    eng = Engine(device=0, order="cell")
+    eng = Engine(device=0, order="pair")
     eng.load_batch(scs)
     ntot = sum(sc.n for sc in scs)
     print(f"# batch: {n_inst} instances, {ntot} particles, setup {time.perf_counter() - t:.1f} s",
@@ -189,7 +196,7 @@ def surrogate_bench(steps, seeds, capture_steps, rows_total, train_steps, stress
             # control: the exact map with the tile summation order, a change of
             # round-off size only, to show how far the crush's own dynamics
             # carry a perturbation over the same steps
-            eng = Engine(device=0, order="tile" if mode == "control" else "cell")
+            eng = Engine(device=0, order="tile" if mode == "control" else "pair")
             eng.load_scene(sc)
             if mode == "surrogate":
                 eng.set_surrogate(s, material=0, residual_tolerance=1e-4)
@@ -213,7 +220,7 @@ def surrogate_bench(steps, seeds, capture_steps, rows_total, train_steps, stress
         if control:
             c = stress["control"]
             rc = np.linalg.norm(a - c, axis=0) / np.maximum(np.linalg.norm(a, axis=0), mat["yield_stress"])
-            extra["order_control"] = {"what": "exact map, tile summation order, against the exact cell-order run",
+            extra["order_control"] = {"what": "exact map, tile summation order, against the exact pair-order run",
                                       "stress_rel_error_max": float(rc.max()),
                                       "stress_rel_error_mean": float(rc.mean()),
                                       "stress_rel_error_l2": float(np.linalg.norm(a - c) / np.linalg.norm(a))}
@@ -266,7 +273,7 @@ def _run_curve(scene, surrogate, steps, every, material=0):
     its console output out of d_Runtime_Total."""
     import numpy as np
     from code_mpm_amd import Engine
-    eng = Engine(device=0, order="cell")
+    eng = Engine(device=0, order="pair")
     eng.load_scene(scene)
     if surrogate is not None:
         eng.set_surrogate(surrogate, material=material, residual_tolerance=1e-4)
@@ -378,7 +385,7 @@ def beso_bench(n_designs):
     the GPU smoothing filter and the soft-kill / restore pass per design."""
     from code_mpm_amd import beso, capi
     designs = [beso.Design(beso.l_problem(), radius=0.0012 * (1.0 + 0.25 * k)) for k in range(n_designs)]
-    run = beso.BESO(designs, order="cell")
+    run = beso.BESO(designs, order="pair")
     t = time.perf_counter()
     lines = run.step()
     el = time.perf_counter() - t
@@ -420,8 +427,8 @@ def main():
                     help="skip the second timed window at steps 1951-2000 of the crush")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--order", default="cell", choices=["tile", "cell", "particle"],
-                    help="node summation order (tile, cell: fast; particle: the reference's one-thread order)")
+    ap.add_argument("--order", default="pair", choices=["tile", "cell", "pair", "particle"],
+                    help="node summation order (cell, pair, tile: fast; particle: the reference's one-thread order)")
     ap.add_argument("--variant-steps", type=int, default=10,
                     help="timed steps of the contact and single-pass loops on the same state (0: skip)")
     ap.add_argument("--batch-instances", type=int, default=256,
@@ -542,7 +549,7 @@ def main():
         print(f"# late window (steps {c0 + 1}-{c0 + 50}): {late['ms_per_step']:.3f} ms/step, plastic fraction "
               f"{late['plastic_fraction']}", file=sys.stderr, flush=True)
     kernels = KERNELS[args.order]
-    measured = read_traffic(args.traffic_file) if args.order == "cell" else {}
+    measured = read_traffic(args.traffic_file)
     per_kernel = {}
     for k, ms in avg_ms.items():
         if kernels[k] is None:

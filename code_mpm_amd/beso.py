@@ -324,7 +324,7 @@ class BESO:
     of (strain energy, maximum-history strain energy) per scene) and
     `smooth` (see Canvas2D.filterObjective_Smooth) replace the GPU steps."""
 
-    def __init__(self, designs, order="cell", scene_kw=None, device=0, use_graph=True, simulate=None, smooth=None):
+    def __init__(self, designs, order="pair", scene_kw=None, device=0, use_graph=True, simulate=None, smooth=None):
         self.designs = list(designs)
         self.simulate = simulate
         self.smooth = smooth

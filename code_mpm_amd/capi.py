@@ -28,7 +28,7 @@ NF = dict(MASS=(0, 1), MOMENTUM=(1, 3), VELOCITY=(2, 3), FORCE=(3, 3), FORCE_TEM
 MON = dict(DISPLACEMENT=0, MOMENTUM=1, ENERGY=2, STRESS=3)
 VARIANT = dict(doublepass=0, doublepass_contact=1, singlepass=2)
 STRAIN = dict(small=0, large=1)
-ORDER = dict(particle=0, cell=1, tile=2)
+ORDER = dict(particle=0, cell=1, tile=2, pair=3)
 KSLOTS = ("bin", "p2g_partials", "p2g_nodes", "g2p_velocity", "remap_partials", "remap_nodes", "g2p_kinematics",
           "g2p_constitutive")
 PHASES = ("bin", "p2g_grid", "g2p_velocity", "remap_grid", "g2p_strain")

@@ -124,7 +124,7 @@ struct mpm_engine {
     int tile[3] = {8, 8, 4};
 
     /* node summation order and the cell-order partial sums */
-    int order = ORDER_CELL;
+    int order = ORDER_PAIR;
     int variant = VAR_DP;
     int strain = STRAIN_SMALL;
     uint8_t *dtflag = nullptr; /* node tile flags, see Batch::tflag */
@@ -217,15 +217,16 @@ struct mpm_engine {
         parts_cap = 0;
         occ_cap = 0;
     }
-    /* The occupied-cell list written by every scan: occupied cells never
-     * exceed min(cells, particles). */
+    /* The occupied-cell list written by every scan, and the particle start
+     * of each occupied cell in rank order (one more entry: the total):
+     * occupied cells never exceed min(cells, particles). */
     int64_t occ_cap = 0;
     int ensure_occ() {
         int64_t cap = std::max<int64_t>(1, std::min<int64_t>(ncells, n));
         if (cap == occ_cap && bocc) return MPM_OK;
         dfree(bocc);
         occ_cap = 0;
-        int rc = hip(hipMalloc(&bocc, 4 * (size_t)cap), "alloc occupied cells");
+        int rc = hip(hipMalloc(&bocc, 4 * (2 * (size_t)cap + 1)), "alloc occupied cells");
         if (rc) return rc;
         occ_cap = cap;
         drop_graphs();
@@ -343,6 +344,7 @@ struct mpm_engine {
         BView B;
         B.key = bkey; B.count = bcount; B.start = bstart; B.sorted = bsorted;
         B.slot = bslot; B.occ = bocc; B.nocc = bnocc;
+        B.ostart = bocc ? bocc + occ_cap : nullptr;
         B.necells = ncells;
         return B;
     }
@@ -364,7 +366,7 @@ struct mpm_engine {
         b.strain = strain;
         b.fext = has_fext ? 1 : 0;
         b.nmat = mt.n;
-        b.tflag = order == ORDER_CELL ? dtflag : nullptr;
+        b.tflag = order == ORDER_CELL || order == ORDER_PAIR ? dtflag : nullptr;
         b.stile_off = dstile_off;
         b.nstiles = nstiles;
         b.sflag = dsflag;
@@ -560,7 +562,7 @@ struct mpm_engine {
         if ((rc = upload_table())) return rc;
         if (sur && (rc = ensure_queue())) return rc;
         if ((rc = ensure_occ())) return rc;
-        if (order == ORDER_CELL && (rc = ensure_parts())) return rc;
+        if ((order == ORDER_CELL || order == ORDER_PAIR) && (rc = ensure_parts())) return rc;
         if (order == ORDER_PARTICLE && (rc = ensure_list())) return rc;
         if (order == ORDER_TILE && (rc = ensure_stiles())) return rc;
         return ensure_keys();
@@ -1263,11 +1265,14 @@ int mpm_get_variant(const mpm_engine *e, int32_t *variant) {
 int mpm_set_summation_order(mpm_engine *e, int32_t order) {
     if (e && e->phase_step >= 0) return e->fail(MPM_ERR_STATE, "a step taken phase by phase is open");
     CHECK_E(e);
-    if (order != MPM_ORDER_PARTICLE && order != MPM_ORDER_CELL && order != MPM_ORDER_TILE)
+    if (order != MPM_ORDER_PARTICLE && order != MPM_ORDER_CELL && order != MPM_ORDER_TILE && order != MPM_ORDER_PAIR)
         return e->fail(MPM_ERR_ARG, "unknown order %d", order);
     (void)hipStreamSynchronize(e->stream);
     e->drop_graphs();
-    e->order = order == MPM_ORDER_CELL ? ORDER_CELL : (order == MPM_ORDER_TILE ? ORDER_TILE : ORDER_PARTICLE);
+    e->order = order == MPM_ORDER_CELL   ? ORDER_CELL
+               : order == MPM_ORDER_TILE ? ORDER_TILE
+               : order == MPM_ORDER_PAIR ? ORDER_PAIR
+                                         : ORDER_PARTICLE;
     return MPM_OK;
 }
 

@@ -85,12 +85,13 @@ struct BView {
     uint32_t *sorted;
     uint32_t *slot;    /* necells, valid for occupied cells */
     uint32_t *occ;     /* up to min(necells, particles) */
+    uint32_t *ostart;  /* start of each occupied cell (by rank), then the total */
     uint32_t *nocc;
     int64_t necells;
 };
 
 /* Node summation order. */
-enum { ORDER_PARTICLE = 0, ORDER_CELL = 1, ORDER_TILE = 2 };
+enum { ORDER_PARTICLE = 0, ORDER_CELL = 1, ORDER_TILE = 2, ORDER_PAIR = 3 };
 
 /* Tile order: binning cells (e = cell + 1 per axis) grouped in summation
  * tiles of ST_X x ST_Y x ST_Z; a tile's node box is the (ST_X + 1) x

@@ -282,20 +282,26 @@ __global__ __launch_bounds__(BLOCK) void k_scan_blocks(uint64_t *block_sum, int6
 }
 
 __device__ __forceinline__ void scan_emit(uint32_t c, int64_t cell, uint64_t &ex, uint32_t &st, uint32_t &sl,
-                                          uint32_t *occ) {
+                                          uint32_t *occ, uint32_t *ostart) {
     st = (uint32_t)ex;
     const uint32_t r = (uint32_t)(ex >> 32);
     sl = c ? r : MPM_NONE;
-    if (c) occ[r] = (uint32_t)cell;
+    if (c) {
+        occ[r] = (uint32_t)cell;
+        ostart[r] = st;
+    }
     ex += pack_count(c);
 }
 
-/* Cell starts, slots and the occupied-cell list.  The last thread of the
- * last block also writes start[padded] and the occupied count, which are
- * the grand totals (start[necells] equals them too, padding cells being
- * empty). */
+/* Cell starts, slots, the occupied-cell list and the occupied cells' starts
+ * by rank (so that a group of consecutive occupied cells finds its particle
+ * ranges with one load each, not a cell lookup then a start lookup).  The
+ * last thread of the last block also writes start[padded], ostart[nocc]
+ * and the occupied count, which are the grand totals (start[necells]
+ * equals them too, padding cells being empty). */
 __global__ __launch_bounds__(BLOCK) void k_scan_apply(const uint32_t *count, const uint64_t *block_sum,
-                                                     uint32_t *start, uint32_t *slot, uint32_t *occ, uint32_t *nocc) {
+                                                     uint32_t *start, uint32_t *slot, uint32_t *occ, uint32_t *ostart,
+                                                     uint32_t *nocc) {
     __shared__ uint64_t lds[SCAN_SUB][4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t tile = (int64_t)blockIdx.x * SCAN_TILE;
@@ -338,22 +344,23 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(const uint32_t *count, con
         uint64_t ex = off + before + x[q] - s[q];
         const int64_t cell = tile + (int64_t)q * BLOCK * 4 + (int64_t)threadIdx.x * 4;
         uint4 st, sl;
-        scan_emit(c[q].x, cell + 0, ex, st.x, sl.x, occ);
-        scan_emit(c[q].y, cell + 1, ex, st.y, sl.y, occ);
-        scan_emit(c[q].z, cell + 2, ex, st.z, sl.z, occ);
-        scan_emit(c[q].w, cell + 3, ex, st.w, sl.w, occ);
+        scan_emit(c[q].x, cell + 0, ex, st.x, sl.x, occ, ostart);
+        scan_emit(c[q].y, cell + 1, ex, st.y, sl.y, occ, ostart);
+        scan_emit(c[q].z, cell + 2, ex, st.z, sl.z, occ, ostart);
+        scan_emit(c[q].w, cell + 3, ex, st.w, sl.w, occ, ostart);
         st4[q * BLOCK] = st;
         sl4[q * BLOCK] = sl;
         off += sub;
     }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == BLOCK - 1) {
         start[tile + SCAN_TILE] = (uint32_t)off;
+        ostart[off >> 32] = (uint32_t)off;
         *nocc = (uint32_t)(off >> 32);
     }
 }
 
-__device__ __forceinline__ void sort_cell(const BView &B, uint32_t c) {
-    uint32_t s = B.start[c], e = B.start[c + 1];
+__device__ __forceinline__ void sort_cell(const BView &B, uint32_t t) {
+    uint32_t s = B.ostart[t], e = B.ostart[t + 1];
     if (e - s < 2) return;
     for (uint32_t i = s + 1; i < e; i++) {
         uint32_t v = B.sorted[i];
@@ -389,8 +396,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PView P, BView B) {
 /* Ascending ids inside each occupied cell: the order every node sum uses. */
 __global__ __launch_bounds__(BLOCK) void k_cell_sort(BView B) {
     const uint32_t nocc = *B.nocc;
-    for (uint32_t t = blockIdx.x * BLOCK + threadIdx.x; t < nocc; t += gridDim.x * BLOCK)
-        sort_cell(B, B.occ[t]);
+    for (uint32_t t = blockIdx.x * BLOCK + threadIdx.x; t < nocc; t += gridDim.x * BLOCK) sort_cell(B, t);
 }
 /* ---- 8-way merge of a node's adjacent cell lists in ascending id order ---- */
 
@@ -923,9 +929,21 @@ __device__ __forceinline__ void particle_factors(const GridGeom &g, const double
  * particles' terms at its corner node in C.list instead of summing them;
  * k_p2g_list / k_remap_list then add them per node in ascending particle
  * id.  The remap stores terms only at nodes above the mass minimum, the
- * only ones that add them. */
-template <bool REMAP, bool MULTI, bool FEXT = false, bool LIST = false>
-__global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NView Nd, BView B, CellParts C) {
+ * only ones that add them.
+ *
+ * PAIR (the pair order): a block's groups take 31 occupied cells and the
+ * next occupied cell as a halo, so that the +x neighbour of every cell a
+ * group owns is in the group when it is occupied.  After the partial sums,
+ * the lane of a cell's +x corner adds the partial of its +x neighbour's
+ * matching -x corner (the same node) to its own, lower cell first, as the
+ * pair order adds them: from the lane 4 above it in the same wave, or, for
+ * the last cell of a wave, through LDS from the first cell of the next wave.
+ * Its record (corner ox = 1) then holds the pair sum; a -x corner writes
+ * its own record only when its -x neighbour cell is empty (else that
+ * cell's record holds the pair).  The halo cell writes nothing (the next
+ * group owns it). */
+template <bool REMAP, bool MULTI, bool FEXT = false, bool LIST = false, bool PAIR = false>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(REMAP && PAIR ? 6 : 1))) void k_cell_partials(Batch bt, PView P, NView Nd, BView B, CellParts C) {
     /* staged per particle: m v (3) and either the shape factors s (6, remap)
      * or x (3, P2G), then for P2G m, V, sigma (6) and, with FEXT, f_ext (3);
      * remap rows padded by one double so that the corner-dependent rows of
@@ -941,10 +959,22 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
      * gradient quotients */
     __shared__ double xs_s[REMAP ? 1 : PART_CELLS][6], xs_q[REMAP ? 1 : PART_CELLS][12];
     __shared__ uint8_t xs_b[REMAP ? 1 : PART_CELLS][6];
+    /* PAIR remap: the -x corner partials a wave's first cell passes to the
+     * wave below (P2G passes them in its own wave's rows of xs_q) */
+    __shared__ double xw[REMAP && PAIR ? BLOCK / 64 : 1][4][3];
     const uint32_t nocc = *B.nocc;
     const size_t n = P.n;
     const int lc = threadIdx.x >> 3, corner = threadIdx.x & 7;
     const int ox = corner >> 2, oy = (corner >> 1) & 1, oz = corner & 1;
+    /* remap, one instance: its row read once, in uniform control flow and
+     * ahead of every store (scalar registers); read per group and lane it is
+     * a chain of vector loads that each wait on their own (0.295 -> 0.247 ms
+     * on the 8.0 M-particle crush).  The P2G pass keeps the per-group reads:
+     * with the row held, it needs more vector registers than five waves per
+     * SIMD allow (0.440 -> 0.464 ms). */
+    Inst I0;
+    if (!MULTI && REMAP) I0 = bt.inst[0];
+    auto inst_of = [&](uint32_t id) -> const Inst & { return (MULTI || !REMAP) ? inst_p<MULTI>(bt, id) : I0; };
     /* P2G sharing: the axis-shape pair of lanes 0..5, and the (type, index)
      * of the gradient quotients each lane computes (type 0 x, 1 y, 2 z; -1
      * none); every quotient is computed by a lane that holds its factors */
@@ -956,28 +986,42 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
     const int qk1 = qindex(qt1), qk2 = qindex(qt2);
     const bool skip_dc = REMAP ? bt.variant != VAR_DP : bt.variant == VAR_SP;
     constexpr bool fext = SF;
-    for (uint32_t g0 = blockIdx.x * PART_CELLS; g0 < nocc; g0 += gridDim.x * PART_CELLS) {
+    /* cells a group owns (PAIR: one fewer, the last lane group is a halo) */
+    constexpr uint32_t GS = PAIR ? PART_CELLS - 1 : PART_CELLS;
+    for (uint32_t g0 = blockIdx.x * GS; g0 < nocc; g0 += gridDim.x * GS) {
         const uint32_t g1 = min(g0 + (uint32_t)PART_CELLS, nocc);
-        const uint32_t r0 = B.start[B.occ[g0]], r1 = B.start[B.occ[g1 - 1] + 1];
+        const uint32_t r0 = B.ostart[g0], r1 = B.ostart[g1];
         /* this lane's (cell, corner) */
         const uint32_t slot = g0 + (uint32_t)lc;
         const bool cell = slot < g1;
         bool have = cell;
+        /* PAIR: the cell's -x / +x neighbour cells are occupied (the +x one
+         * then sits in the next lane group) */
+        bool xprev = false, xnext = false;
         uint32_t s0 = 0, s1 = 0;
         double mn = 0.0, xpair = 0.0, hpair = 1.0, hx_ = 1.0, hy_ = 1.0, hz_ = 1.0;
         bool massive = false;
-        if (cell) {
-            const uint32_t c = B.occ[slot];
-            s0 = B.start[c];
-            s1 = B.start[c + 1];
-            const Inst &I = inst_p<MULTI>(bt, B.sorted[s0]);
+        {
+            /* every lane reads a valid rank, so the loads below are not
+             * under a branch and issue together (a lane past the group's
+             * end only has have = false) */
+            const uint32_t sr = cell ? slot : g0;
+            const uint32_t c = B.occ[sr];
+            const uint32_t cprev = B.occ[sr > 0 ? sr - 1 : sr], cnext = B.occ[sr + 1 < g1 ? sr + 1 : sr];
+            s0 = B.ostart[sr];
+            s1 = B.ostart[sr + 1];
+            const Inst &I = inst_of(MULTI ? B.sorted[s0] : 0u);
             const GridGeom &g = I.g;
             const uint32_t e = c - (uint32_t)I.cell_off;
             const int ex = (int)(e % (uint32_t)g.ec[0]);
             const int ey = (int)((e / (uint32_t)g.ec[0]) % (uint32_t)g.ec[1]);
             const int ez = (int)(e / ((uint32_t)g.ec[0] * (uint32_t)g.ec[1]));
             const int ix = ex - 1 + ox, iy = ey - 1 + oy, iz = ez - 1 + oz;
-            if (ix < 0 || iy < 0 || iz < 0 || ix >= g.nn[0] || iy >= g.nn[1] || iz >= g.nn[2]) have = false;
+            have = cell & !((ix < 0) | (iy < 0) | (iz < 0) | (ix >= g.nn[0]) | (iy >= g.nn[1]) | (iz >= g.nn[2]));
+            if (PAIR) {
+                xprev = cell & (ex > 0) & (sr > 0) & (cprev == c - 1u);
+                xnext = cell & (ex < g.ec[0] - 1) & (sr + 1 < g1) & (cnext == c + 1u);
+            }
             if (!REMAP) {
                 /* the node coordinate of this lane's shape pair: axis pa,
                  * the cell's node pa-index e - 1 + po */
@@ -1021,7 +1065,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
                     }
                 } else {
                     double sh[3][2];
-                    particle_factors(inst_p<MULTI>(bt, id).g, x, sh);
+                    particle_factors(inst_of(id).g, x, sh);
 #pragma unroll
                     for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -1114,6 +1158,52 @@ __global__ __launch_bounds__(BLOCK) void k_cell_partials(Batch bt, PView P, NVie
                     f[1] += N * spv[1][j] / mn;
                     f[2] += N * spv[2][j] / mn;
                 }
+            }
+        }
+        if (PAIR) {
+            /* the partial of the +x neighbour's matching -x corner: lane + 4
+             * (cell lc + 1, corner - 4) in the same wave; for a wave's last
+             * cell, the next wave's first cell hands it over through LDS
+             * (P2G: that wave's own rows of xs_q, which no other wave reads
+             * in the particle loop; the wave has left its loop) */
+            constexpr int NV = REMAP ? 3 : 7;
+            double mine[NV], nb[NV];
+            if (!REMAP) {
+                mine[0] = m;
+                mine[1] = p[0];
+                mine[2] = p[1];
+                mine[3] = p[2];
+            }
+            mine[NV - 3] = f[0];
+            mine[NV - 2] = f[1];
+            mine[NV - 1] = f[2];
+#pragma unroll
+            for (int v = 0; v < NV; v++) nb[v] = __shfl_down(mine[v], 4);
+            double *xwv = REMAP ? &xw[0][0][0] : &xs_q[0][0];
+            /* wave w's slot: P2G rows 8w .. 8w + 2 of xs_q (36 doubles); remap xw[w] */
+            auto xslot = [&](int w, int c4) { return xwv + (REMAP ? (w * 4 + c4) * 3 : (w * 8 * 12 + c4 * 7)); };
+            if ((lc & 7) == 0 && lc > 0 && ox == 0 && cell) {
+                double *d = xslot(lc >> 3, corner & 3);
+#pragma unroll
+                for (int v = 0; v < NV; v++) d[v] = mine[v];
+            }
+            __syncthreads();
+            if ((lc & 7) == 7 && lc < PART_CELLS - 1 && ox == 1 && xnext) {
+                const double *d = xslot((lc >> 3) + 1, corner & 3);
+#pragma unroll
+                for (int v = 0; v < NV; v++) nb[v] = d[v];
+            }
+            if (lc == PART_CELLS - 1 || (ox == 0 && xprev)) continue;
+            if (ox == 1 && xnext && (REMAP ? massive : true)) {
+                if (!REMAP) {
+                    m = m + nb[0];
+                    p[0] = p[0] + nb[1];
+                    p[1] = p[1] + nb[2];
+                    p[2] = p[2] + nb[3];
+                }
+                f[0] = f[0] + nb[NV - 3];
+                f[1] = f[1] + nb[NV - 2];
+                f[2] = f[2] + nb[NV - 1];
             }
         }
         if (!have || LIST) continue;
@@ -1241,9 +1331,12 @@ __global__ __launch_bounds__(BLOCK) void k_remap_list(Batch bt, PView P, NView N
 
 /* Node side of the cell order: adds the 8 partials in neighbour order
  * k = 0..7 (x outer, z inner), then the same grid update as the particle
- * order (P2G) or the same remap finish (REMAP). */
-template <bool REMAP, bool MULTI>
-__global__ __launch_bounds__(BLOCK) void k_node_cells(Batch bt, PView P, NView Nd, BView B, CellParts C) {
+ * order (P2G) or the same remap finish (REMAP).  PAIR (the pair order):
+ * adds 4 records, j = 0..3: the pair sum the lower x cell k = j holds at
+ * its +x corner, or, when that cell is empty, the upper cell k = j + 4's
+ * own partial at its -x corner. */
+template <bool REMAP, bool MULTI, bool PAIR = false>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PAIR ? (REMAP ? 6 : 5) : 1))) void k_node_cells(Batch bt, PView P, NView Nd, BView B, CellParts C) {
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     /* tiles with no corner of an occupied cell: all their nodes are inactive
      * (the remap pass also re-arms the flag for the next step, after every
@@ -1295,7 +1388,45 @@ __global__ __launch_bounds__(BLOCK) void k_node_cells(Batch bt, PView P, NView N
     }
     /* all partial records first (an empty cell reads record 0 and adds
      * nothing), then the sums in neighbour order */
-    if (!REMAP) {
+    if (PAIR) {
+        constexpr int RW = REMAP ? 4 : 8; /* doubles per record */
+        double2 ra[4], rb[4], rc[4], rd[4];
+        bool o[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t lo = sl[j], hi = sl[j + 4];
+            o[j] = lo != MPM_NONE || hi != MPM_NONE;
+            const size_t t = lo != MPM_NONE ? (size_t)(7 - j) * (size_t)C.cap + (size_t)(lo & ~SLOT_DC)
+                                            : (hi != MPM_NONE ? (size_t)(3 - j) * (size_t)C.cap + (size_t)(hi & ~SLOT_DC)
+                                                              : 0);
+            const double2 *q = reinterpret_cast<const double2 *>(C.w + t * RW);
+            ra[j] = q[0];
+            rb[j] = q[1];
+            if (!REMAP) {
+                rc[j] = q[2];
+                rd[j] = q[3];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (!REMAP) {
+                const double t0 = m + ra[j].x, t1 = p[0] + ra[j].y, t2 = p[1] + rb[j].x, t3 = p[2] + rb[j].y;
+                const double t4 = f[0] + rc[j].x, t5 = f[1] + rc[j].y, t6 = f[2] + rd[j].x;
+                m = o[j] ? t0 : m;
+                p[0] = o[j] ? t1 : p[0];
+                p[1] = o[j] ? t2 : p[1];
+                p[2] = o[j] ? t3 : p[2];
+                f[0] = o[j] ? t4 : f[0];
+                f[1] = o[j] ? t5 : f[1];
+                f[2] = o[j] ? t6 : f[2];
+            } else {
+                const double t0 = f[0] + ra[j].x, t1 = f[1] + ra[j].y, t2 = f[2] + rb[j].x;
+                f[0] = o[j] ? t0 : f[0];
+                f[1] = o[j] ? t1 : f[1];
+                f[2] = o[j] ? t2 : f[2];
+            }
+        }
+    } else if (!REMAP) {
         double2 ra[8], rb[8], rc[8], rd[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
@@ -3061,7 +3192,7 @@ hipError_t launch_bin(const PView &P, const BView &B, uint64_t *block_sum, hipSt
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(BLOCK), 0, s, B.count, block_sum);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(BLOCK), 0, s, block_sum, nb);
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(BLOCK), 0, s, B.count, block_sum, B.start, B.slot,
-                       B.occ, B.nocc);
+                       B.occ, B.ostart, B.nocc);
     if (P.n > 0) hipLaunchKernelGGL(k_scatter, dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, P, B);
     hipLaunchKernelGGL(k_cell_sort, dim3((unsigned)std::min<int64_t>(blocks_for(std::max<int64_t>(P.n, 1), BLOCK), 4096)),
                        dim3(BLOCK), 0, s, B);
@@ -3161,6 +3292,17 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
             }
             if (mid) (void)hipEventRecord(mid, s);
             LAUNCH_MULTI(bt, (k_node_cells<false, M>), dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
+        } else if (order == ORDER_PAIR) {
+            if (P.n > 0) {
+                if (bt.fext)
+                    LAUNCH_MULTI(bt, (k_cell_partials<false, M, true, false, true>), dim3(partial_grid(P.n)),
+                                 dim3(BLOCK), 0, s, bt, P, N, B, C);
+                else
+                    LAUNCH_MULTI(bt, (k_cell_partials<false, M, false, false, true>), dim3(partial_grid(P.n)),
+                                 dim3(BLOCK), 0, s, bt, P, N, B, C);
+            }
+            if (mid) (void)hipEventRecord(mid, s);
+            LAUNCH_MULTI(bt, (k_node_cells<false, M, true>), dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
         } else {
             if (!C.list) return hipErrorInvalidValue;
             if (P.n > 0) {
@@ -3201,6 +3343,13 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
             }
             if (mid) (void)hipEventRecord(mid, s);
             LAUNCH_MULTI(bt, (k_node_cells<true, M>), dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
+        } else if (order == ORDER_PAIR) {
+            if (P.n > 0) {
+                LAUNCH_MULTI(bt, (k_cell_partials<true, M, false, false, true>), dim3(partial_grid(P.n)),
+                             dim3(BLOCK), 0, s, bt, P, N, B, C);
+            }
+            if (mid) (void)hipEventRecord(mid, s);
+            LAUNCH_MULTI(bt, (k_node_cells<true, M, true>), dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
         } else {
             if (!C.list) return hipErrorInvalidValue;
             if (P.n > 0)

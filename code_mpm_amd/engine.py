@@ -52,7 +52,7 @@ def console_line(m, runtime, description=""):
 class Engine:
     """One simulation on one GPU (one HIP stream)."""
 
-    def __init__(self, device=0, order="cell", variant="doublepass"):
+    def __init__(self, device=0, order="pair", variant="doublepass"):
         self._lib = capi.load()
         h = ctypes.c_void_p()
         rc = self._lib.mpm_create(ctypes.byref(h), int(device))
@@ -73,7 +73,9 @@ class Engine:
 
     def set_order(self, order):
         """Node summation order: "particle" (the reference's one-thread order,
-        bit for bit) or "cell" (per-cell partial sums in a fixed order; fast)."""
+        bit for bit), or a grouped order of per-cell partial sums in a fixed
+        association: "pair" (default; the two cells along x paired first),
+        "cell" or "tile" (the same terms, rounding differently; fast)."""
         self._check(self._lib.mpm_set_summation_order(self._h, capi.ORDER[order]), "set_order")
         self.order = order
 

@@ -270,7 +270,7 @@ class PhysicsEngine {
     double d_TimeConsole_Last = 0.0;
     double d_DampingCoefficient = 0.0;
     double d_Mass_Minimum = 0.0;
-    int32_t i_SummationOrder = MPM_ORDER_CELL;
+    int32_t i_SummationOrder = MPM_ORDER_PAIR;
     double d_Runtime_Total = 0.0;     /* PhysicsEngine.h:116, wall seconds of the steps */
     std::string str_Log_FileName;     /* PhysicsEngine.h:82; empty: no log file */
     bool b_Console_Print = true;

@@ -201,7 +201,7 @@ def capture_from_runs(seeds, steps, capacity_per_run, scene_kw=None, device=0, l
     out, info = [], []
     for seed in seeds:
         sc = scenes.honeycomb_crush(seed=seed, steps=steps, **(scene_kw or {}))
-        eng = Engine(device=device, order="cell")
+        eng = Engine(device=device, order="pair")
         eng.load_scene(sc)
         eng.set_sample_capture(capacity_per_run)
         done, chunk = 0, max(1, steps // 20)

@@ -213,14 +213,14 @@ int mpm_get_instance_monitors(mpm_engine *e, int32_t instance, mpm_monitors *out
 
 /* --- node summation order -------------------------------------------------- */
 
-/* All three orders are deterministic and use no floating-point atomics.
+/* All four orders are deterministic and use no floating-point atomics.
  * MPM_ORDER_PARTICLE: each node adds its particles' contributions in
  *   ascending particle index, the order of the reference's scatters run by
  *   one thread (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:48-89,
  *   539-564); results equal that run bit for bit.  Steps in this order
  *   hold a device buffer of 640 bytes per particle (each particle's terms
  *   at its 8 corner nodes).
- * MPM_ORDER_CELL (default): each node adds 8 per-cell partial sums in a
+ * MPM_ORDER_CELL: each node adds 8 per-cell partial sums in a
  *   fixed neighbour order, each partial taken over the cell's particles in
  *   ascending index; the same terms in another association, so results
  *   differ from the particle order only by rounding, and much faster on the
@@ -234,8 +234,15 @@ int mpm_get_instance_monitors(mpm_engine *e, int32_t instance, mpm_monitors *out
  *   The same terms in another association again; on the GPU a tile's node
  *   sums are accumulated in on-chip memory and only the sums of nodes on a
  *   tile face pass through device memory.  Steps in this order hold a
- *   device buffer of 16.1 KB per summation tile. */
-enum { MPM_ORDER_PARTICLE = 0, MPM_ORDER_CELL = 1, MPM_ORDER_TILE = 2 };
+ *   device buffer of 16.1 KB per summation tile.
+ * MPM_ORDER_PAIR (default): as the cell order, but each node first adds the partial
+ *   sums of its two neighbour cells that differ only along x (lower x cell
+ *   first), then adds the four pair sums in the cell order's neighbour
+ *   order of their lower cells.  On the GPU the cells of a block are
+ *   consecutive along x, so a pair's two partials meet in on-chip memory
+ *   and a run of occupied cells along x passes one record per node and
+ *   (y, z) corner through device memory instead of two. */
+enum { MPM_ORDER_PARTICLE = 0, MPM_ORDER_CELL = 1, MPM_ORDER_TILE = 2, MPM_ORDER_PAIR = 3 };
 #define MPM_SUM_TILE_X 8
 #define MPM_SUM_TILE_Y 8
 #define MPM_SUM_TILE_Z 4

@@ -543,7 +543,9 @@ struct oracle_engine {
      * z inner; binning cell = node + k per axis), each cell its own group.
      * Tile order: the cells grouped by summation tile (MPM_SUM_TILE_X x Y x Z
      * binning cells) in ascending tile index, and within a tile in parity
-     * class order, class = 4 (e_x & 1) + 2 (e_y & 1) + (e_z & 1). */
+     * class order, class = 4 (e_x & 1) + 2 (e_y & 1) + (e_z & 1).  Pair
+     * order: cells k and k + 4 (the lower and upper x cell of the same y, z
+     * offsets) form a group, lower first, and the groups follow k = 0..3. */
     void neighbour_sequence(long ni, long cell[8], bool last_of_group[8]) const {
         int ix = (int)(ni % nn[0]), iy = (int)((ni / nn[0]) % nn[1]), iz = (int)(ni / ((long)nn[0] * nn[1]));
         long key[8];
@@ -557,6 +559,8 @@ struct oracle_engine {
                             ntx * ((long)(e[1] / MPM_SUM_TILE_Y) + (long)nty * (e[2] / MPM_SUM_TILE_Z));
                 int cls = 4 * (e[0] & 1) + 2 * (e[1] & 1) + (e[2] & 1);
                 key[k] = tile * 8 + cls;
+            } else if (order == MPM_ORDER_PAIR) {
+                key[k] = 2 * (k & 3) + (k >> 2);
             } else {
                 key[k] = k;
             }
@@ -568,7 +572,8 @@ struct oracle_engine {
                 std::swap(cell[b - 1], cell[b]);
             }
         for (int k = 0; k < 8; k++)
-            last_of_group[k] = order != MPM_ORDER_TILE || k == 7 || key[k] / 8 != key[k + 1] / 8;
+            last_of_group[k] = order == MPM_ORDER_PAIR ? (k & 1) != 0
+                                                       : (order != MPM_ORDER_TILE || k == 7 || key[k] / 8 != key[k + 1] / 8);
     }
 
     void m2g_cells(bool skip_dc = false) {
@@ -1453,7 +1458,8 @@ int oracle_set_damping(oracle_engine *e, double damping, double mass_min) {
 }
 
 int oracle_set_order(oracle_engine *e, int32_t order) {
-    if (!e || (order != MPM_ORDER_PARTICLE && order != MPM_ORDER_CELL && order != MPM_ORDER_TILE))
+    if (!e || (order != MPM_ORDER_PARTICLE && order != MPM_ORDER_CELL && order != MPM_ORDER_TILE &&
+               order != MPM_ORDER_PAIR))
         return MPM_ERR_ARG;
     e->order = order;
     return MPM_OK;

@@ -99,7 +99,7 @@ class OracleEngine:
         h = ctypes.c_void_p()
         _ok(self._lib.oracle_create(ctypes.byref(h)), "create")
         self._h = h
-        _ok(self._lib.oracle_set_order(self._h, {"particle": 0, "cell": 1, "tile": 2}[order]), "order")
+        _ok(self._lib.oracle_set_order(self._h, {"particle": 0, "cell": 1, "tile": 2, "pair": 3}[order]), "order")
         self.set_variant(variant)
         self.threads = int(threads)
         self.n = 0

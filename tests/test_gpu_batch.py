@@ -17,7 +17,7 @@ def _scenes():
     return [a, b, c]
 
 
-@pytest.mark.parametrize("order", ["particle", "tile"])
+@pytest.mark.parametrize("order", ["particle", "tile", "pair"])
 @pytest.mark.parametrize("variant", ["doublepass", "doublepass_contact", "singlepass"])
 def test_batch_instances_match_separate_runs_and_oracle(variant, order):
     from code_mpm_amd import Engine
@@ -42,7 +42,7 @@ def test_batch_instances_match_separate_runs_and_oracle(variant, order):
         assert batch.instance_time(k) == alone.getTime_Current()
 
 
-@pytest.mark.parametrize("order", ["cell", "tile"])
+@pytest.mark.parametrize("order", ["cell", "tile", "pair"])
 @pytest.mark.parametrize("variant", ["doublepass", "singlepass"])
 def test_graph_replay_matches_eager(variant, order):
     from code_mpm_amd import Engine, scenes
@@ -59,7 +59,7 @@ def test_graph_replay_matches_eager(variant, order):
         assert eager.monitors() == graph.monitors()
 
 
-@pytest.mark.parametrize("order", ["cell", "tile"])
+@pytest.mark.parametrize("order", ["cell", "tile", "pair"])
 def test_batch_graph_replay_matches_eager(order):
     from code_mpm_amd import Engine
     sc = _scenes()
@@ -77,7 +77,8 @@ def test_batch_graph_replay_matches_eager(order):
 
 @pytest.mark.parametrize("order,variant", [("particle", "doublepass"), ("cell", "doublepass"), ("tile", "doublepass"),
                                            ("cell", "doublepass_contact"), ("cell", "singlepass"),
-                                           ("particle", "singlepass")])
+                                           ("particle", "singlepass"), ("pair", "doublepass"),
+                                           ("pair", "doublepass_contact"), ("pair", "singlepass")])
 def test_autotuned_step_equals_ordinary_step(order, variant):
     from code_mpm_amd import Engine, scenes
     sc = scenes.honeycomb_crush(cells=(48, 48, 8), steps=100)
@@ -108,7 +109,7 @@ def _foams():
             for i, n in enumerate((10000, 25000, 14000, 18000))]
 
 
-@pytest.mark.parametrize("order", ["particle", "cell", "tile"])
+@pytest.mark.parametrize("order", ["particle", "cell", "tile", "pair"])
 def test_graded_foam_batch_matches_oracle(order):
     """Each graded-foam instance of a batch equals its own oracle run, bit
     for bit, in the loop the batched workload runs (DoublePass, perfectly

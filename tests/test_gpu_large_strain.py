@@ -11,7 +11,7 @@ PARTICLE_FIELDS = ["POSITION", "VELOCITY", "MASS", "VOLUME", "DEFORMATION_GRADIE
                    "STRAIN", "STRESS", "STRAIN_PLASTIC", "BACKSTRESS", "ENERGY_STRAIN", "ENERGY_STRAIN_MAX",
                    "ENERGY_STRAIN_TI", "ENERGY_PLASTIC"]
 NODE_FIELDS = ["MASS", "MOMENTUM", "VELOCITY", "FORCE", "FORCE_TEMP", "ACTIVE"]
-ORDERS = ["particle", "cell", "tile"]
+ORDERS = ["particle", "cell", "tile", "pair"]
 
 
 def _same(a, b):

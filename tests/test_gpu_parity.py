@@ -15,7 +15,7 @@ PARTICLE_FIELDS = ["POSITION", "VELOCITY", "MASS", "VOLUME", "DEFORMATION_GRADIE
 NODE_FIELDS = ["MASS", "MOMENTUM", "VELOCITY", "FORCE", "FORCE_TEMP", "ACTIVE"]
 
 
-ORDERS = ["particle", "cell", "tile"]
+ORDERS = ["particle", "cell", "tile", "pair"]
 
 
 def _pair(scene, order="particle", variant="doublepass"):
@@ -223,7 +223,7 @@ def _abs_scales(c, scene):
 
 
 @pytest.mark.parametrize("workload", ["elastic_bar", "honeycomb_48x48x8", "half_ring_h0.6mm"])
-@pytest.mark.parametrize("order", ["cell", "tile"])
+@pytest.mark.parametrize("order", ["cell", "tile", "pair"])
 def test_grouped_order_tracks_reference_order(order, workload):
     """The grouped summation orders (cell, and tile, the default) against the reference's
     one-thread particle order (the oracle) on the elastic bar, the J2

@@ -1,4 +1,5 @@
-"""Pins the oracle's two grouped summation orders (cell and tile) exactly.
+"""Pins the oracle's three grouped summation orders (cell, tile and pair)
+exactly.
 
 A node's mass in either order is an association of per-cell partial sums.
 A cell's partial at a node is what the node holds when that cell's
@@ -12,12 +13,16 @@ Python, by each order's definition:
   tile order: the neighbour cells grouped by summation tile (8 x 8 x 4
               binning cells) in ascending tile index, cells of a tile in
               parity-class order 4 (e_x & 1) + 2 (e_y & 1) + (e_z & 1), each
-              tile's sum formed from 0 and then added to the node.
+              tile's sum formed from 0 and then added to the node;
+  pair order: the neighbour cells k and k + 4 (lower and upper x cell of the
+              same y, z offsets), lower first, summed from 0 as a pair, the
+              pairs then added to the node in k = 0..3 order.
 
 The combined masses must equal the oracle's own runs in those orders double
 for double.  Masses spread over eight decades make the associations round
 differently, so a wrong grouping or class order fails."""
 import numpy as np
+import pytest
 
 from code_mpm_amd import scenes
 from oracle.oracle import OracleEngine
@@ -70,7 +75,7 @@ def test_grouped_orders_associate_as_defined():
         partial[int(k)] = _node_mass(_subset(sc, np.nonzero(key == k)[0]), "particle")
     nn = np.asarray(sc.cells) + 1
     ntx, nty = -(-ec[0] // TILE[0]), -(-ec[1] // TILE[1])
-    want = {"cell": np.zeros(int(np.prod(nn))), "tile": np.zeros(int(np.prod(nn)))}
+    want = {o: np.zeros(int(np.prod(nn))) for o in ("cell", "tile", "pair")}
     for node in range(int(np.prod(nn))):
         ix, iy, iz = node % nn[0], (node // nn[0]) % nn[1], node // (nn[0] * nn[1])
         nb = []
@@ -95,20 +100,33 @@ def test_grouped_orders_associate_as_defined():
                     ts = ts + p
             m = m + ts
         want["tile"][node] = m
-    got = {o: _node_mass(sc, o) for o in ("cell", "tile")}
-    for o in ("cell", "tile"):
+        by_k = {k: p for k, _, _, p in nb}
+        m = 0.0
+        for j in range(4):                                  # pair order
+            if j in by_k or j + 4 in by_k:
+                ps = 0.0
+                for k in (j, j + 4):
+                    if k in by_k:
+                        ps = ps + by_k[k]
+                m = m + ps
+        want["pair"][node] = m
+    got = {o: _node_mass(sc, o) for o in ("cell", "tile", "pair")}
+    for o in ("cell", "tile", "pair"):
         assert np.array_equal(got[o], want[o]), (o, np.nonzero(got[o] != want[o])[0][:10])
-    # the two associations do round differently here, and differ from the particle order
+    # the associations do round differently here, and differ from the particle order
     assert not np.array_equal(got["cell"], got["tile"])
+    assert not np.array_equal(got["cell"], got["pair"])
     assert not np.array_equal(got["tile"], _node_mass(sc, "particle"))
+    assert not np.array_equal(got["pair"], _node_mass(sc, "particle"))
 
 
-def test_tile_order_tracks_particle_order():
+@pytest.mark.parametrize("order", ["tile", "pair"])
+def test_grouped_order_tracks_particle_order(order):
     """Same terms, another association: every field within round-off of the
     reference's one-thread order over a few steps of a yielding block."""
     sc = scenes.random_block(seed=3, cells=(10, 9, 9), n=4000, features=False)
     a = OracleEngine(threads=1, order="particle")
-    b = OracleEngine(threads=1, order="tile")
+    b = OracleEngine(threads=1, order=order)
     a.load_scene(sc)
     b.load_scene(sc)
     a.step(sc.dt, 8)
