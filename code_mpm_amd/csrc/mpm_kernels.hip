@@ -1001,13 +1001,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(REMAP && 
         uint32_t s0 = 0, s1 = 0;
         double mn = 0.0, xpair = 0.0, hpair = 1.0, hx_ = 1.0, hy_ = 1.0, hz_ = 1.0;
         bool massive = false;
-        {
-            /* every lane reads a valid rank, so the loads below are not
-             * under a branch and issue together (a lane past the group's
-             * end only has have = false) */
-            const uint32_t sr = cell ? slot : g0;
+        /* remap: every lane reads a valid rank, so the loads below are not
+         * under a branch and issue together (a lane past the group's end
+         * only has have = false; 0.267 -> 0.254 ms).  P2G keeps the branch:
+         * the loads it adds take its registers past five waves per SIMD
+         * (0.441 -> 0.464 ms). */
+        const uint32_t sr = REMAP ? (cell ? slot : g0) : slot;
+        if (REMAP || cell) {
             const uint32_t c = B.occ[sr];
-            const uint32_t cprev = B.occ[sr > 0 ? sr - 1 : sr], cnext = B.occ[sr + 1 < g1 ? sr + 1 : sr];
             s0 = B.ostart[sr];
             s1 = B.ostart[sr + 1];
             const Inst &I = inst_of(MULTI ? B.sorted[s0] : 0u);
@@ -1018,9 +1019,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(REMAP && 
             const int ez = (int)(e / ((uint32_t)g.ec[0] * (uint32_t)g.ec[1]));
             const int ix = ex - 1 + ox, iy = ey - 1 + oy, iz = ez - 1 + oz;
             have = cell & !((ix < 0) | (iy < 0) | (iz < 0) | (ix >= g.nn[0]) | (iy >= g.nn[1]) | (iz >= g.nn[2]));
-            if (PAIR) {
+            if (PAIR && REMAP) {
+                const uint32_t cprev = B.occ[sr > 0 ? sr - 1 : sr], cnext = B.occ[sr + 1 < g1 ? sr + 1 : sr];
                 xprev = cell & (ex > 0) & (sr > 0) & (cprev == c - 1u);
                 xnext = cell & (ex < g.ec[0] - 1) & (sr + 1 < g1) & (cnext == c + 1u);
+            } else if (PAIR) {
+                xprev = ex > 0 && sr > 0 && B.occ[sr - 1] == c - 1u;
+                xnext = ex < g.ec[0] - 1 && sr + 1 < g1 && B.occ[sr + 1] == c + 1u;
             }
             if (!REMAP) {
                 /* the node coordinate of this lane's shape pair: axis pa,
