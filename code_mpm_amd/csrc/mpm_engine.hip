@@ -75,8 +75,8 @@ uint32_t tiles_of(const GridGeom &g, const int shape[3]) {
 }
 
 /* Node tile shapes the autotuner tries (all NODE_BLOCK nodes). */
-const int kTileShapes[][3] = {{8, 8, 4}, {16, 4, 4}, {4, 8, 8}, {16, 16, 1}, {32, 8, 1},
-                              {64, 4, 1}, {8, 4, 8}, {4, 4, 16}, {16, 8, 2}, {32, 4, 2}};
+const int kTileShapes[][3] = {{4, 4, 4}, {8, 4, 2}, {4, 8, 2}, {8, 8, 1}, {16, 4, 1},
+                              {4, 16, 1}, {2, 8, 4}, {16, 2, 2}, {32, 2, 1}, {64, 1, 1}};
 const int kNumTileShapes = sizeof(kTileShapes) / sizeof(kTileShapes[0]);
 
 }  // namespace
@@ -121,7 +121,7 @@ struct mpm_engine {
     bool keys_valid = false;
 
     /* node tile shape */
-    int tile[3] = {8, 8, 4};
+    int tile[3] = {32, 2, 1};
 
     /* node summation order and the cell-order partial sums */
     int order = ORDER_PAIR;

@@ -1241,7 +1241,7 @@ __device__ __forceinline__ void dc_scan_cell(DCWin &w, const PView &P, const BVi
  * ascending id, the order of the reference's one-thread M2G loop
  * (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:48-89). */
 template <bool FEXT>
-__global__ __launch_bounds__(BLOCK) void k_p2g_list(Batch bt, PView P, NView Nd, BView B, CellParts C) {
+__global__ __launch_bounds__(NODE_BLOCK) void k_p2g_list(Batch bt, PView P, NView Nd, BView B, CellParts C) {
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const Inst &I = bt.inst[inst_of_tile(bt, tile)];
     const GridGeom &g = I.g;
@@ -1296,7 +1296,7 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_list(Batch bt, PView P, NView Nd,
 
 /* The remap's node side in the particle order from the stored terms (the
  * ascending-id sums of the reference's one-thread loop). */
-__global__ __launch_bounds__(BLOCK) void k_remap_list(Batch bt, PView P, NView Nd, BView B, CellParts C) {
+__global__ __launch_bounds__(NODE_BLOCK) void k_remap_list(Batch bt, PView P, NView Nd, BView B, CellParts C) {
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const Inst &I = bt.inst[inst_of_tile(bt, tile)];
     const GridGeom &g = I.g;
@@ -1341,7 +1341,7 @@ __global__ __launch_bounds__(BLOCK) void k_remap_list(Batch bt, PView P, NView N
  * its +x corner, or, when that cell is empty, the upper cell k = j + 4's
  * own partial at its -x corner. */
 template <bool REMAP, bool MULTI, bool PAIR = false>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PAIR ? (REMAP ? 6 : 5) : 1))) void k_node_cells(Batch bt, PView P, NView Nd, BView B, CellParts C) {
+__global__ __launch_bounds__(NODE_BLOCK) __attribute__((amdgpu_waves_per_eu(PAIR ? (REMAP ? 6 : 5) : 1))) void k_node_cells(Batch bt, PView P, NView Nd, BView B, CellParts C) {
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     /* tiles with no corner of an occupied cell: all their nodes are inactive
      * (the remap pass also re-arms the flag for the next step, after every
@@ -3296,7 +3296,7 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
                                  bt, P, N, B, C);
             }
             if (mid) (void)hipEventRecord(mid, s);
-            LAUNCH_MULTI(bt, (k_node_cells<false, M>), dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
+            LAUNCH_MULTI(bt, (k_node_cells<false, M>), dim3(bt.ntiles), dim3(NODE_BLOCK), 0, s, bt, P, N, B, C);
         } else if (order == ORDER_PAIR) {
             if (P.n > 0) {
                 if (bt.fext)
@@ -3307,7 +3307,7 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
                                  dim3(BLOCK), 0, s, bt, P, N, B, C);
             }
             if (mid) (void)hipEventRecord(mid, s);
-            LAUNCH_MULTI(bt, (k_node_cells<false, M, true>), dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
+            LAUNCH_MULTI(bt, (k_node_cells<false, M, true>), dim3(bt.ntiles), dim3(NODE_BLOCK), 0, s, bt, P, N, B, C);
         } else {
             if (!C.list) return hipErrorInvalidValue;
             if (P.n > 0) {
@@ -3320,9 +3320,9 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
             }
             if (mid) (void)hipEventRecord(mid, s);
             if (bt.fext)
-                hipLaunchKernelGGL(k_p2g_list<true>, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
+                hipLaunchKernelGGL(k_p2g_list<true>, dim3(bt.ntiles), dim3(NODE_BLOCK), 0, s, bt, P, N, B, C);
             else
-                hipLaunchKernelGGL(k_p2g_list<false>, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
+                hipLaunchKernelGGL(k_p2g_list<false>, dim3(bt.ntiles), dim3(NODE_BLOCK), 0, s, bt, P, N, B, C);
         }
         break;
     case PHASE_G2P_VELOCITY:
@@ -3347,21 +3347,21 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
                              B, C);
             }
             if (mid) (void)hipEventRecord(mid, s);
-            LAUNCH_MULTI(bt, (k_node_cells<true, M>), dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
+            LAUNCH_MULTI(bt, (k_node_cells<true, M>), dim3(bt.ntiles), dim3(NODE_BLOCK), 0, s, bt, P, N, B, C);
         } else if (order == ORDER_PAIR) {
             if (P.n > 0) {
                 LAUNCH_MULTI(bt, (k_cell_partials<true, M, false, false, true>), dim3(partial_grid(P.n)),
                              dim3(BLOCK), 0, s, bt, P, N, B, C);
             }
             if (mid) (void)hipEventRecord(mid, s);
-            LAUNCH_MULTI(bt, (k_node_cells<true, M, true>), dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
+            LAUNCH_MULTI(bt, (k_node_cells<true, M, true>), dim3(bt.ntiles), dim3(NODE_BLOCK), 0, s, bt, P, N, B, C);
         } else {
             if (!C.list) return hipErrorInvalidValue;
             if (P.n > 0)
                 LAUNCH_MULTI(bt, (k_cell_partials<true, M, false, true>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s,
                              bt, P, N, B, C);
             if (mid) (void)hipEventRecord(mid, s);
-            hipLaunchKernelGGL(k_remap_list, dim3(bt.ntiles), dim3(BLOCK), 0, s, bt, P, N, B, C);
+            hipLaunchKernelGGL(k_remap_list, dim3(bt.ntiles), dim3(NODE_BLOCK), 0, s, bt, P, N, B, C);
         }
         break;
     case PHASE_G2P_STRAIN:

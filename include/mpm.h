@@ -297,8 +297,8 @@ int mpm_get_strain_formulation(const mpm_engine *e, int32_t *formulation);
 
 /* --- node tiles and the autotuner (no counterpart in the reference) -------- */
 
-/* Nodes are processed in tiles of shape[0] x shape[1] x shape[2] = 256 nodes
- * per workgroup (default 8 x 8 x 4). */
+/* Nodes are processed in tiles of shape[0] x shape[1] x shape[2] = 64 nodes
+ * per workgroup (default 32 x 2 x 1). */
 int mpm_set_tile_shape(mpm_engine *e, const int32_t shape[3]);
 int mpm_get_tile_shape(const mpm_engine *e, int32_t shape[3]);
 /* Performs one step during which the particle-to-grid phase is launched

@@ -88,7 +88,7 @@ def test_autotuned_step_equals_ordinary_step(order, variant):
     a.step(sc.dt, 3)
     b.step(sc.dt, 2)
     shape, ms = b.autotune(reps=2)
-    assert shape[0] * shape[1] * shape[2] == 256 and ms > 0
+    assert shape[0] * shape[1] * shape[2] == 64 and ms > 0
     for name in FIELDS:
         assert np.array_equal(a.get_particle(name), b.get_particle(name)), name
     assert a.monitors() == b.monitors()
