@@ -359,9 +359,43 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(const uint32_t *count, con
     }
 }
 
+/* Ascending sort of N keys held in registers (bitonic network: every index
+ * is a compile-time constant, so the keys never leave registers). */
+template <int N>
+__device__ __forceinline__ void bitonic_sort(uint32_t (&v)[N]) {
+#pragma unroll
+    for (int size = 2; size <= N; size <<= 1)
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    const uint32_t lo = min(v[i], v[j]), hi = max(v[i], v[j]);
+                    v[i] = up ? lo : hi;
+                    v[j] = up ? hi : lo;
+                }
+            }
+}
+
+/* The ids of occupied cell t (rank) in ascending order.  Up to SORT_REG ids
+ * are loaded together, sorted in registers and stored back; a larger cell
+ * takes an insertion sort in place. */
+#define SORT_REG 16
 __device__ __forceinline__ void sort_cell(const BView &B, uint32_t t) {
-    uint32_t s = B.ostart[t], e = B.ostart[t + 1];
+    const uint32_t s = B.ostart[t], e = B.ostart[t + 1];
     if (e - s < 2) return;
+    if (e - s <= SORT_REG) {
+        uint32_t v[SORT_REG];
+#pragma unroll
+        for (int k = 0; k < SORT_REG; k++) v[k] = s + k < e ? B.sorted[s + k] : MPM_NONE;
+        bitonic_sort(v); /* MPM_NONE (no particle id) sorts last */
+#pragma unroll
+        for (int k = 0; k < SORT_REG; k++)
+            if (s + k < e) B.sorted[s + k] = v[k];
+        return;
+    }
     for (uint32_t i = s + 1; i < e; i++) {
         uint32_t v = B.sorted[i];
         uint32_t j = i;
@@ -374,23 +408,43 @@ __device__ __forceinline__ void sort_cell(const BView &B, uint32_t t) {
 }
 
 
-/* Particle ids into their cell's range of the sorted list.  Each run of
- * equal keys in a wave takes consecutive slots in id order with one atomic
- * decrement of the cell's count (which also leaves the counts at zero for
- * the next step); k_cell_sort orders what different runs put in one cell. */
+/* Particle ids into their cell's range of the sorted list.  A wave takes
+ * SCATTER_RUNS groups of 64 consecutive particles; in each, every run of
+ * equal keys takes consecutive slots in id order with one atomic decrement
+ * of the cell's count (which also leaves the counts at zero for the next
+ * step).  The wave's key loads, then its atomics and start loads, are all
+ * issued before any result is used, so their latencies overlap (one group
+ * per wave waited for each in turn).  k_cell_sort orders what different
+ * runs put in one cell. */
+#define SCATTER_RUNS 4
 __global__ __launch_bounds__(BLOCK) void k_scatter(PView P, BView B) {
-    uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-    const bool live = i < P.n;
-    const uint32_t k = live ? B.key[i] : MPM_NONE;
-    const bool want = k != MPM_NONE;
     const int lane = (int)(threadIdx.x & 63);
-    const KeyRun r = wave_key_run(k);
-    uint32_t top = 0;
-    if (want && lane == r.head) top = atomicSub(&B.count[k], r.len);
-    top = __shfl(top, r.head, 64);
-    if (!want) return;
-    B.sorted[B.start[k] + top - r.len + (uint32_t)(lane - r.head)] = i;
-    if (P.dc_rank[i] >= 0) atomicOr(&B.slot[k], SLOT_DC); /* the cell holds a controlled particle */
+    const uint32_t w = xcd_remap(blockIdx.x, gridDim.x) * (BLOCK / 64) + (threadIdx.x >> 6);
+    const uint32_t base = w * 64u * SCATTER_RUNS;
+    uint32_t id[SCATTER_RUNS], k[SCATTER_RUNS], top[SCATTER_RUNS], st[SCATTER_RUNS];
+    int32_t dcr[SCATTER_RUNS];
+#pragma unroll
+    for (int u = 0; u < SCATTER_RUNS; u++) {
+        id[u] = base + (uint32_t)(u * 64 + lane);
+        const bool live = id[u] < P.n;
+        k[u] = live ? B.key[id[u]] : MPM_NONE;
+        dcr[u] = live ? P.dc_rank[id[u]] : -1;
+    }
+    KeyRun r[SCATTER_RUNS];
+#pragma unroll
+    for (int u = 0; u < SCATTER_RUNS; u++) {
+        r[u] = wave_key_run(k[u]);
+        top[u] = 0;
+        if (k[u] != MPM_NONE && lane == r[u].head) top[u] = atomicSub(&B.count[k[u]], r[u].len);
+        st[u] = k[u] != MPM_NONE ? B.start[k[u]] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < SCATTER_RUNS; u++) {
+        top[u] = __shfl(top[u], r[u].head, 64);
+        if (k[u] == MPM_NONE) continue;
+        B.sorted[st[u] + top[u] - r[u].len + (uint32_t)(lane - r[u].head)] = id[u];
+        if (dcr[u] >= 0) atomicOr(&B.slot[k[u]], SLOT_DC); /* the cell holds a controlled particle */
+    }
 }
 
 /* Ascending ids inside each occupied cell: the order every node sum uses. */
@@ -3198,7 +3252,8 @@ hipError_t launch_bin(const PView &P, const BView &B, uint64_t *block_sum, hipSt
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(BLOCK), 0, s, block_sum, nb);
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(BLOCK), 0, s, B.count, block_sum, B.start, B.slot,
                        B.occ, B.ostart, B.nocc);
-    if (P.n > 0) hipLaunchKernelGGL(k_scatter, dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s, P, B);
+    if (P.n > 0)
+        hipLaunchKernelGGL(k_scatter, dim3(blocks_for(P.n, (int64_t)BLOCK * SCATTER_RUNS)), dim3(BLOCK), 0, s, P, B);
     hipLaunchKernelGGL(k_cell_sort, dim3((unsigned)std::min<int64_t>(blocks_for(std::max<int64_t>(P.n, 1), BLOCK), 4096)),
                        dim3(BLOCK), 0, s, B);
     return hipGetLastError();
