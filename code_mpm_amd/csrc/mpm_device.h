@@ -38,6 +38,25 @@ struct MatTable {
     double c1[MPM_MAX_MATERIALS];
 };
 
+/* Quotient a / b, correctly rounded, from y = RN(1 / b) (one IEEE division
+ * per divisor instead of one per quotient).  q0 = RN(a y) is within 1.5 ulp
+ * of a / b; the first residual correction q1 = RN(q0 + RN(a - b q0) y)
+ * brings it within one ulp (the error of the correction is below 2^-50 ulp);
+ * for a faithful q1 and y within half an ulp of 1 / b, r = a - b q1 is exact
+ * in one FMA and RN(q1 + r y) is RN(a / b) (Markstein's theorem,
+ * P. Markstein, IBM J. Res. Dev. 34(1), 1990; Muller et al., Handbook of
+ * Floating-Point Arithmetic, division by FMA).  copysign restores the sign
+ * of a zero quotient.  Exact when b > 0 with 2^-100 <= b <= 2^100, and
+ * a = +-0 or 2^-860 <= |a| <= 2^860 (no quotient, residual or product
+ * leaves the normal range); callers keep to that. */
+__host__ __device__ __forceinline__ double div_rcp(double a, double b, double y) {
+    double q = a * y;
+    double r = fma(-q, b, a);
+    q = fma(r, y, q);
+    r = fma(-q, b, a);
+    return copysign(fma(r, y, q), a);
+}
+
 /* Node position along one axis: (i * L) / cells, GridPoint_Factory.h:53-59. */
 __device__ __forceinline__ double node_coord(int i, double L, int cells) {
     return (double)i * L / (double)cells;

@@ -997,7 +997,7 @@ __device__ __forceinline__ void particle_factors(const GridGeom &g, const double
  * cell's record holds the pair).  The halo cell writes nothing (the next
  * group owns it). */
 template <bool REMAP, bool MULTI, bool FEXT = false, bool LIST = false, bool PAIR = false>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(REMAP && PAIR ? 6 : 1))) void k_cell_partials(Batch bt, PView P, NView Nd, BView B, CellParts C) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(REMAP ? 6 : 1))) void k_cell_partials(Batch bt, PView P, NView Nd, BView B, CellParts C) {
     /* staged per particle: m v (3) and either the shape factors s (6, remap)
      * or x (3, P2G), then for P2G m, V, sigma (6) and, with FEXT, f_ext (3);
      * remap rows padded by one double so that the corner-dependent rows of
@@ -1053,7 +1053,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(REMAP && 
          * then sits in the next lane group) */
         bool xprev = false, xnext = false;
         uint32_t s0 = 0, s1 = 0;
-        double mn = 0.0, xpair = 0.0, hpair = 1.0, hx_ = 1.0, hy_ = 1.0, hz_ = 1.0;
+        double mn = 0.0, xpair = 0.0, hpair = 1.0, hx_ = 1.0, hy_ = 1.0, hz_ = 1.0, rmn = 0.0;
         bool massive = false;
         /* remap: every lane reads a valid rank, so the loads below are not
          * under a branch and issue together (a lane past the group's end
@@ -1097,6 +1097,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(REMAP && 
                 const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
                 mn = Nd.m[node];
                 massive = mn > I.mass_min;
+                /* the remap's per-pair division by m_i through div_rcp (its
+                 * range: rmn = 0 marks a mass outside it) */
+                rmn = (massive && mn >= 0x1p-100 && mn <= 0x1p100) ? 1.0 / mn : 0.0;
             }
             if (!REMAP && !LIST && have && bt.tflag) bt.tflag[I.tile_off + tile_of_node(bt, g, ix, iy, iz)] = 1;
         }
@@ -1130,9 +1133,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(REMAP && 
 #pragma unroll
                         for (int o = 0; o < 2; o++) sS[2 * a + o][j] = sh[a][o];
                 }
+                /* flags: bit 0 left out of the scatter, bit 1 every component
+                 * of m v within div_rcp's range for the remap (0 or within
+                 * [2^-600, 2^600]: with N 0 or within [2^-159, 1], each
+                 * numerator N m v is 0 or within [2^-759, 2^600]) */
+                bool pv_ok = true;
 #pragma unroll
-                for (int k = 0; k < 3; k++) spv[k][j] = mp * v[k];
-                sdc[j] = (skip_dc && dcr >= 0) ? 1 : 0;
+                for (int k = 0; k < 3; k++) {
+                    spv[k][j] = mp * v[k];
+                    const double am = fabs(spv[k][j]);
+                    pv_ok &= (am >= 0x1p-600 && am <= 0x1p600) || am == 0.0;
+                }
+                sdc[j] = (int8_t)(((skip_dc && dcr >= 0) ? 1 : 0) | (pv_ok ? 2 : 0));
             }
             __syncthreads();
             if (REMAP ? !have : !cell) continue;
@@ -1140,7 +1152,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(REMAP && 
             if (!REMAP) {
                 for (uint32_t q = a; q < b; q++) {
                     const uint32_t j = q - base;
-                    if (sdc[j]) continue; /* SP:116-117, 157-158 */
+                    if (sdc[j] & 1) continue; /* SP:116-117, 157-158 */
                     /* lanes 0..5 of the cell: the axis shape of pair (axis
                      * corner >> 1, offset corner & 1), as axis_shape */
                     if (corner < 6) {
@@ -1204,18 +1216,35 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(REMAP && 
             } else if (massive) {
                 for (uint32_t q = a; q < b; q++) {
                     const uint32_t j = q - base;
-                    if (sdc[j]) continue; /* DPC:358-359, SP:157-158 */
+                    const int8_t fl = sdc[j];
+                    if (fl & 1) continue; /* DPC:358-359, SP:157-158 */
                     const double N = sS[0 + ox][j] * sS[2 + oy][j] * sS[4 + oz][j];
+                    const double a0 = N * spv[0][j], a1 = N * spv[1][j], a2 = N * spv[2][j];
+                    double t0, t1, t2;
+                    /* N (m_p v_p) / m_i (DP:560): the same doubles as the IEEE
+                     * division, which a wave takes when any of its pairs is
+                     * outside div_rcp's range (0.252 -> 0.230 ms on the 8.0 M-
+                     * particle crush; the divisions were 80 % of this loop's
+                     * FP64 instructions) */
+                    if (__any(!((fl & 2) && rmn != 0.0))) {
+                        t0 = a0 / mn;
+                        t1 = a1 / mn;
+                        t2 = a2 / mn;
+                    } else {
+                        t0 = div_rcp(a0, mn, rmn);
+                        t1 = div_rcp(a1, mn, rmn);
+                        t2 = div_rcp(a2, mn, rmn);
+                    }
                     if (LIST) {
                         double *r = C.list + ((size_t)q * 8u + (size_t)corner) * 3u;
-                        r[0] = N * spv[0][j] / mn;
-                        r[1] = N * spv[1][j] / mn;
-                        r[2] = N * spv[2][j] / mn;
+                        r[0] = t0;
+                        r[1] = t1;
+                        r[2] = t2;
                         continue;
                     }
-                    f[0] += N * spv[0][j] / mn;
-                    f[1] += N * spv[1][j] / mn;
-                    f[2] += N * spv[2][j] / mn;
+                    f[0] += t0;
+                    f[1] += t1;
+                    f[2] += t2;
                 }
             }
         }
