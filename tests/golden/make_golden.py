@@ -58,6 +58,20 @@ CASES = {
     "halfring_doublepass_tile_30": ("half_ring_impact", {"h": 4.0e-4, "depth_cells": 4, "foam_cells": 8,
                                                          "impact_speed": 60.0, "steps": 90},
                                     "doublepass", "tile", "small", 30),
+    # the pair order (the default)
+    "bar_doublepass_pair_100": ("elastic_bar", {}, "doublepass", "pair", "small", 100),
+    "block8_doublepass_pair_12": ("random_block", {"seed": 8, "cells": [20, 18, 10], "n": 3000}, "doublepass",
+                                  "pair", "small", 12),
+    "block9_contact_pair_12": ("random_block", {"seed": 9, "cells": [18, 12, 9], "n": 2500}, "doublepass_contact",
+                               "pair", "small", 12),
+    "block10_singlepass_pair_12": ("random_block", {"seed": 10, "cells": [12, 20, 9], "n": 2500}, "singlepass",
+                                   "pair", "small", 12),
+    "block11_large_pair_10": ("random_block_fe", {"seed": 11}, "doublepass", "pair", "large", 10),
+    "honeycomb64_doublepass_pair_60": ("honeycomb_crush", {"cells": [64, 64, 8], "steps": 300}, "doublepass", "pair",
+                                       "small", 60),
+    "halfring_doublepass_pair_30": ("half_ring_impact", {"h": 4.0e-4, "depth_cells": 4, "foam_cells": 8,
+                                                         "impact_speed": 60.0, "steps": 90},
+                                    "doublepass", "pair", "small", 30),
 }
 
 

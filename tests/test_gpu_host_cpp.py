@@ -1,8 +1,8 @@
 """The C++ host classes on the GPU: the bar driver (PhysicsEngine::
 initializeWorld_Classic_Bar, then runSimulation_Classic_DoublePass_MPLocks
 over 250 steps' worth of time and getMaterialPoints) against the oracle
-running the same world over the same request, bit for bit, in both
-summation orders; the console reports fire at the reference's interval.
+running the same world over the same request, bit for bit, in the
+particle, cell and pair summation orders; the console reports fire at the reference's interval.
 The per-point ConstitutiveRelation calls of the C++ layer against the
 oracle's return maps."""
 import os
@@ -17,14 +17,15 @@ from tests.test_host_cpp import CHECK, DEMO, read_dump
 pytestmark = pytest.mark.gpu
 
 STEPS = 250
+ORDER_CODE = {"particle": 0, "cell": 1, "tile": 2, "pair": 3}      # MPM_ORDER_* of mpm.h
 
 
-@pytest.mark.parametrize("order", ["particle", "cell"])
+@pytest.mark.parametrize("order", ["particle", "cell", "pair"])
 def test_cpp_driver_matches_oracle(order, tmp_path):
     from code_mpm_amd import scenes
     from oracle.oracle import OracleEngine
     path = str(tmp_path / f"bar_{order}.bin")
-    r = subprocess.run([DEMO, str(STEPS), path, "0" if order == "particle" else "1"], capture_output=True, text=True,
+    r = subprocess.run([DEMO, str(STEPS), path, str(ORDER_CODE[order])], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stderr
     got, reports = read_dump(path)
