@@ -49,15 +49,15 @@ FUSED_G2P = {"g2p_kinematics": (26 * 8 + 5 + 30 * 8, 7 * 8)}
 # kernel of each slot, per summation order (names as rocprofv3 reports them)
 KERNELS = {
     "cell": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
-             "p2g_partials": "k_cell_partials<false, false, false, false, false>",
+             "p2g_partials": "k_cell_partials<false, false, false, false>",
              "p2g_nodes": "k_node_cells<false, false, false>", "g2p_velocity": "k_g2p_velocity<false>",
-             "remap_partials": "k_cell_partials<true, false, false, false, false>",
+             "remap_partials": "k_cell_partials<true, false, false, false>",
              "remap_nodes": "k_node_cells<true, false, false>", "g2p_kinematics": "k_g2p_fused<0, false>",
              "g2p_constitutive": None},
     "pair": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
-             "p2g_partials": "k_cell_partials<false, false, false, false, true>",
+             "p2g_partials": "k_cell_partials<false, false, false, true>",
              "p2g_nodes": "k_node_cells<false, false, true>", "g2p_velocity": "k_g2p_velocity<false>",
-             "remap_partials": "k_cell_partials<true, false, false, false, true>",
+             "remap_partials": "k_cell_partials<true, false, false, true>",
              "remap_nodes": "k_node_cells<true, false, true>", "g2p_kinematics": "k_g2p_fused<0, false>",
              "g2p_constitutive": None},
     "tile": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
@@ -66,9 +66,9 @@ KERNELS = {
              "remap_nodes": "k_tile_bnd<true, false>", "g2p_kinematics": "k_g2p_fused<0, false>",
              "g2p_constitutive": None},
     "particle": {"bin": "k_scan_reduce+k_scan_blocks+k_scan_apply+k_scatter+k_cell_sort",
-                 "p2g_partials": "k_cell_partials<false, false, false, true, false>",
+                 "p2g_partials": "k_list_terms<false, false, false>",
                  "p2g_nodes": "k_p2g_list<false>", "g2p_velocity": "k_g2p_velocity<false>",
-                 "remap_partials": "k_cell_partials<true, false, false, true, false>", "remap_nodes": "k_remap_list",
+                 "remap_partials": "k_list_terms<true, false, false>", "remap_nodes": "k_remap_list",
                  "g2p_kinematics": "k_g2p_fused<0, false>", "g2p_constitutive": None},
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_honeycomb8m.txt")

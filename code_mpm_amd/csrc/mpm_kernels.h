@@ -126,8 +126,8 @@ enum { STRAIN_SMALL = 0, STRAIN_LARGE = 1 };
 struct CellParts {
     double *w;
     int64_t cap; /* occupied-cell capacity */
-    /* particle order: per (sorted position, corner) the P2G terms of that
-     * particle at that corner node (7 doubles, 10 with f_ext) */
+    /* particle order: each binned particle's terms at its corner nodes
+     * (P2G 7 doubles, 10 with f_ext; remap 3), cell-major (list_term) */
     double *list;
     int64_t lcap; /* particle capacity */
 };

@@ -454,21 +454,35 @@ __global__ __launch_bounds__(BLOCK) void k_cell_sort(BView B) {
 }
 /* ---- 8-way merge of a node's adjacent cell lists in ascending id order ---- */
 
+/* s / p / e: the start, next and end position of each cell's list in the
+ * sorted ids, h: the id at p (MPM_NONE past the end); dc: bit k set when
+ * cell k holds a displacement-controlled particle (its slot's SLOT_DC
+ * flag). */
 struct Merge8 {
-    uint32_t p[8], e[8], h[8];
+    uint32_t s[8], p[8], e[8], h[8];
+    uint32_t dc;
 };
 
+/* Every list's bounds and slot in one round of loads, then every head in a
+ * second (a round per cell would chain 8 waits). */
 __device__ __forceinline__ void merge_init(Merge8 &M, const BView &B, const GridGeom &g, int64_t cell_off, int ix,
                                            int iy, int iz) {
     /* node i touches cells i-1 and i per axis = ext cells i and i+1 */
+    uint32_t sl[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         int cx = ix + ((k >> 2) & 1), cy = iy + ((k >> 1) & 1), cz = iz + (k & 1);
         uint32_t c = (uint32_t)cell_off +
                      (uint32_t)cx + (uint32_t)g.ec[0] * ((uint32_t)cy + (uint32_t)g.ec[1] * (uint32_t)cz);
-        M.p[k] = B.start[c];
+        M.s[k] = M.p[k] = B.start[c];
         M.e[k] = B.start[c + 1];
+        sl[k] = B.slot[c];
+    }
+    M.dc = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
         M.h[k] = (M.p[k] < M.e[k]) ? B.sorted[M.p[k]] : MPM_NONE;
+        M.dc |= (sl[k] != MPM_NONE && (sl[k] & SLOT_DC)) ? 1u << k : 0u;
     }
 }
 
@@ -478,20 +492,45 @@ __device__ __forceinline__ uint32_t merge_min(const Merge8 &M) {
     return min(a, b);
 }
 
+/* The list holding id (the minimum head): its index k, the id's position
+ * and the list's start and length. */
+__device__ __forceinline__ void merge_find(const Merge8 &M, uint32_t id, uint32_t &k, uint32_t &pos, uint32_t &s0,
+                                           uint32_t &cnt) {
+    k = 0;
+    pos = s0 = cnt = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        if (M.h[j] == id) {
+            k = (uint32_t)j;
+            pos = M.p[j];
+            s0 = M.s[j];
+            cnt = M.e[j] - M.s[j];
+        }
+}
+
+/* Past the head of list k: one load (the list's next id, or again the
+ * current one at the end of the list), not under a branch, so that it
+ * issues with the caller's other loads of the step and they wait together. */
+__device__ __forceinline__ void merge_step(Merge8 &M, const BView &B, uint32_t k, uint32_t pos) {
+    uint32_t e = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        if ((uint32_t)j == k) e = M.e[j];
+    const uint32_t q = pos + 1u;
+    const uint32_t nx = B.sorted[q < e ? q : pos];
+    const uint32_t h = q < e ? nx : MPM_NONE;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        if ((uint32_t)j == k) {
+            M.p[j] = q;
+            M.h[j] = h;
+        }
+}
+
 /* Inverse of node_of_tile: the instance-local tile holding node (ix, iy, iz). */
 __device__ __forceinline__ uint32_t tile_of_node(const Batch &bt, const GridGeom &g, int ix, int iy, int iz) {
     const uint32_t tnx = (uint32_t)((g.nn[0] + bt.tx - 1) / bt.tx), tny = (uint32_t)((g.nn[1] + bt.ty - 1) / bt.ty);
     return (uint32_t)(ix / bt.tx) + tnx * ((uint32_t)(iy / bt.ty) + tny * (uint32_t)(iz / bt.tz));
-}
-
-__device__ __forceinline__ void merge_advance(Merge8 &M, const BView &B, uint32_t id) {
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        if (M.h[k] == id) {
-            M.p[k]++;
-            M.h[k] = (M.p[k] < M.e[k]) ? B.sorted[M.p[k]] : MPM_NONE;
-        }
-    }
 }
 
 __device__ __forceinline__ void node_of_tile(const Batch &bt, const GridGeom &g, uint32_t tile, int &ix, int &iy,
@@ -979,13 +1018,7 @@ __device__ __forceinline__ void particle_factors(const GridGeom &g, const double
     }
 }
 
-/* LIST (for the particle summation order): each lane stores its cell's
- * particles' terms at its corner node in C.list instead of summing them;
- * k_p2g_list / k_remap_list then add them per node in ascending particle
- * id.  The remap stores terms only at nodes above the mass minimum, the
- * only ones that add them.
- *
- * PAIR (the pair order): a block's groups take 31 occupied cells and the
+/* PAIR (the pair order): a block's groups take 31 occupied cells and the
  * next occupied cell as a halo, so that the +x neighbour of every cell a
  * group owns is in the group when it is occupied.  After the partial sums,
  * the lane of a cell's +x corner adds the partial of its +x neighbour's
@@ -996,7 +1029,7 @@ __device__ __forceinline__ void particle_factors(const GridGeom &g, const double
  * its own record only when its -x neighbour cell is empty (else that
  * cell's record holds the pair).  The halo cell writes nothing (the next
  * group owns it). */
-template <bool REMAP, bool MULTI, bool FEXT = false, bool LIST = false, bool PAIR = false>
+template <bool REMAP, bool MULTI, bool FEXT = false, bool PAIR = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(REMAP ? 6 : 1))) void k_cell_partials(Batch bt, PView P, NView Nd, BView B, CellParts C) {
     /* staged per particle: m v (3) and either the shape factors s (6, remap)
      * or x (3, P2G), then for P2G m, V, sigma (6) and, with FEXT, f_ext (3);
@@ -1101,7 +1134,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(REMAP ? 6
                  * range: rmn = 0 marks a mass outside it) */
                 rmn = (massive && mn >= 0x1p-100 && mn <= 0x1p100) ? 1.0 / mn : 0.0;
             }
-            if (!REMAP && !LIST && have && bt.tflag) bt.tflag[I.tile_off + tile_of_node(bt, g, ix, iy, iz)] = 1;
+            if (!REMAP && have && bt.tflag) bt.tflag[I.tile_off + tile_of_node(bt, g, ix, iy, iz)] = 1;
         }
         double m = 0.0, p[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0};
         for (uint32_t base = r0; base < r1; base += PS) {
@@ -1180,25 +1213,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(REMAP ? 6
                     const double gy = (by & 1u) ? ((by & 2u) ? -qy : qy) : 0.0;
                     const double gz = (bz & 1u) ? ((bz & 2u) ? -qz : qz) : 0.0;
                     const double N = sx * sy * sz;
-                    if (LIST) {
-                        if (have) {
-                            double *r = C.list + ((size_t)q * 8u + (size_t)corner) * (FEXT ? 10u : 7u);
-                            const double V = sV[j];
-                            r[0] = N * sm[j];
-                            r[1] = N * spv[0][j];
-                            r[2] = N * spv[1][j];
-                            r[3] = N * spv[2][j];
-                            r[4] = -V * (gx * ss[0][j] + gy * ss[3][j] + gz * ss[5][j]);
-                            r[5] = -V * (gy * ss[1][j] + gx * ss[3][j] + gz * ss[4][j]);
-                            r[6] = -V * (gz * ss[2][j] + gx * ss[5][j] + gy * ss[4][j]);
-                            if (fext) {
-                                r[FEXT ? 7 : 0] = N * sf[0][j];
-                                r[FEXT ? 8 : 0] = N * sf[1][j];
-                                r[FEXT ? 9 : 0] = N * sf[2][j];
-                            }
-                        }
-                        continue;
-                    }
                     m += N * sm[j];
                     p[0] += N * spv[0][j];
                     p[1] += N * spv[1][j];
@@ -1234,13 +1248,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(REMAP ? 6
                         t0 = div_rcp(a0, mn, rmn);
                         t1 = div_rcp(a1, mn, rmn);
                         t2 = div_rcp(a2, mn, rmn);
-                    }
-                    if (LIST) {
-                        double *r = C.list + ((size_t)q * 8u + (size_t)corner) * 3u;
-                        r[0] = t0;
-                        r[1] = t1;
-                        r[2] = t2;
-                        continue;
                     }
                     f[0] += t0;
                     f[1] += t1;
@@ -1294,7 +1301,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(REMAP ? 6
                 f[2] = f[2] + nb[NV - 1];
             }
         }
-        if (!have || LIST) continue;
+        if (!have) continue;
         const size_t t = (size_t)corner * (size_t)C.cap + (size_t)slot;
         if (!REMAP) {
             /* one 64-byte record per (cell, corner): m, p, f */
@@ -1318,8 +1325,115 @@ __device__ __forceinline__ void dc_scan_cell(DCWin &w, const PView &P, const BVi
     for (uint32_t q = B.start[c]; q < B.start[c + 1]; q++) dc_consider(w, P, B.sorted[q], all_axes);
 }
 
-/* Node side of the particle order from per-particle terms (k_cell_partials
- * with LIST): the node merges its 8 cells' sorted id lists and adds each
+/* ---- particle order: per-particle terms, then ascending-id node sums ----
+ *
+ * Index of the stored term of the particle at sorted position q of a cell
+ * whose ids occupy sorted positions s0 .. s0 + cnt - 1, at the cell's
+ * corner node `corner` (k_list_terms writes, k_p2g_list / k_remap_list
+ * read).  Cell-major: a cell's 8 cnt terms together, corner by corner, its
+ * particles in ascending id within a corner.  A node then reads the
+ * particles of one cell at one corner from consecutive terms, and the
+ * threads of one cell's particles store one corner's terms consecutively. */
+__device__ __forceinline__ size_t list_term(uint32_t s0, uint32_t cnt, uint32_t corner, uint32_t q) {
+    return (size_t)s0 * 8u + (size_t)corner * (size_t)cnt + (size_t)(q - s0);
+}
+
+/* Particle side: one thread per binned particle (by sorted position) forms
+ * its terms at its 8 corner nodes with the expressions of the grouped
+ * orders' partial sums: P2G m N, N m v and the internal force
+ * -V (sigma . grad N) (+ N f_ext), DP:48-89; remap N (m v) / m_i at nodes
+ * above the mass minimum, DP:539-564.  Terms go only to corner nodes inside
+ * the grid.  A particle the pass leaves out of the scatter (SP:116-117,
+ * 157-158; DPC:358-359) stores none: the node side skips it too. */
+template <bool REMAP, bool MULTI, bool FEXT>
+__global__ __launch_bounds__(BLOCK) void k_list_terms(Batch bt, PView P, NView Nd, BView B, CellParts C) {
+    const size_t n = P.n;
+    const uint32_t q = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (q >= B.ostart[*B.nocc]) return; /* past the binned particles */
+    const uint32_t id = B.sorted[q], c = B.key[id];
+    const uint32_t s0 = B.start[c], cnt = B.start[c + 1] - s0;
+    const bool skip_dc = REMAP ? bt.variant != VAR_DP : bt.variant == VAR_SP;
+    if (skip_dc && P.dc_rank[id] >= 0) return;
+    const Inst &I = inst_p<MULTI>(bt, id);
+    const GridGeom &g = I.g;
+    const uint32_t e = c - (uint32_t)I.cell_off;
+    const int ec[3] = {(int)(e % (uint32_t)g.ec[0]), (int)((e / (uint32_t)g.ec[0]) % (uint32_t)g.ec[1]),
+                       (int)(e / ((uint32_t)g.ec[0] * (uint32_t)g.ec[1]))};
+    const double x[3] = {P.x[id], P.x[n + id], P.x[2 * n + id]};
+    const double mp = P.m[id];
+    double pv[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) pv[a] = mp * P.v[a * n + id];
+    constexpr size_t RS = REMAP ? 3u : (FEXT ? 10u : 7u);
+    if (REMAP) {
+        double s[3][2];
+        particle_factors(g, x, s);
+#pragma unroll
+        for (int corner = 0; corner < 8; corner++) {
+            const int ox = corner >> 2, oy = (corner >> 1) & 1, oz = corner & 1;
+            const int ix = ec[0] - 1 + ox, iy = ec[1] - 1 + oy, iz = ec[2] - 1 + oz;
+            if ((ix < 0) | (iy < 0) | (iz < 0) | (ix >= g.nn[0]) | (iy >= g.nn[1]) | (iz >= g.nn[2])) continue;
+            const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
+            const double mn = Nd.m[node];
+            if (!(mn > I.mass_min)) continue;
+            const double N = s[0][ox] * s[1][oy] * s[2][oz];
+            double *r = C.list + list_term(s0, cnt, (uint32_t)corner, q) * RS;
+            r[0] = N * pv[0] / mn;
+            r[1] = N * pv[1] / mn;
+            r[2] = N * pv[2] / mn;
+        }
+        return;
+    }
+    /* the six axis shapes (GridPoint_Mediator.cpp:95-144) and the twelve
+     * gradient quotients (s_j s_l) / h_k; shape_full's gradient
+     * (-s_j s_l sgn) / h is exactly -sgn ((s_j s_l) / h) */
+    double s[3][2];
+    bool gr[3][2], neg[3][2];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int o = 0; o < 2; o++) {
+            const AxisShape A = axis_shape(x[a], node_coord(ec[a] - 1 + o, g.L[a], g.cells[a]), g.h[a]);
+            s[a][o] = A.s;
+            gr[a][o] = A.grad;
+            neg[a][o] = A.sgn > 0.0;
+        }
+    const double V = P.V[id];
+    double sig[6], fx[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 6; k++) sig[k] = P.sig[k * n + id];
+    if (FEXT) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) fx[a] = P.fext[a * n + id];
+    }
+#pragma unroll
+    for (int corner = 0; corner < 8; corner++) {
+        const int ox = corner >> 2, oy = (corner >> 1) & 1, oz = corner & 1;
+        const int ix = ec[0] - 1 + ox, iy = ec[1] - 1 + oy, iz = ec[2] - 1 + oz;
+        if ((ix < 0) | (iy < 0) | (iz < 0) | (ix >= g.nn[0]) | (iy >= g.nn[1]) | (iz >= g.nn[2])) continue;
+        const double sx = s[0][ox], sy = s[1][oy], sz = s[2][oz];
+        const double qx = (sy * sz) / g.h[0], qy = (sx * sz) / g.h[1], qz = (sx * sy) / g.h[2];
+        const double gx = gr[0][ox] ? (neg[0][ox] ? -qx : qx) : 0.0;
+        const double gy = gr[1][oy] ? (neg[1][oy] ? -qy : qy) : 0.0;
+        const double gz = gr[2][oz] ? (neg[2][oz] ? -qz : qz) : 0.0;
+        const double N = sx * sy * sz;
+        double *r = C.list + list_term(s0, cnt, (uint32_t)corner, q) * RS;
+        r[0] = N * mp;
+        r[1] = N * pv[0];
+        r[2] = N * pv[1];
+        r[3] = N * pv[2];
+        r[4] = -V * (gx * sig[0] + gy * sig[3] + gz * sig[5]);
+        r[5] = -V * (gy * sig[1] + gx * sig[3] + gz * sig[4]);
+        r[6] = -V * (gz * sig[2] + gx * sig[5] + gy * sig[4]);
+        if (FEXT) {
+            r[FEXT ? 7 : 0] = N * fx[0];
+            r[FEXT ? 8 : 0] = N * fx[1];
+            r[FEXT ? 9 : 0] = N * fx[2];
+        }
+    }
+}
+
+/* Node side: the node merges its 8 cells' sorted id lists and adds each
  * particle's stored terms at this node (its corner 7 - k in cell k) in
  * ascending id, the order of the reference's one-thread M2G loop
  * (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:48-89). */
@@ -1347,31 +1461,36 @@ __global__ __launch_bounds__(NODE_BLOCK) void k_p2g_list(Batch bt, PView P, NVie
     const bool all_axes = bt.variant != VAR_DP, skip_dc = bt.variant == VAR_SP;
     const bool fext = FEXT && I.has_fext;
     constexpr size_t RS = FEXT ? 10u : 7u;
+    /* one round of loads per particle: the list's next id and this
+     * particle's terms issue together; only a cell holding a controlled
+     * particle reads per-particle control data */
     while (id != MPM_NONE) {
-        uint32_t pos = 0, corner = 0;
+        uint32_t k, pos, s0, cnt;
+        merge_find(M, id, k, pos, s0, cnt);
+        merge_step(M, B, k, pos);
+        const double *r = C.list + list_term(s0, cnt, 7u - k, pos) * RS;
+        double t[RS];
 #pragma unroll
-        for (int k = 0; k < 8; k++)
-            if (M.h[k] == id) {
-                pos = M.p[k];
-                corner = 7u - (uint32_t)k;
-            }
-        if (!(skip_dc && P.dc_rank[id] >= 0)) { /* SP:116-117, 157-158 */
-            const double *r = C.list + ((size_t)pos * 8u + (size_t)corner) * RS;
-            m += r[0];
-            p[0] += r[1];
-            p[1] += r[2];
-            p[2] += r[3];
-            f[0] += r[4];
-            f[1] += r[5];
-            f[2] += r[6];
+        for (int j = 0; j < (int)RS; j++) t[j] = (j < 7 || fext) ? r[j] : 0.0;
+        bool add = true;
+        if ((M.dc >> k) & 1u) {
+            add = !(skip_dc && P.dc_rank[id] >= 0); /* SP:116-117, 157-158 */
+            dc_consider(w, P, id, all_axes);
+        }
+        if (add) {
+            m += t[0];
+            p[0] += t[1];
+            p[1] += t[2];
+            p[2] += t[3];
+            f[0] += t[4];
+            f[1] += t[5];
+            f[2] += t[6];
             if (fext) {
-                f[0] += r[FEXT ? 7 : 0];
-                f[1] += r[FEXT ? 8 : 0];
-                f[2] += r[FEXT ? 9 : 0];
+                f[0] += t[FEXT ? 7 : 0];
+                f[1] += t[FEXT ? 8 : 0];
+                f[2] += t[FEXT ? 9 : 0];
             }
         }
-        dc_consider(w, P, id, all_axes);
-        merge_advance(M, B, id);
         id = merge_min(M);
     }
     node_p2g_finish(bt, in, Nd, node, m, p, f, w);
@@ -1398,21 +1517,26 @@ __global__ __launch_bounds__(NODE_BLOCK) void k_remap_list(Batch bt, PView P, NV
     w.v[0] = w.v[1] = w.v[2] = 0.0;
     const bool all_axes = bt.variant != VAR_DP, skip_dc = bt.variant != VAR_DP;
     for (uint32_t id = merge_min(M); id != MPM_NONE; id = merge_min(M)) {
-        uint32_t pos = 0, corner = 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-            if (M.h[k] == id) {
-                pos = M.p[k];
-                corner = 7u - (uint32_t)k;
-            }
-        if (massive && !(skip_dc && P.dc_rank[id] >= 0)) { /* DPC:358-359, SP:157-158 */
-            const double *r = C.list + ((size_t)pos * 8u + (size_t)corner) * 3u;
-            v[0] += r[0];
-            v[1] += r[1];
-            v[2] += r[2];
+        uint32_t k, pos, s0, cnt;
+        merge_find(M, id, k, pos, s0, cnt);
+        merge_step(M, B, k, pos);
+        double t[3] = {0.0, 0.0, 0.0};
+        if (massive) {
+            const double *r = C.list + list_term(s0, cnt, 7u - k, pos) * 3u;
+            t[0] = r[0];
+            t[1] = r[1];
+            t[2] = r[2];
         }
-        dc_consider(w, P, id, all_axes);
-        merge_advance(M, B, id);
+        bool add = massive;
+        if ((M.dc >> k) & 1u) {
+            add = add && !(skip_dc && P.dc_rank[id] >= 0); /* DPC:358-359, SP:157-158 */
+            dc_consider(w, P, id, all_axes);
+        }
+        if (add) {
+            v[0] += t[0];
+            v[1] += t[1];
+            v[2] += t[2];
+        }
     }
     node_remap_dispatch(bt, in, Nd, node, v, w);
 }
@@ -3384,10 +3508,10 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
         } else if (order == ORDER_PAIR) {
             if (P.n > 0) {
                 if (bt.fext)
-                    LAUNCH_MULTI(bt, (k_cell_partials<false, M, true, false, true>), dim3(partial_grid(P.n)),
+                    LAUNCH_MULTI(bt, (k_cell_partials<false, M, true, true>), dim3(partial_grid(P.n)),
                                  dim3(BLOCK), 0, s, bt, P, N, B, C);
                 else
-                    LAUNCH_MULTI(bt, (k_cell_partials<false, M, false, false, true>), dim3(partial_grid(P.n)),
+                    LAUNCH_MULTI(bt, (k_cell_partials<false, M, false, true>), dim3(partial_grid(P.n)),
                                  dim3(BLOCK), 0, s, bt, P, N, B, C);
             }
             if (mid) (void)hipEventRecord(mid, s);
@@ -3396,11 +3520,11 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
             if (!C.list) return hipErrorInvalidValue;
             if (P.n > 0) {
                 if (bt.fext)
-                    LAUNCH_MULTI(bt, (k_cell_partials<false, M, true, true>), dim3(partial_grid(P.n)), dim3(BLOCK), 0,
-                                 s, bt, P, N, B, C);
+                    LAUNCH_MULTI(bt, (k_list_terms<false, M, true>), dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s,
+                                 bt, P, N, B, C);
                 else
-                    LAUNCH_MULTI(bt, (k_cell_partials<false, M, false, true>), dim3(partial_grid(P.n)), dim3(BLOCK),
-                                 0, s, bt, P, N, B, C);
+                    LAUNCH_MULTI(bt, (k_list_terms<false, M, false>), dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s,
+                                 bt, P, N, B, C);
             }
             if (mid) (void)hipEventRecord(mid, s);
             if (bt.fext)
@@ -3434,7 +3558,7 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
             LAUNCH_MULTI(bt, (k_node_cells<true, M>), dim3(bt.ntiles), dim3(NODE_BLOCK), 0, s, bt, P, N, B, C);
         } else if (order == ORDER_PAIR) {
             if (P.n > 0) {
-                LAUNCH_MULTI(bt, (k_cell_partials<true, M, false, false, true>), dim3(partial_grid(P.n)),
+                LAUNCH_MULTI(bt, (k_cell_partials<true, M, false, true>), dim3(partial_grid(P.n)),
                              dim3(BLOCK), 0, s, bt, P, N, B, C);
             }
             if (mid) (void)hipEventRecord(mid, s);
@@ -3442,7 +3566,7 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
         } else {
             if (!C.list) return hipErrorInvalidValue;
             if (P.n > 0)
-                LAUNCH_MULTI(bt, (k_cell_partials<true, M, false, true>), dim3(partial_grid(P.n)), dim3(BLOCK), 0, s,
+                LAUNCH_MULTI(bt, (k_list_terms<true, M, false>), dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s,
                              bt, P, N, B, C);
             if (mid) (void)hipEventRecord(mid, s);
             hipLaunchKernelGGL(k_remap_list, dim3(bt.ntiles), dim3(NODE_BLOCK), 0, s, bt, P, N, B, C);
