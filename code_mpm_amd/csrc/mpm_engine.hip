@@ -254,8 +254,9 @@ struct mpm_engine {
         drop_graphs();
         return MPM_OK;
     }
-    /* Particle order: the per-(particle, corner) P2G terms, 10 doubles each
-     * (7 without f_ext; sized for the larger). */
+    /* Particle order: the per-(particle, corner) terms, 4 doubles each in
+     * P2G (3 in the remap), and a P2G row of 8 doubles (4 without f_ext)
+     * per particle: 40 doubles per particle at most. */
     int ensure_list() {
         const int64_t cap = std::max<int64_t>(n, 1);
         if (parts.list && parts.lcap == cap) return MPM_OK;
@@ -263,7 +264,7 @@ struct mpm_engine {
         parts.list = nullptr;
         parts.lcap = 0;
         int rc;
-        if ((rc = hip(hipMalloc(&parts.list, (size_t)cap * 8u * 10u * sizeof(double)), "alloc particle-order terms")))
+        if ((rc = hip(hipMalloc(&parts.list, (size_t)cap * 40u * sizeof(double)), "alloc particle-order terms")))
             return rc;
         parts.lcap = cap;
         drop_graphs();

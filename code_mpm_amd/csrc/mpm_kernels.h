@@ -126,8 +126,9 @@ enum { STRAIN_SMALL = 0, STRAIN_LARGE = 1 };
 struct CellParts {
     double *w;
     int64_t cap; /* occupied-cell capacity */
-    /* particle order: each binned particle's terms at its corner nodes
-     * (P2G 7 doubles, 10 with f_ext; remap 3), cell-major (list_term) */
+    /* particle order: each binned particle's terms at its corner nodes,
+     * cell-major (list_term), P2G also a row per particle: at most 40
+     * doubles per particle */
     double *list;
     int64_t lcap; /* particle capacity */
 };

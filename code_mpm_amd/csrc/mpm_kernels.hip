@@ -1333,15 +1333,28 @@ __device__ __forceinline__ void dc_scan_cell(DCWin &w, const PView &P, const BVi
  * read).  Cell-major: a cell's 8 cnt terms together, corner by corner, its
  * particles in ascending id within a corner.  A node then reads the
  * particles of one cell at one corner from consecutive terms, and the
- * threads of one cell's particles store one corner's terms consecutively. */
+ * threads of one cell's particles store one corner's terms consecutively.
+ *
+ * P2G term (4 doubles, 32 bytes): N and the internal force -V (sigma .
+ * grad N), whose expression needs the particle's stress; N m and N m v
+ * (and N f_ext) are formed by the node from N and the particle's row (4
+ * doubles m, m v; 8 with f_ext: 3 more and a pad), one per sorted
+ * position after the 8 lcap terms, so that a particle's mass and momentum
+ * are stored once and not per corner.  Remap term (3 doubles): N (m v) / m_i
+ * as the reference adds it. */
 __device__ __forceinline__ size_t list_term(uint32_t s0, uint32_t cnt, uint32_t corner, uint32_t q) {
     return (size_t)s0 * 8u + (size_t)corner * (size_t)cnt + (size_t)(q - s0);
+}
+template <bool FEXT>
+__device__ __forceinline__ double *list_row(const CellParts &C, uint32_t q) {
+    return C.list + (size_t)C.lcap * 32u + (size_t)q * (FEXT ? 8u : 4u);
 }
 
 /* Particle side: one thread per binned particle (by sorted position) forms
  * its terms at its 8 corner nodes with the expressions of the grouped
- * orders' partial sums: P2G m N, N m v and the internal force
- * -V (sigma . grad N) (+ N f_ext), DP:48-89; remap N (m v) / m_i at nodes
+ * orders' partial sums: P2G N and the internal force -V (sigma . grad N)
+ * per corner and the particle's row (m, m v, f_ext), from which the node
+ * forms m N, N m v and N f_ext, DP:48-89; remap N (m v) / m_i at nodes
  * above the mass minimum, DP:539-564.  Terms go only to corner nodes inside
  * the grid.  A particle the pass leaves out of the scatter (SP:116-117,
  * 157-158; DPC:358-359) stores none: the node side skips it too. */
@@ -1364,7 +1377,7 @@ __global__ __launch_bounds__(BLOCK) void k_list_terms(Batch bt, PView P, NView N
     double pv[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) pv[a] = mp * P.v[a * n + id];
-    constexpr size_t RS = REMAP ? 3u : (FEXT ? 10u : 7u);
+    constexpr size_t RS = 3u; /* remap term */
     if (REMAP) {
         double s[3][2];
         particle_factors(g, x, s);
@@ -1417,19 +1430,17 @@ __global__ __launch_bounds__(BLOCK) void k_list_terms(Batch bt, PView P, NView N
         const double gy = gr[1][oy] ? (neg[1][oy] ? -qy : qy) : 0.0;
         const double gz = gr[2][oz] ? (neg[2][oz] ? -qz : qz) : 0.0;
         const double N = sx * sy * sz;
-        double *r = C.list + list_term(s0, cnt, (uint32_t)corner, q) * RS;
-        r[0] = N * mp;
-        r[1] = N * pv[0];
-        r[2] = N * pv[1];
-        r[3] = N * pv[2];
-        r[4] = -V * (gx * sig[0] + gy * sig[3] + gz * sig[5]);
-        r[5] = -V * (gy * sig[1] + gx * sig[3] + gz * sig[4]);
-        r[6] = -V * (gz * sig[2] + gx * sig[5] + gy * sig[4]);
-        if (FEXT) {
-            r[FEXT ? 7 : 0] = N * fx[0];
-            r[FEXT ? 8 : 0] = N * fx[1];
-            r[FEXT ? 9 : 0] = N * fx[2];
-        }
+        double2 *r = reinterpret_cast<double2 *>(C.list + list_term(s0, cnt, (uint32_t)corner, q) * 4u);
+        r[0] = make_double2(N, -V * (gx * sig[0] + gy * sig[3] + gz * sig[5]));
+        r[1] = make_double2(-V * (gy * sig[1] + gx * sig[3] + gz * sig[4]), -V * (gz * sig[2] + gx * sig[5] + gy * sig[4]));
+    }
+    /* the particle's row: m, m v (and f_ext) */
+    double2 *t = reinterpret_cast<double2 *>(list_row<FEXT>(C, q));
+    t[0] = make_double2(mp, pv[0]);
+    t[1] = make_double2(pv[1], pv[2]);
+    if (FEXT) {
+        t[FEXT ? 2 : 0] = make_double2(fx[0], fx[1]);
+        t[FEXT ? 3 : 0] = make_double2(fx[2], 0.0);
     }
 }
 
@@ -1460,7 +1471,6 @@ __global__ __launch_bounds__(NODE_BLOCK) void k_p2g_list(Batch bt, PView P, NVie
     w.v[0] = w.v[1] = w.v[2] = 0.0;
     const bool all_axes = bt.variant != VAR_DP, skip_dc = bt.variant == VAR_SP;
     const bool fext = FEXT && I.has_fext;
-    constexpr size_t RS = FEXT ? 10u : 7u;
     /* one round of loads per particle: the list's next id and this
      * particle's terms issue together; only a cell holding a controlled
      * particle reads per-particle control data */
@@ -1468,27 +1478,32 @@ __global__ __launch_bounds__(NODE_BLOCK) void k_p2g_list(Batch bt, PView P, NVie
         uint32_t k, pos, s0, cnt;
         merge_find(M, id, k, pos, s0, cnt);
         merge_step(M, B, k, pos);
-        const double *r = C.list + list_term(s0, cnt, 7u - k, pos) * RS;
-        double t[RS];
-#pragma unroll
-        for (int j = 0; j < (int)RS; j++) t[j] = (j < 7 || fext) ? r[j] : 0.0;
+        const double2 *r = reinterpret_cast<const double2 *>(C.list + list_term(s0, cnt, 7u - k, pos) * 4u);
+        const double2 *pr = reinterpret_cast<const double2 *>(list_row<FEXT>(C, pos));
+        const double2 a0 = r[0], a1 = r[1], b0 = pr[0], b1 = pr[1];
+        const double2 b2 = fext ? pr[FEXT ? 2 : 0] : make_double2(0.0, 0.0);
+        const double2 b3 = fext ? pr[FEXT ? 3 : 0] : make_double2(0.0, 0.0);
         bool add = true;
         if ((M.dc >> k) & 1u) {
             add = !(skip_dc && P.dc_rank[id] >= 0); /* SP:116-117, 157-158 */
             dc_consider(w, P, id, all_axes);
         }
         if (add) {
-            m += t[0];
-            p[0] += t[1];
-            p[1] += t[2];
-            p[2] += t[3];
-            f[0] += t[4];
-            f[1] += t[5];
-            f[2] += t[6];
+            /* the terms k_cell_partials sums: m N and N m v from the
+             * corner's N and the particle's row, the internal force as
+             * stored, N f_ext */
+            const double N = a0.x;
+            m += N * b0.x;
+            p[0] += N * b0.y;
+            p[1] += N * b1.x;
+            p[2] += N * b1.y;
+            f[0] += a0.y;
+            f[1] += a1.x;
+            f[2] += a1.y;
             if (fext) {
-                f[0] += t[FEXT ? 7 : 0];
-                f[1] += t[FEXT ? 8 : 0];
-                f[2] += t[FEXT ? 9 : 0];
+                f[0] += N * b2.x;
+                f[1] += N * b2.y;
+                f[2] += N * b3.x;
             }
         }
         id = merge_min(M);

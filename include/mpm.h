@@ -218,7 +218,7 @@ int mpm_get_instance_monitors(mpm_engine *e, int32_t instance, mpm_monitors *out
  *   ascending particle index, the order of the reference's scatters run by
  *   one thread (PhysicsEngine_runSimulation_Classic_DoublePass_MPLocks.cpp:48-89,
  *   539-564); results equal that run bit for bit.  Steps in this order
- *   hold a device buffer of 640 bytes per particle (each particle's terms
+ *   hold a device buffer of 320 bytes per particle (each particle's terms
  *   at its 8 corner nodes).
  * MPM_ORDER_CELL: each node adds 8 per-cell partial sums in a
  *   fixed neighbour order, each partial taken over the cell's particles in
