@@ -1340,8 +1340,10 @@ __device__ __forceinline__ void dc_scan_cell(DCWin &w, const PView &P, const BVi
  * (and N f_ext) are formed by the node from N and the particle's row (4
  * doubles m, m v; 8 with f_ext: 3 more and a pad), one per sorted
  * position after the 8 lcap terms, so that a particle's mass and momentum
- * are stored once and not per corner.  Remap term (3 doubles): N (m v) / m_i
- * as the reference adds it. */
+ * are stored once and not per corner.  The remap reuses the P2G terms' N
+ * (the same shape values: the particles have not moved, and the binning
+ * cell is the base cell of both) with a row of the particle's updated m v
+ * in place of the P2G row; the node forms N (m v) / m_i itself. */
 __device__ __forceinline__ size_t list_term(uint32_t s0, uint32_t cnt, uint32_t corner, uint32_t q) {
     return (size_t)s0 * 8u + (size_t)corner * (size_t)cnt + (size_t)(q - s0);
 }
@@ -1354,12 +1356,14 @@ __device__ __forceinline__ double *list_row(const CellParts &C, uint32_t q) {
  * its terms at its 8 corner nodes with the expressions of the grouped
  * orders' partial sums: P2G N and the internal force -V (sigma . grad N)
  * per corner and the particle's row (m, m v, f_ext), from which the node
- * forms m N, N m v and N f_ext, DP:48-89; remap N (m v) / m_i at nodes
- * above the mass minimum, DP:539-564.  Terms go only to corner nodes inside
- * the grid.  A particle the pass leaves out of the scatter (SP:116-117,
- * 157-158; DPC:358-359) stores none: the node side skips it too. */
+ * forms m N, N m v and N f_ext, DP:48-89; the remap only the row of m v,
+ * from which the node forms N (m v) / m_i with the P2G N, DP:539-564.
+ * Terms go only to corner nodes inside the grid.  A particle the pass
+ * leaves out of the scatter (SP:116-117, 157-158; DPC:358-359) stores
+ * none: the node side skips it too (a particle the remap includes the
+ * P2G pass includes as well). */
 template <bool REMAP, bool MULTI, bool FEXT>
-__global__ __launch_bounds__(BLOCK) void k_list_terms(Batch bt, PView P, NView Nd, BView B, CellParts C) {
+__global__ __launch_bounds__(BLOCK) void k_list_terms(Batch bt, PView P, BView B, CellParts C) {
     const size_t n = P.n;
     const uint32_t q = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (q >= B.ostart[*B.nocc]) return; /* past the binned particles */
@@ -1367,36 +1371,22 @@ __global__ __launch_bounds__(BLOCK) void k_list_terms(Batch bt, PView P, NView N
     const uint32_t s0 = B.start[c], cnt = B.start[c + 1] - s0;
     const bool skip_dc = REMAP ? bt.variant != VAR_DP : bt.variant == VAR_SP;
     if (skip_dc && P.dc_rank[id] >= 0) return;
+    const double mp = P.m[id];
+    double pv[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) pv[a] = mp * P.v[a * n + id];
+    if (REMAP) {
+        double2 *t = reinterpret_cast<double2 *>(list_row<false>(C, q));
+        t[0] = make_double2(pv[0], pv[1]);
+        t[1] = make_double2(pv[2], 0.0);
+        return;
+    }
     const Inst &I = inst_p<MULTI>(bt, id);
     const GridGeom &g = I.g;
     const uint32_t e = c - (uint32_t)I.cell_off;
     const int ec[3] = {(int)(e % (uint32_t)g.ec[0]), (int)((e / (uint32_t)g.ec[0]) % (uint32_t)g.ec[1]),
                        (int)(e / ((uint32_t)g.ec[0] * (uint32_t)g.ec[1]))};
     const double x[3] = {P.x[id], P.x[n + id], P.x[2 * n + id]};
-    const double mp = P.m[id];
-    double pv[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) pv[a] = mp * P.v[a * n + id];
-    constexpr size_t RS = 3u; /* remap term */
-    if (REMAP) {
-        double s[3][2];
-        particle_factors(g, x, s);
-#pragma unroll
-        for (int corner = 0; corner < 8; corner++) {
-            const int ox = corner >> 2, oy = (corner >> 1) & 1, oz = corner & 1;
-            const int ix = ec[0] - 1 + ox, iy = ec[1] - 1 + oy, iz = ec[2] - 1 + oz;
-            if ((ix < 0) | (iy < 0) | (iz < 0) | (ix >= g.nn[0]) | (iy >= g.nn[1]) | (iz >= g.nn[2])) continue;
-            const int64_t node = I.node_off + (int64_t)ix + (int64_t)g.nn[0] * ((int64_t)iy + (int64_t)g.nn[1] * iz);
-            const double mn = Nd.m[node];
-            if (!(mn > I.mass_min)) continue;
-            const double N = s[0][ox] * s[1][oy] * s[2][oz];
-            double *r = C.list + list_term(s0, cnt, (uint32_t)corner, q) * RS;
-            r[0] = N * pv[0] / mn;
-            r[1] = N * pv[1] / mn;
-            r[2] = N * pv[2] / mn;
-        }
-        return;
-    }
     /* the six axis shapes (GridPoint_Mediator.cpp:95-144) and the twelve
      * gradient quotients (s_j s_l) / h_k; shape_full's gradient
      * (-s_j s_l sgn) / h is exactly -sgn ((s_j s_l) / h) */
@@ -1537,10 +1527,12 @@ __global__ __launch_bounds__(NODE_BLOCK) void k_remap_list(Batch bt, PView P, NV
         merge_step(M, B, k, pos);
         double t[3] = {0.0, 0.0, 0.0};
         if (massive) {
-            const double *r = C.list + list_term(s0, cnt, 7u - k, pos) * 3u;
-            t[0] = r[0];
-            t[1] = r[1];
-            t[2] = r[2];
+            const double N = C.list[list_term(s0, cnt, 7u - k, pos) * 4u];
+            const double2 *pr = reinterpret_cast<const double2 *>(list_row<false>(C, pos));
+            const double2 b0 = pr[0], b1 = pr[1];
+            t[0] = N * b0.x / in.m;
+            t[1] = N * b0.y / in.m;
+            t[2] = N * b1.x / in.m;
         }
         bool add = massive;
         if ((M.dc >> k) & 1u) {
@@ -3536,10 +3528,10 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
             if (P.n > 0) {
                 if (bt.fext)
                     LAUNCH_MULTI(bt, (k_list_terms<false, M, true>), dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s,
-                                 bt, P, N, B, C);
+                                 bt, P, B, C);
                 else
                     LAUNCH_MULTI(bt, (k_list_terms<false, M, false>), dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s,
-                                 bt, P, N, B, C);
+                                 bt, P, B, C);
             }
             if (mid) (void)hipEventRecord(mid, s);
             if (bt.fext)
@@ -3582,7 +3574,7 @@ hipError_t launch_phase(int phase, const Batch &bt, const PView &P, const NView 
             if (!C.list) return hipErrorInvalidValue;
             if (P.n > 0)
                 LAUNCH_MULTI(bt, (k_list_terms<true, M, false>), dim3(blocks_for(P.n, BLOCK)), dim3(BLOCK), 0, s,
-                             bt, P, N, B, C);
+                             bt, P, B, C);
             if (mid) (void)hipEventRecord(mid, s);
             hipLaunchKernelGGL(k_remap_list, dim3(bt.ntiles), dim3(NODE_BLOCK), 0, s, bt, P, N, B, C);
         }
