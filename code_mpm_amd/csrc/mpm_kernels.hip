@@ -1185,7 +1185,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(REMAP ? 6
             if (!REMAP) {
                 for (uint32_t q = a; q < b; q++) {
                     const uint32_t j = q - base;
-                    if (sdc[j] & 1) continue; /* SP:116-117, 157-158 */
+                    /* only the single-pass loop leaves particles out here (the
+                     * flag's read and wait skipped in the others) */
+                    if (skip_dc && (sdc[j] & 1)) continue; /* SP:116-117, 157-158 */
                     /* lanes 0..5 of the cell: the axis shape of pair (axis
                      * corner >> 1, offset corner & 1), as axis_shape */
                     if (corner < 6) {
