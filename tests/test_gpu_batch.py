@@ -42,7 +42,7 @@ def test_batch_instances_match_separate_runs_and_oracle(variant, order):
         assert batch.instance_time(k) == alone.getTime_Current()
 
 
-@pytest.mark.parametrize("order", ["cell", "tile", "pair"])
+@pytest.mark.parametrize("order", ["particle", "cell", "tile", "pair"])
 @pytest.mark.parametrize("variant", ["doublepass", "singlepass"])
 def test_graph_replay_matches_eager(variant, order):
     from code_mpm_amd import Engine, scenes
@@ -59,7 +59,7 @@ def test_graph_replay_matches_eager(variant, order):
         assert eager.monitors() == graph.monitors()
 
 
-@pytest.mark.parametrize("order", ["cell", "tile", "pair"])
+@pytest.mark.parametrize("order", ["particle", "cell", "tile", "pair"])
 def test_batch_graph_replay_matches_eager(order):
     from code_mpm_amd import Engine
     sc = _scenes()
