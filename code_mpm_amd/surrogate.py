@@ -189,13 +189,34 @@ class Surrogate:
         return hist
 
 
+def content_order(rows):
+    """A permutation of `rows` that depends only on their contents: rows
+    sorted by a 64-bit mix of each row's bit patterns.  The capture appends
+    rows with one atomic per block, so the same set arrives in a different
+    order from run to run; in this order it does not (rows with equal bits
+    are interchangeable, and two different rows share a key with
+    probability about k^2 / 2^65)."""
+    b = np.ascontiguousarray(rows, dtype=np.float64).view(np.uint64)
+    h = np.zeros(b.shape[0], dtype=np.uint64)
+    mul, sh = np.uint64(0x9E3779B97F4A7C15), np.uint64(29)
+    for j in range(b.shape[1]):
+        h ^= b[:, j]
+        h *= mul
+        h ^= h >> sh
+    return np.argsort(h, kind="stable")
+
+
 def capture_from_runs(seeds, steps, capacity_per_run, scene_kw=None, device=0, log=None):
     """Return-map samples the exact map itself produces on seeded honeycomb
     crushes (scenes.honeycomb_crush(seed=s, **scene_kw)): each run steps
     with the exact FP64 map and records every yielding point's inputs
     (sigma6, dstrain6, R, material index) through mpm_set_sample_capture,
-    until `capacity_per_run` rows are held or `steps` steps are done.
-    Returns (rows (k, 14), per-run [(seed, steps run, rows kept)])."""
+    until at least `capacity_per_run` rows are held or `steps` steps are
+    done.  The device keeps every row of those steps (its buffer has room
+    for the last chunk of steps too), and the run keeps `capacity_per_run`
+    of them chosen by content_order, so the training set, and its order,
+    is the same in every run.  Returns (rows (k, 14), per-run
+    [(seed, steps run, rows kept)])."""
     from . import scenes
     from .engine import Engine
     out, info = [], []
@@ -203,7 +224,7 @@ def capture_from_runs(seeds, steps, capacity_per_run, scene_kw=None, device=0, l
         sc = scenes.honeycomb_crush(seed=seed, steps=steps, **(scene_kw or {}))
         eng = Engine(device=device, order="pair")
         eng.load_scene(sc)
-        eng.set_sample_capture(capacity_per_run)
+        eng.set_sample_capture(capacity_per_run + capacity_per_run // 2 + 1)
         done, chunk = 0, max(1, steps // 20)
         while done < steps:
             k = min(chunk, steps - done)
@@ -213,6 +234,9 @@ def capture_from_runs(seeds, steps, capacity_per_run, scene_kw=None, device=0, l
                 break
         rows, produced = eng.captured_samples()
         eng.close()
+        if produced > rows.shape[0] and log:
+            log(f"capture seed {seed}: {produced - rows.shape[0]} rows beyond the buffer were dropped")
+        rows = rows[content_order(rows)[:capacity_per_run]]
         out.append(rows)
         info.append((int(seed), int(done), int(rows.shape[0])))
         if log:
