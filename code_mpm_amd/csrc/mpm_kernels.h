@@ -126,8 +126,9 @@ enum { STRAIN_SMALL = 0, STRAIN_LARGE = 1 };
 struct CellParts {
     double *w;
     int64_t cap; /* occupied-cell capacity */
-    /* particle order: each binned particle's terms at its corner nodes,
-     * cell-major (list_term), P2G also a row per particle: at most 40
+    /* particle order: each binned particle's P2G terms at its corner nodes,
+     * cell-major (list_term), then one row per particle (P2G: m, m v,
+     * f_ext; the remap overwrites it with the updated m v): at most 40
      * doubles per particle */
     double *list;
     int64_t lcap; /* particle capacity */
